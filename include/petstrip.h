@@ -1,0 +1,217 @@
+/*
+ * petstrip: MI355X (gfx950) list-mode MLEM for the 2D strip PET scanner.
+ *
+ * C ABI of libpetstrip_hip.so.  It replaces the reference's --gpu entry
+ * point, the C++ template
+ *   PET2D::Strip::GPU::Reconstruction::run<F>(scanner, responses, n,
+ *       input_output, n_iteration_blocks, n_iterations_in_block, output,
+ *       progress, device, n_blocks, n_threads_per_block, device_name)
+ * declared in src/2d/strip/cuda/reconstruction.h:31-44 and defined in
+ * src/2d/strip/cuda/reconstruction.cu:31-113 (grey277/j-pet-mlem).
+ *
+ * Conventions
+ *   - Every call returns 0 on success or a negative PETSTRIP_E* code; the
+ *     message of the last failure on the calling thread is returned by
+ *     petstrip_last_error().  Nothing aborts the process (the reference
+ *     abort()s on any CUDA error, src/util/cuda/debug.h:17-27).
+ *   - Host buffers belong to the caller; device buffers belong to the handle.
+ *   - Images are float32, W = n_z_pixels columns by H = n_y_pixels rows,
+ *     row-major, row 0 on top (y = +H*pixel_height/2), column 0 on the left
+ *     (z = -W*pixel_width/2), as PixelMap and Scanner use them
+ *     (src/2d/geometry/pixel_map.h:15-40, src/2d/strip/scanner.h:61-66).
+ *   - Responses are packed float32 triples (z_u, z_d, dl)
+ *     (src/2d/strip/response.h:13-45).
+ *   - One handle drives one device on one HIP stream; calls on one handle
+ *     must not overlap.  Callbacks run on the calling thread.
+ */
+#ifndef PETSTRIP_H
+#define PETSTRIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PETSTRIP_ABI_VERSION 1
+
+enum {
+  PETSTRIP_OK = 0,
+  PETSTRIP_EINVAL = -1,   /* bad argument (null pointer, bad size, ...) */
+  PETSTRIP_EDEVICE = -2,  /* no such device / HIP runtime failure       */
+  PETSTRIP_ENOMEM = -3,   /* device or host allocation failed           */
+  PETSTRIP_ESTATE = -4,   /* call out of order (e.g. no events loaded)  */
+  PETSTRIP_ECALLBACK = -5 /* a callback asked to stop                   */
+};
+
+/* Scanner geometry, the fields of the Scanner constructor in its order
+ * (src/2d/strip/scanner.h:39-49).  Values are converted to float exactly as
+ * the reference's Scanner<float, short> stores them. */
+typedef struct petstrip_scanner {
+  double radius;              /* R: the strips lie at y = +-R [m]       */
+  double scintillator_length; /* L [m]                                  */
+  int32_t n_y_pixels;         /* H                                      */
+  int32_t n_z_pixels;         /* W                                      */
+  double pixel_height;        /* [m]                                    */
+  double pixel_width;         /* [m]                                    */
+  double sigma_z;             /* TOF sigma along z [m]                  */
+  double sigma_dl;            /* TOF sigma of delta-l [m]               */
+  double center_y;            /* scanner centre, usually 0              */
+  double center_z;
+} petstrip_scanner;
+
+/* How the backprojection is accumulated. */
+enum {
+  PETSTRIP_ACCUM_FP32 = 0,        /* plain float sums                       */
+  PETSTRIP_ACCUM_COMPENSATED = 1, /* compensated (two-sum) float sums       */
+  PETSTRIP_ACCUM_FIXED64 = 2      /* int64 fixed point, 2^-40 units:
+                                     bit-reproducible run to run          */
+};
+
+/* Which per-event update is computed. */
+enum {
+  /* CPU semantics (the parity target): Gaussian kernel, no sensitivity,
+   * rho'[p] = sum_e K_e(p) rho[p] / sum_q K_e(q) rho[q]
+   * (src/2d/strip/reconstruction.h:220-298). */
+  PETSTRIP_UPDATE_CPU = 0,
+  /* The reference --gpu variant: denominator weighted by the 5-point pixel
+   * sensitivity (src/2d/strip/cuda/reconstruction/warp_granularity.cuh:
+   * 101-108,145-150), Gaussian kernel, guarded inverse. */
+  PETSTRIP_UPDATE_SENSITIVITY_DENOMINATOR = 1
+};
+
+typedef struct petstrip_options {
+  int32_t accumulation; /* PETSTRIP_ACCUM_*                                */
+  int32_t update;       /* PETSTRIP_UPDATE_*                               */
+  int32_t reserved[6];  /* zero                                            */
+} petstrip_options;
+
+typedef struct petstrip_handle petstrip_handle;
+
+/* Per-run counters of the last iteration. */
+typedef struct petstrip_stats {
+  int64_t n_events;          /* events on the device                     */
+  int64_t n_box_pixels;      /* sum over events of clamped-box pixels    */
+  int64_t n_ellipse_pixels;  /* sum over events of 3-sigma-ellipse pixels*/
+  int64_t n_zero_denominator;/* events whose denominator was not > 0     */
+  double flop_alg;           /* sum_e 15 N_box + 41 N_ell + 50           */
+  double last_iteration_ms;  /* device time of the last iteration        */
+} petstrip_stats;
+
+/* Called after an output is ready.  iteration >= 1 for images, -1 for the
+ * sensitivity map (as the reference's output delegate,
+ * cuda/reconstruction.cu:63-70,98-110).  image is W*H floats, valid only
+ * during the call.  Return nonzero to stop (run returns
+ * PETSTRIP_ECALLBACK). */
+typedef int (*petstrip_output_cb)(void* user, int iteration,
+                                  const float* image, int32_t width,
+                                  int32_t height);
+/* Progress, as the reference's progress delegate (completed, finished). */
+typedef void (*petstrip_progress_cb)(void* user, int completed, int finished);
+
+/* --- lifetime ---------------------------------------------------------- */
+
+/* Returns PETSTRIP_ABI_VERSION. */
+int petstrip_abi_version(void);
+/* Message of the last failure on this thread ("" if none). */
+const char* petstrip_last_error(void);
+/* Number of visible HIP devices (0 without a GPU; no error). */
+int petstrip_device_count(int* count);
+
+/* Creates a handle on `device` (cudaSetDevice + properties,
+ * cuda/reconstruction.cu:54-57).  opts may be NULL for defaults. */
+int petstrip_create(const petstrip_scanner* scanner, int device,
+                    const petstrip_options* opts, petstrip_handle** out);
+int petstrip_destroy(petstrip_handle* h);
+/* Device marketing name (reference: device_name delegate). */
+int petstrip_device_name(petstrip_handle* h, char* buf, size_t len);
+/* The HIP stream (hipStream_t) the handle launches on. */
+int petstrip_stream(petstrip_handle* h, void** stream);
+int petstrip_set_options(petstrip_handle* h, const petstrip_options* opts);
+
+/* --- data ---------------------------------------------------------------- */
+
+/* Uploads n responses (z_u, z_d, dl) from host memory (replaces the
+ * ResponsesSOA upload, src/2d/strip/cuda/gpu_events_soa.h:15-44) and
+ * prepares per-event geometry.  n may be 0. */
+int petstrip_set_events(petstrip_handle* h, const float* responses,
+                        int64_t n);
+/* Same from device memory already holding n packed triples. */
+int petstrip_set_events_device(petstrip_handle* h,
+                               const float* d_responses, int64_t n);
+/* Sets / reads the current image rho (W*H floats, host memory). */
+int petstrip_set_rho(petstrip_handle* h, const float* rho);
+int petstrip_get_rho(petstrip_handle* h, float* rho);
+/* Sets rho from device memory. */
+int petstrip_set_rho_device(petstrip_handle* h, const float* d_rho);
+/* Device pointer of the current image (valid until the next call). */
+int petstrip_rho_device(petstrip_handle* h, const float** d_rho);
+
+/* --- MLEM ---------------------------------------------------------------- */
+
+/* Runs n_iterations MLEM iterations on the device-resident image and
+ * events, then waits for the device. */
+int petstrip_iterate(petstrip_handle* h, int n_iterations);
+
+/* The reference run(): rho_inout is the start image and receives every
+ * output; images are reported through `output` at iterations 1-5, 10, 15,
+ * 20, 30, 50, 100 of the first block and at the end of every block, and the
+ * 5-point sensitivity map first with iteration -1
+ * (cuda/reconstruction.cu:63-70,79-110).  Callbacks may be NULL. */
+int petstrip_run(petstrip_handle* h, float* rho_inout, int n_iteration_blocks,
+                 int n_iterations_in_block, petstrip_output_cb output,
+                 petstrip_progress_cb progress, void* user);
+
+/* Sensitivity map: mode 0 = Scanner::sensitivity at pixel centres
+ * (src/2d/strip/reconstruction.h:67-73), mode 1 = 5-point
+ * Scanner::pixel_sensitivity (src/2d/strip/scanner.h:174-185). */
+int petstrip_sensitivity(petstrip_handle* h, int mode, float* out);
+
+int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out);
+/* Per-event footprints {box pixels, ellipse pixels} (host arrays of n). */
+int petstrip_footprints(petstrip_handle* h, int32_t* n_box, int32_t* n_ell);
+
+/* --- differentiation ----------------------------------------------------- */
+
+/* Forward projector: lambda[e] = sum_{p in E_e} K_e(p) rho[p] for the
+ * current image (the MLEM denominator, reconstruction.h:264-290).  lambda
+ * is a host array of n floats (may be NULL: then only kept on device). */
+int petstrip_forward(petstrip_handle* h, float* lambda);
+
+/* Adjoint of the forward projector and the sigma derivative.  With upstream
+ * weights g_e (host array of n, or NULL for g_e = 1/lambda_e, the Poisson
+ * list-mode log-likelihood l = sum_e log lambda_e - sum_p s_p rho_p with
+ * s_p = 1):
+ *   grad_rho[p]   = sum_e g_e K_e(p)  - (g == NULL ? 1 : 0)
+ *   grad_sigma[0] = sum_e g_e d lambda_e / d sigma_z
+ *   grad_sigma[1] = sum_e g_e d lambda_e / d sigma_dl
+ * with the support (ellipse and box) held at the current sigma.  Kernels
+ * are recomputed, not stored.  loglik (nullable) receives l.  Uses the
+ * lambda of the last petstrip_forward (computes it if absent). */
+int petstrip_backward(petstrip_handle* h, const float* g, float* grad_rho,
+                      double grad_sigma[2], double* loglik);
+
+/* Changes (sigma_z, sigma_dl) of the kernel.  freeze_support = 1 keeps the
+ * ellipse/box support of the previous sigma (for finite differences). */
+int petstrip_set_sigma(petstrip_handle* h, double sigma_z, double sigma_dl,
+                       int freeze_support);
+
+/* --- event generation (input side) --------------------------------------- */
+
+/* Seeded list-mode events from an elliptical phantom, independent of the
+ * thread count: slot i draws from Philox4x32-10 keyed by (seed, i) until an
+ * emission is detected.  Regions are (x, y, a, b, angle_deg, intensity)
+ * sextuples; non-additive: earlier regions lie on top (the reference
+ * phantom, src/2d/geometry/phantom.h:183-215, detection and noise
+ * src/2d/strip/scanner.h:107-113,195-267).  out receives n triples. */
+int petstrip_phantom_events(const petstrip_scanner* scanner,
+                            const double* regions, int32_t n_regions,
+                            uint64_t seed, int64_t n, float* out,
+                            int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

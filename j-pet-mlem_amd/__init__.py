@@ -1,0 +1,11 @@
+"""petstrip: MI355X-native list-mode MLEM for the 2D strip PET scanner.
+
+The compute path is libpetstrip_hip.so (hand-written HIP for gfx950 behind
+the C ABI in include/petstrip.h); this package is its Python host side.
+"""
+from .petstrip import (  # noqa: F401
+    ACCUM_COMPENSATED, ACCUM_FIXED64, ACCUM_FP32, UPDATE_CPU,
+    UPDATE_SENSITIVITY_DENOMINATOR, PetstripError, Reconstruction, Scanner,
+    device_count, load_library, phantom_events,
+)
+from .workloads import WORKLOADS, workload_scanner, workload_events  # noqa: F401

@@ -1,0 +1,561 @@
+// Host driver and C ABI of libpetstrip_hip.so (declared in include/petstrip.h).
+//
+// Replaces PET2D::Strip::GPU::Reconstruction::run
+// (src/2d/strip/cuda/reconstruction.cu:31-113): owns the device buffers and
+// one HIP stream, uploads the responses once, runs iterations on
+// device-resident data and reports images through callbacks.  Errors are
+// returned as codes with a message (the reference aborts).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/petstrip.h"
+#include "strip_geometry.h"
+#include "strip_kernels.h"
+
+using petstrip::EventGeom;
+using petstrip::StripConst;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess)                                                    \
+      return fail(_e == hipErrorOutOfMemory ? PETSTRIP_ENOMEM               \
+                                            : PETSTRIP_EDEVICE,             \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));        \
+  } while (0)
+
+// Scanner<float, short> and GaussianKernel<float> constants, computed with the
+// reference's float operation order (scanner.h:50-67, gaussian_kernel.h:27-33).
+StripConst make_const(const petstrip_scanner& sc, double sigma_z,
+                      double sigma_dl) {
+  StripConst s;
+  s.R = (float)sc.radius;
+  s.L = (float)sc.scintillator_length;
+  s.ph = (float)sc.pixel_height;
+  s.pw = (float)sc.pixel_width;
+  float cy = (float)sc.center_y, cz = (float)sc.center_z;
+  float size_y = (float)(short)sc.n_y_pixels * s.ph;
+  float size_z = (float)(short)sc.n_z_pixels * s.pw;
+  s.tl_y = cy + size_y / 2;
+  s.tl_z = cz - size_z / 2;
+  s.tl_y_half_h = s.tl_y - s.ph / 2;
+  s.tl_z_half_w = s.tl_z + s.pw / 2;
+  s.half_L = s.L / 2;
+  float sz = (float)sigma_z, sdl = (float)sigma_dl;
+  s.sigma_z = sz;
+  s.sigma_dl = sdl;
+  s.isz = 1 / (sz * sz);
+  s.isdl = 1 / (sdl * sdl);
+  s.c = (float)(1 / (2 * M_PI * M_PI) * (double)sqrtf(s.isz * s.isz * s.isdl));
+  s.W = sc.n_z_pixels;
+  s.H = sc.n_y_pixels;
+  return s;
+}
+
+int check_scanner(const petstrip_scanner* sc) {
+  if (!sc) return fail(PETSTRIP_EINVAL, "scanner is NULL");
+  if (sc->n_y_pixels < 2 || sc->n_z_pixels < 2 || sc->n_y_pixels > 32767 ||
+      sc->n_z_pixels > 32767)
+    return fail(PETSTRIP_EINVAL,
+                "n_y_pixels and n_z_pixels must be in [2, 32767]");
+  if (!(sc->pixel_height > 0) || !(sc->pixel_width > 0))
+    return fail(PETSTRIP_EINVAL, "pixel sizes must be positive");
+  if (!(sc->sigma_z > 0) || !(sc->sigma_dl > 0))
+    return fail(PETSTRIP_EINVAL, "sigma_z and sigma_dl must be positive");
+  if (!(sc->radius > 0) || !(sc->scintillator_length > 0))
+    return fail(PETSTRIP_EINVAL, "radius and length must be positive");
+  return PETSTRIP_OK;
+}
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? PETSTRIP_ENOMEM : PETSTRIP_EDEVICE,
+                std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                    " bytes): " + hipGetErrorString(e));
+  return PETSTRIP_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct petstrip_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  petstrip_scanner scanner{};
+  petstrip_options opts{};
+  StripConst sg{};  // constants the support (ellipse, box) was built with
+  StripConst sk{};  // constants of the kernel value
+  std::string name;
+  size_t n_pix = 0;
+  int64_t n_events = 0;
+  size_t cap_events = 0;
+  float* d_resp = nullptr;
+  EventGeom* d_geom = nullptr;
+  float* d_lambda = nullptr;
+  float* d_inv = nullptr;
+  int32_t* d_nbox = nullptr;
+  int32_t* d_nell = nullptr;
+  float* d_rho = nullptr;
+  float* d_out = nullptr;
+  float* d_sens = nullptr;  // 5-point sensitivity for the --gpu variant
+  double* d_scalars = nullptr;
+  bool lambda_valid = false;
+  petstrip_stats stats{};
+};
+
+namespace {
+
+int ensure_device(petstrip_handle* h) {
+  HIP_TRY(hipSetDevice(h->device));
+  return PETSTRIP_OK;
+}
+
+// Event arrays sized for n events.
+int reserve_events(petstrip_handle* h, int64_t n) {
+  if ((size_t)n <= h->cap_events && h->d_geom) return PETSTRIP_OK;
+  dfree(h->d_resp);
+  dfree(h->d_geom);
+  dfree(h->d_lambda);
+  dfree(h->d_inv);
+  dfree(h->d_nbox);
+  dfree(h->d_nell);
+  h->cap_events = 0;
+  size_t c = (size_t)std::max<int64_t>(n, 1);
+  int rc;
+  if ((rc = dalloc(&h->d_resp, 3 * c))) return rc;
+  if ((rc = dalloc(&h->d_geom, c))) return rc;
+  if ((rc = dalloc(&h->d_lambda, c))) return rc;
+  if ((rc = dalloc(&h->d_inv, c))) return rc;
+  if ((rc = dalloc(&h->d_nbox, c))) return rc;
+  if ((rc = dalloc(&h->d_nell, c))) return rc;
+  h->cap_events = c;
+  return PETSTRIP_OK;
+}
+
+// Geometry and footprint statistics for the responses in d_resp.
+int prepare_events(petstrip_handle* h) {
+  const int64_t n = h->n_events;
+  HIP_TRY(petstrip::launch_event_setup(h->stream, h->sg, h->d_resp, n,
+                                       h->d_geom));
+  HIP_TRY(petstrip::launch_footprint(h->stream, h->sg, h->d_geom, n,
+                                     h->d_nbox, h->d_nell));
+  std::vector<int32_t> nb((size_t)n), ne((size_t)n);
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(nb.data(), h->d_nbox, n * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(ne.data(), h->d_nell, n * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  int64_t sb = 0, se = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    sb += nb[(size_t)i];
+    se += ne[(size_t)i];
+  }
+  h->stats.n_events = n;
+  h->stats.n_box_pixels = sb;
+  h->stats.n_ellipse_pixels = se;
+  h->stats.flop_alg = 15.0 * (double)sb + 41.0 * (double)se + 50.0 * (double)n;
+  h->lambda_valid = false;
+  return PETSTRIP_OK;
+}
+
+const float* denominator_weight(petstrip_handle* h) {
+  return h->opts.update == PETSTRIP_UPDATE_SENSITIVITY_DENOMINATOR ? h->d_sens
+                                                                   : nullptr;
+}
+
+// One MLEM iteration: rho <- rho * backprojection(inv) (kept as the
+// reference form out[p] += K rho[p] inv_e).
+int one_iteration(petstrip_handle* h) {
+  const int64_t n = h->n_events;
+  HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
+                                       h->d_rho, denominator_weight(h),
+                                       h->d_lambda, h->d_inv));
+  HIP_TRY(petstrip::launch_fill(h->stream, h->d_out, 0.0f, (int64_t)h->n_pix));
+  HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n,
+                                       h->d_rho, h->d_inv, h->d_out));
+  std::swap(h->d_rho, h->d_out);
+  h->lambda_valid = false;
+  return PETSTRIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int petstrip_abi_version(void) { return PETSTRIP_ABI_VERSION; }
+
+const char* petstrip_last_error(void) { return g_last_error.c_str(); }
+
+int petstrip_device_count(int* count) {
+  if (!count) return fail(PETSTRIP_EINVAL, "count is NULL");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  return PETSTRIP_OK;
+}
+
+int petstrip_create(const petstrip_scanner* scanner, int device,
+                    const petstrip_options* opts, petstrip_handle** out) {
+  if (!out) return fail(PETSTRIP_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = check_scanner(scanner);
+  if (rc) return rc;
+  if (opts && (opts->accumulation < 0 || opts->accumulation > 2 ||
+               opts->update < 0 || opts->update > 1))
+    return fail(PETSTRIP_EINVAL, "invalid options");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(PETSTRIP_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= count)
+    return fail(PETSTRIP_EDEVICE, "device index " + std::to_string(device) +
+                                      " out of range (have " +
+                                      std::to_string(count) + ")");
+  petstrip_handle* h = new petstrip_handle();
+  h->device = device;
+  h->scanner = *scanner;
+  if (opts) h->opts = *opts;
+  h->sg = make_const(*scanner, scanner->sigma_z, scanner->sigma_dl);
+  h->sk = h->sg;
+  h->n_pix = (size_t)scanner->n_y_pixels * (size_t)scanner->n_z_pixels;
+  auto bail = [&](int code) {
+    petstrip_destroy(h);
+    return code;
+  };
+  if ((rc = ensure_device(h))) return bail(rc);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+    return bail(fail(PETSTRIP_EDEVICE, "hipGetDeviceProperties failed"));
+  h->name = prop.name;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(PETSTRIP_EDEVICE, "hipStreamCreate failed"));
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return bail(fail(PETSTRIP_EDEVICE, "hipEventCreate failed"));
+  if ((rc = dalloc(&h->d_rho, h->n_pix))) return bail(rc);
+  if ((rc = dalloc(&h->d_out, h->n_pix))) return bail(rc);
+  if ((rc = dalloc(&h->d_sens, h->n_pix))) return bail(rc);
+  if ((rc = dalloc(&h->d_scalars, 8))) return bail(rc);
+  if ((rc = reserve_events(h, 1))) return bail(rc);
+  // rho = 1 (reconstruction.h:59), 5-point sensitivity for the --gpu variant
+  if (petstrip::launch_fill(h->stream, h->d_rho, 1.0f, (int64_t)h->n_pix) ||
+      petstrip::launch_sensitivity(h->stream, h->sg, 1, h->d_sens) ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return bail(fail(PETSTRIP_EDEVICE, "initialisation kernels failed"));
+  *out = h;
+  return PETSTRIP_OK;
+}
+
+int petstrip_destroy(petstrip_handle* h) {
+  if (!h) return PETSTRIP_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  dfree(h->d_resp);
+  dfree(h->d_geom);
+  dfree(h->d_lambda);
+  dfree(h->d_inv);
+  dfree(h->d_nbox);
+  dfree(h->d_nell);
+  dfree(h->d_rho);
+  dfree(h->d_out);
+  dfree(h->d_sens);
+  dfree(h->d_scalars);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return PETSTRIP_OK;
+}
+
+int petstrip_device_name(petstrip_handle* h, char* buf, size_t len) {
+  if (!h || !buf || len == 0) return fail(PETSTRIP_EINVAL, "bad argument");
+  snprintf(buf, len, "%s", h->name.c_str());
+  return PETSTRIP_OK;
+}
+
+int petstrip_stream(petstrip_handle* h, void** stream) {
+  if (!h || !stream) return fail(PETSTRIP_EINVAL, "bad argument");
+  *stream = (void*)h->stream;
+  return PETSTRIP_OK;
+}
+
+int petstrip_set_options(petstrip_handle* h, const petstrip_options* opts) {
+  if (!h || !opts) return fail(PETSTRIP_EINVAL, "bad argument");
+  if (opts->accumulation < 0 || opts->accumulation > 2 || opts->update < 0 ||
+      opts->update > 1)
+    return fail(PETSTRIP_EINVAL, "invalid options");
+  h->opts = *opts;
+  h->lambda_valid = false;
+  return PETSTRIP_OK;
+}
+
+int petstrip_set_events(petstrip_handle* h, const float* responses, int64_t n) {
+  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
+  if (n < 0 || (n > 0 && !responses))
+    return fail(PETSTRIP_EINVAL, "bad responses/n");
+  if (n > ((int64_t)1 << 40)) return fail(PETSTRIP_EINVAL, "n too large");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  if ((rc = reserve_events(h, n))) return rc;
+  h->n_events = n;
+  if (n > 0)
+    HIP_TRY(hipMemcpyAsync(h->d_resp, responses, (size_t)n * 3 * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
+  return prepare_events(h);
+}
+
+int petstrip_set_events_device(petstrip_handle* h, const float* d_responses,
+                               int64_t n) {
+  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
+  if (n < 0 || (n > 0 && !d_responses))
+    return fail(PETSTRIP_EINVAL, "bad responses/n");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  if ((rc = reserve_events(h, n))) return rc;
+  h->n_events = n;
+  if (n > 0)
+    HIP_TRY(hipMemcpyAsync(h->d_resp, d_responses, (size_t)n * 3 * sizeof(float),
+                           hipMemcpyDeviceToDevice, h->stream));
+  return prepare_events(h);
+}
+
+int petstrip_set_rho(petstrip_handle* h, const float* rho) {
+  if (!h || !rho) return fail(PETSTRIP_EINVAL, "bad argument");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  HIP_TRY(hipMemcpyAsync(h->d_rho, rho, h->n_pix * sizeof(float),
+                         hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->lambda_valid = false;
+  return PETSTRIP_OK;
+}
+
+int petstrip_set_rho_device(petstrip_handle* h, const float* d_rho) {
+  if (!h || !d_rho) return fail(PETSTRIP_EINVAL, "bad argument");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  HIP_TRY(hipMemcpyAsync(h->d_rho, d_rho, h->n_pix * sizeof(float),
+                         hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->lambda_valid = false;
+  return PETSTRIP_OK;
+}
+
+int petstrip_get_rho(petstrip_handle* h, float* rho) {
+  if (!h || !rho) return fail(PETSTRIP_EINVAL, "bad argument");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  HIP_TRY(hipMemcpyAsync(rho, h->d_rho, h->n_pix * sizeof(float),
+                         hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return PETSTRIP_OK;
+}
+
+int petstrip_rho_device(petstrip_handle* h, const float** d_rho) {
+  if (!h || !d_rho) return fail(PETSTRIP_EINVAL, "bad argument");
+  *d_rho = h->d_rho;
+  return PETSTRIP_OK;
+}
+
+int petstrip_iterate(petstrip_handle* h, int n_iterations) {
+  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
+  if (n_iterations < 0) return fail(PETSTRIP_EINVAL, "negative iteration count");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  for (int i = 0; i < n_iterations; ++i) {
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    if ((rc = one_iteration(h))) return rc;
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (n_iterations > 0) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->stats.last_iteration_ms = ms;
+  }
+  return PETSTRIP_OK;
+}
+
+int petstrip_run(petstrip_handle* h, float* rho_inout, int n_blocks,
+                 int n_iter_in_block, petstrip_output_cb output,
+                 petstrip_progress_cb progress, void* user) {
+  if (!h || !rho_inout) return fail(PETSTRIP_EINVAL, "bad argument");
+  if (n_blocks < 0 || n_iter_in_block < 0)
+    return fail(PETSTRIP_EINVAL, "negative block/iteration count");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  const int W = h->scanner.n_z_pixels, H = h->scanner.n_y_pixels;
+  if (output) {
+    std::vector<float> sens(h->n_pix);
+    HIP_TRY(hipMemcpyAsync(sens.data(), h->d_sens, h->n_pix * sizeof(float),
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (output(user, -1, sens.data(), W, H))
+      return fail(PETSTRIP_ECALLBACK, "output callback stopped the run");
+  }
+  if ((rc = petstrip_set_rho(h, rho_inout))) return rc;
+  for (int block = 0; block < n_blocks; ++block) {
+    for (int i = 0; i < n_iter_in_block; ++i) {
+      int iteration = block * n_iter_in_block + i;
+      if (progress) progress(user, iteration, 0);
+      if ((rc = petstrip_iterate(h, 1))) return rc;
+      if (progress) progress(user, iteration, 1);
+      // first block: iterations 1-5, 10, 15, 20, 30, 50, 100 (.cu:98-104)
+      if (!block && i < n_iter_in_block - 1 &&
+          (i < 5 || i == 9 || i == 14 || i == 19 || i == 29 || i == 49 ||
+           i == 99)) {
+        if ((rc = petstrip_get_rho(h, rho_inout))) return rc;
+        if (output && output(user, i + 1, rho_inout, W, H))
+          return fail(PETSTRIP_ECALLBACK, "output callback stopped the run");
+      }
+    }
+    if ((rc = petstrip_get_rho(h, rho_inout))) return rc;
+    if (output && output(user, (block + 1) * n_iter_in_block, rho_inout, W, H))
+      return fail(PETSTRIP_ECALLBACK, "output callback stopped the run");
+  }
+  if (progress) progress(user, n_blocks * n_iter_in_block, 0);
+  return PETSTRIP_OK;
+}
+
+int petstrip_sensitivity(petstrip_handle* h, int mode, float* out) {
+  if (!h || !out || (mode != 0 && mode != 1))
+    return fail(PETSTRIP_EINVAL, "bad argument");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  HIP_TRY(petstrip::launch_sensitivity(h->stream, h->sg, mode, h->d_out));
+  HIP_TRY(hipMemcpyAsync(out, h->d_out, h->n_pix * sizeof(float),
+                         hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return PETSTRIP_OK;
+}
+
+int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
+  if (!h || !out) return fail(PETSTRIP_EINVAL, "bad argument");
+  *out = h->stats;
+  return PETSTRIP_OK;
+}
+
+int petstrip_footprints(petstrip_handle* h, int32_t* n_box, int32_t* n_ell) {
+  if (!h || (h->n_events > 0 && (!n_box || !n_ell)))
+    return fail(PETSTRIP_EINVAL, "bad argument");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  if (h->n_events > 0) {
+    HIP_TRY(hipMemcpyAsync(n_box, h->d_nbox, h->n_events * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(n_ell, h->d_nell, h->n_events * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return PETSTRIP_OK;
+}
+
+int petstrip_forward(petstrip_handle* h, float* lambda) {
+  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom,
+                                       h->n_events, h->d_rho, nullptr,
+                                       h->d_lambda, h->d_inv));
+  if (lambda && h->n_events > 0)
+    HIP_TRY(hipMemcpyAsync(lambda, h->d_lambda, h->n_events * sizeof(float),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->lambda_valid = true;
+  return PETSTRIP_OK;
+}
+
+int petstrip_backward(petstrip_handle* h, const float* g, float* grad_rho,
+                      double grad_sigma[2], double* loglik) {
+  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  const int64_t n = h->n_events;
+  if (!h->lambda_valid && (rc = petstrip_forward(h, nullptr))) return rc;
+  // weights into d_inv: upstream g, or 1/lambda (log-likelihood)
+  HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
+  if (g) {
+    if (n > 0)
+      HIP_TRY(hipMemcpyAsync(h->d_inv, g, n * sizeof(float),
+                             hipMemcpyHostToDevice, h->stream));
+  } else {
+    HIP_TRY(petstrip::launch_loglik_weights(h->stream, h->d_lambda, n, h->d_inv,
+                                            h->d_scalars + 2));
+  }
+  if (grad_rho) {
+    HIP_TRY(petstrip::launch_fill(h->stream, h->d_out, 0.0f, (int64_t)h->n_pix));
+    HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n,
+                                         nullptr, h->d_inv, h->d_out));
+    if (!g)
+      HIP_TRY(petstrip::launch_add_scalar(h->stream, h->d_out, -1.0f,
+                                          (int64_t)h->n_pix));
+    HIP_TRY(hipMemcpyAsync(grad_rho, h->d_out, h->n_pix * sizeof(float),
+                           hipMemcpyDeviceToHost, h->stream));
+  }
+  if (grad_sigma)
+    HIP_TRY(petstrip::launch_sigma_grad(h->stream, h->sg, h->sk, h->d_geom, n,
+                                        h->d_rho, h->d_inv, h->d_scalars));
+  double scal[8];
+  HIP_TRY(hipMemcpyAsync(scal, h->d_scalars, sizeof(scal), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (grad_sigma) {
+    grad_sigma[0] = scal[0];
+    grad_sigma[1] = scal[1];
+  }
+  if (loglik) {
+    if (g) return fail(PETSTRIP_EINVAL, "loglik needs g == NULL");
+    std::vector<float> rho(h->n_pix);
+    if ((rc = petstrip_get_rho(h, rho.data()))) return rc;
+    double srho = 0;
+    for (float v : rho) srho += v;
+    *loglik = scal[2] - srho;
+  }
+  return PETSTRIP_OK;
+}
+
+int petstrip_set_sigma(petstrip_handle* h, double sigma_z, double sigma_dl,
+                       int freeze_support) {
+  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
+  if (!(sigma_z > 0) || !(sigma_dl > 0))
+    return fail(PETSTRIP_EINVAL, "sigmas must be positive");
+  h->sk = make_const(h->scanner, sigma_z, sigma_dl);
+  h->lambda_valid = false;
+  if (!freeze_support) {
+    h->sg = h->sk;
+    h->scanner.sigma_z = sigma_z;
+    h->scanner.sigma_dl = sigma_dl;
+    int rc;
+    if ((rc = ensure_device(h))) return rc;
+    return prepare_events(h);
+  }
+  return PETSTRIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,34 @@
+// Launch wrappers of the strip-PET HIP kernels (strip_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "strip_geometry.h"
+
+namespace petstrip {
+
+hipError_t launch_event_setup(hipStream_t st, const StripConst& s,
+                              const float* resp, int64_t n, EventGeom* geom);
+hipError_t launch_footprint(hipStream_t st, const StripConst& s,
+                            const EventGeom* geom, int64_t n, int32_t* nbox,
+                            int32_t* nell);
+hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
+                              const StripConst& sk, const EventGeom* geom,
+                              int64_t n, const float* rho, const float* weight,
+                              float* lambda, float* inv);
+hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
+                              const StripConst& sk, const EventGeom* geom,
+                              int64_t n, const float* rho, const float* w,
+                              float* out);
+hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
+                             const StripConst& sk, const EventGeom* geom,
+                             int64_t n, const float* rho, const float* w,
+                             double* out2);
+hipError_t launch_fill(hipStream_t st, float* p, float v, int64_t n);
+hipError_t launch_add_scalar(hipStream_t st, float* p, float v, int64_t n);
+hipError_t launch_loglik_weights(hipStream_t st, const float* lambda,
+                                 int64_t n, float* g, double* ll);
+hipError_t launch_sensitivity(hipStream_t st, const StripConst& s, int mode,
+                              float* out);
+
+}  // namespace petstrip

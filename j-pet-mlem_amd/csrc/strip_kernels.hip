@@ -1,0 +1,387 @@
+// HIP kernels of the strip-PET list-mode MLEM for gfx950 (wave64).
+//
+// Per iteration (reference: Reconstruction::operator(),
+// src/2d/strip/reconstruction.h:108-154, and bb_update :220-298):
+//   pass 1  k_denominator   one wave per event: lambda_e = sum K_e(p) rho[p]
+//                           over the 3-sigma ellipse inside the clamped box,
+//                           reduced across the wave with DPP; stores
+//                           inv_e = lambda_e > 0 ? 1/lambda_e : 0.
+//   pass 2  k_backproject   one wave per event: out[p] += K_e(p) rho[p] inv_e
+//                           with hardware float atomics.
+// Geometry (which pixels) is bit-identical to the reference CPU build
+// (strip_geometry.h); the kernel value itself is evaluated in FP32 with the
+// device exp/sqrt, within a few ulp of the host libm.
+#include <hip/hip_runtime.h>
+
+#include "strip_geometry.h"
+#include "strip_kernels.h"
+
+namespace petstrip {
+
+// ---------------------------------------------------------------------------
+// Wave64 helpers.
+
+// Sum over the 64 lanes of a wave; every lane returns the same value and the
+// order of additions is fixed (bit-reproducible).  Rows of 16 lanes are
+// reduced with DPP (quad_perm 1032, quad_perm 2301, row_ror 4, row_ror 8),
+// then the four row sums are combined in a fixed order.
+__device__ __forceinline__ float wave_sum(float v) {
+  v += __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xb1, 0xf, 0xf, false));
+  v += __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4e, 0xf, 0xf, false));
+  v += __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false));
+  v += __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false));
+  float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+  float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+  float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ int uniform_i(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// GaussianKernel::operator()(y, tan, sec, R, distance)
+// (src/2d/strip/gaussian_kernel.h:51-79) in FP32.
+__device__ __forceinline__ float kernel_value(const StripConst& k,
+                                              const EventGeom& g, float dy,
+                                              float dz) {
+  float sec_sq = g.sec * g.sec;
+  float yp = dy + g.y;
+  float ap = -(yp - k.R) * sec_sq;
+  float aq = -(k.R + yp) * sec_sq;
+  float ar = ((yp * -2.0f) * g.sec) * g.t;
+  float pq = fmaf(-dy, g.t, dz);
+  float br = (dy * -2.0f) * g.sec;
+  float norm = fmaf(ar * k.isdl, ar, fmaf(ap * k.isz, ap, (aq * k.isz) * aq));
+  float bcb = fmaf((pq * k.isz) * pq, 2.0f, (br * k.isdl) * br);
+  return (k.c / sqrtf(norm)) * expf(bcb * -0.5f);
+}
+
+__device__ __forceinline__ EventGeom load_geom(const EventGeom* geom, int64_t e) {
+  const float4* p = reinterpret_cast<const float4*>(geom + e);
+  float4 a = p[0], b = p[1];
+  int4 c = reinterpret_cast<const int4*>(p)[2];
+  EventGeom g;
+  g.z = a.x; g.y = a.y; g.t = a.z; g.sec = a.w;
+  g.A = b.x; g.B = b.y; g.C = b.z; g.pad = b.w;
+  g.x0 = c.x; g.x1 = c.y; g.y0 = c.z; g.y1 = c.w;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// Setup: responses -> geometry, footprint counts.
+
+__global__ __launch_bounds__(256) void k_event_setup(StripConst s,
+                                                     const float* __restrict__ resp,
+                                                     int64_t n,
+                                                     EventGeom* __restrict__ geom) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  EventGeom g;
+  event_geometry(s, resp[3 * e], resp[3 * e + 1], resp[3 * e + 2], g);
+  geom[e] = g;
+}
+
+// One wave per event: box and ellipse pixel counts.
+__global__ __launch_bounds__(256) void k_footprint(StripConst s,
+                                                   const EventGeom* __restrict__ geom,
+                                                   int64_t n,
+                                                   int32_t* __restrict__ n_box,
+                                                   int32_t* __restrict__ n_ell) {
+  const int lane = threadIdx.x & 63;
+  int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
+  if (e >= n) return;
+  EventGeom g = load_geom(geom, e);
+  int bw = g.x1 - g.x0, bh = g.y1 - g.y0;
+  int nbox = (bw > 0 && bh > 0) ? bw * bh : 0;
+  int cnt = 0;
+  for (int k = lane; k < nbox; k += 64) {
+    int row = k / bw, col = k - row * bw;
+    float dy = row_dy(s, g, g.y0 + row);
+    float dz = col_dz(s, g, g.x0 + col);
+    cnt += in_ellipse(g, dy, dz) ? 1 : 0;
+  }
+  float tot = wave_sum((float)cnt);
+  if (lane == 0) {
+    n_box[e] = nbox;
+    n_ell[e] = (int32_t)tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: denominators.  weight (nullable) multiplies rho (the reference
+// --gpu variant weights the denominator by the pixel sensitivity).
+
+__global__ __launch_bounds__(256) void k_denominator(StripConst sg, StripConst sk,
+                                                     const EventGeom* __restrict__ geom,
+                                                     int64_t n,
+                                                     const float* __restrict__ rho,
+                                                     const float* __restrict__ weight,
+                                                     float* __restrict__ lambda,
+                                                     float* __restrict__ inv) {
+  const int lane = threadIdx.x & 63;
+  int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
+  if (e >= n) return;
+  EventGeom g = load_geom(geom, e);
+  int bw = g.x1 - g.x0, bh = g.y1 - g.y0;
+  int nbox = (bw > 0 && bh > 0) ? bw * bh : 0;
+  float acc = 0.0f;
+  for (int k = lane; k < nbox; k += 64) {
+    int row = k / bw, col = k - row * bw;
+    int iy = g.y0 + row, iz = g.x0 + col;
+    float dy = row_dy(sg, g, iy);
+    float dz = col_dz(sg, g, iz);
+    if (in_ellipse(g, dy, dz)) {
+      int p = iy * sg.W + iz;
+      float r = rho[p];
+      if (weight) r *= weight[p];
+      acc += kernel_value(sk, g, dy, dz) * r;
+    }
+  }
+  float d = wave_sum(acc);
+  if (lane == 0) {
+    lambda[e] = d;
+    inv[e] = (d > 0.0f) ? 1.0f / d : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2: backprojection out[p] += K rho[p] * w_e (w_e = inv_e for MLEM,
+// rho == nullptr gives out[p] += K * w_e, the adjoint of the projector).
+
+__global__ __launch_bounds__(256) void k_backproject(StripConst sg, StripConst sk,
+                                                     const EventGeom* __restrict__ geom,
+                                                     int64_t n,
+                                                     const float* __restrict__ rho,
+                                                     const float* __restrict__ w,
+                                                     float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
+  if (e >= n) return;
+  float we = w[e];
+  if (we == 0.0f) return;
+  EventGeom g = load_geom(geom, e);
+  int bw = g.x1 - g.x0, bh = g.y1 - g.y0;
+  int nbox = (bw > 0 && bh > 0) ? bw * bh : 0;
+  for (int k = lane; k < nbox; k += 64) {
+    int row = k / bw, col = k - row * bw;
+    int iy = g.y0 + row, iz = g.x0 + col;
+    float dy = row_dy(sg, g, iy);
+    float dz = col_dz(sg, g, iz);
+    if (in_ellipse(g, dy, dz)) {
+      int p = iy * sg.W + iz;
+      float kv = kernel_value(sk, g, dy, dz);
+      if (rho) kv *= rho[p];
+      unsafeAtomicAdd(&out[p], kv * we);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sigma derivative of lambda_e weighted by g_e, summed over events:
+//   dlnK/dsz  = (-2 + isz (ap^2+aq^2)/norm + 2 isz pq^2) / sz
+//   dlnK/dsdl = (-1 + isdl ar^2/norm + isdl br^2) / sdl
+// accumulated in FP64 per wave, one atomic per wave.
+
+__global__ __launch_bounds__(256) void k_sigma_grad(StripConst sg, StripConst sk,
+                                                    const EventGeom* __restrict__ geom,
+                                                    int64_t n,
+                                                    const float* __restrict__ rho,
+                                                    const float* __restrict__ w,
+                                                    double* __restrict__ out2) {
+  const int lane = threadIdx.x & 63;
+  int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
+  if (e >= n) return;
+  float we = w[e];
+  if (we == 0.0f) return;
+  EventGeom g = load_geom(geom, e);
+  int bw = g.x1 - g.x0, bh = g.y1 - g.y0;
+  int nbox = (bw > 0 && bh > 0) ? bw * bh : 0;
+  float az = 0.0f, adl = 0.0f;
+  for (int k = lane; k < nbox; k += 64) {
+    int row = k / bw, col = k - row * bw;
+    int iy = g.y0 + row, iz = g.x0 + col;
+    float dy = row_dy(sg, g, iy);
+    float dz = col_dz(sg, g, iz);
+    if (in_ellipse(g, dy, dz)) {
+      int p = iy * sg.W + iz;
+      float sec_sq = g.sec * g.sec;
+      float yp = dy + g.y;
+      float ap = -(yp - sk.R) * sec_sq;
+      float aq = -(sk.R + yp) * sec_sq;
+      float ar = ((yp * -2.0f) * g.sec) * g.t;
+      float pq = fmaf(-dy, g.t, dz);
+      float br = (dy * -2.0f) * g.sec;
+      float apq = ap * ap + aq * aq;
+      float norm = sk.isz * apq + sk.isdl * (ar * ar);
+      float bsb_z = 2.0f * sk.isz * (pq * pq);
+      float bsb_dl = sk.isdl * (br * br);
+      float K = (sk.c / sqrtf(norm)) * expf(-0.5f * (bsb_z + bsb_dl));
+      float kr = K * rho[p];
+      az += kr * (-2.0f + sk.isz * apq / norm + bsb_z);
+      adl += kr * (-1.0f + sk.isdl * (ar * ar) / norm + bsb_dl);
+    }
+  }
+  double sz = wave_sum_d((double)az) * (double)we / (double)sk.sigma_z;
+  double sdl = wave_sum_d((double)adl) * (double)we / (double)sk.sigma_dl;
+  if (lane == 0) {
+    unsafeAtomicAdd(&out2[0], sz);
+    unsafeAtomicAdd(&out2[1], sdl);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise helpers.
+
+__global__ void k_fill(float* __restrict__ p, float v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_add_scalar(float* __restrict__ p, float v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] += v;
+}
+
+// g_e = lambda_e > 0 ? 1/lambda_e : 0 and the log-likelihood data term.
+__global__ __launch_bounds__(256) void k_loglik_weights(const float* __restrict__ lambda,
+                                                        int64_t n,
+                                                        float* __restrict__ g,
+                                                        double* __restrict__ ll) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (e < n) {
+    float l = lambda[e];
+    g[e] = (l > 0.0f) ? 1.0f / l : 0.0f;
+    if (l > 0.0f) v = log((double)l);
+  }
+  v = wave_sum_d(v);
+  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(ll, v);
+}
+
+// Scanner::sensitivity (scanner.h:147-171) at a point.
+__device__ float sens_point(const StripConst& s, float z, float y) {
+  float L_plus = s.half_L + z;
+  float L_minus = s.half_L - z;
+  float R_plus = s.R + y;
+  float R_minus = s.R - y;
+  if (R_plus <= 0) R_plus = 1;
+  if (R_minus <= 0) R_minus = 1;
+  float a = L_minus / R_minus, b = L_plus / R_plus;
+  float mn = (b < a) ? b : a;
+  float c = -L_plus / R_minus, d = -L_minus / R_plus;
+  float mx = (c < d) ? d : c;
+  return 0.318309886183790671538f * (atanf(mn) - atanf(mx));
+}
+
+__global__ void k_sensitivity(StripConst s, int mode, float* __restrict__ out) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= s.W) return;
+  float pz = fmaf((float)x, s.pw, s.tl_z_half_w);
+  float py = fmaf(-(float)y, s.ph, s.tl_y_half_h);
+  float v;
+  if (mode == 0) {
+    v = sens_point(s, pz, py);
+  } else {
+    float hw = s.pw / 2, hh = s.ph / 2;
+    v = sens_point(s, pz, py) / 3 + sens_point(s, pz + hw, py + hh) / 6 +
+        sens_point(s, pz - hw, py - hh) / 6 + sens_point(s, pz - hw, py + hh) / 6 +
+        sens_point(s, pz + hw, py - hh) / 6;
+  }
+  out[(size_t)y * s.W + x] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side).
+
+static inline unsigned blocks_for(int64_t n, int per_block) {
+  return (unsigned)((n + per_block - 1) / per_block);
+}
+
+hipError_t launch_event_setup(hipStream_t st, const StripConst& s,
+                              const float* resp, int64_t n, EventGeom* geom) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_event_setup, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                     s, resp, n, geom);
+  return hipGetLastError();
+}
+
+hipError_t launch_footprint(hipStream_t st, const StripConst& s,
+                            const EventGeom* geom, int64_t n, int32_t* nbox,
+                            int32_t* nell) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_footprint, dim3(blocks_for(n, 4)), dim3(256), 0, st, s,
+                     geom, n, nbox, nell);
+  return hipGetLastError();
+}
+
+hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
+                              const StripConst& sk, const EventGeom* geom,
+                              int64_t n, const float* rho, const float* weight,
+                              float* lambda, float* inv) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_denominator, dim3(blocks_for(n, 4)), dim3(256), 0, st,
+                     sg, sk, geom, n, rho, weight, lambda, inv);
+  return hipGetLastError();
+}
+
+hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
+                              const StripConst& sk, const EventGeom* geom,
+                              int64_t n, const float* rho, const float* w,
+                              float* out) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_backproject, dim3(blocks_for(n, 4)), dim3(256), 0, st,
+                     sg, sk, geom, n, rho, w, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
+                             const StripConst& sk, const EventGeom* geom,
+                             int64_t n, const float* rho, const float* w,
+                             double* out2) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sigma_grad, dim3(blocks_for(n, 4)), dim3(256), 0, st,
+                     sg, sk, geom, n, rho, w, out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t st, float* p, float v, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_scalar(hipStream_t st, float* p, float v, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_add_scalar, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                     p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_loglik_weights(hipStream_t st, const float* lambda, int64_t n,
+                                 float* g, double* ll) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_loglik_weights, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     st, lambda, n, g, ll);
+  return hipGetLastError();
+}
+
+hipError_t launch_sensitivity(hipStream_t st, const StripConst& s, int mode,
+                              float* out) {
+  hipLaunchKernelGGL(k_sensitivity, dim3(blocks_for(s.W, 128), s.H), dim3(128),
+                     0, st, s, mode, out);
+  return hipGetLastError();
+}
+
+}  // namespace petstrip

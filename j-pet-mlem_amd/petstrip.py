@@ -1,0 +1,311 @@
+"""Python host side of petstrip: ctypes binding of libpetstrip_hip.so.
+
+Mirrors the reference's strip reconstruction interface
+(grey277/j-pet-mlem):
+  Scanner(radius, scintillator_length, n_y_pixels, n_z_pixels, pixel_height,
+          pixel_width, sigma_z, sigma_dl, center_y=0, center_z=0)
+                                            src/2d/strip/scanner.h:39-49
+  Reconstruction(scanner): rho = 1, responses, operator()(n_iterations)
+                                            src/2d/strip/reconstruction.h:57-154
+  GPU::Reconstruction::run(...)             src/2d/strip/cuda/reconstruction.h:31-44
+Every compute call goes through the HIP library; there is no CPU fallback.
+A failing call raises PetstripError with the library's message.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpetstrip_hip.so")
+
+ACCUM_FP32, ACCUM_COMPENSATED, ACCUM_FIXED64 = 0, 1, 2
+UPDATE_CPU, UPDATE_SENSITIVITY_DENOMINATOR = 0, 1
+
+
+class PetstripError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"petstrip error {code}: {msg}")
+        self.code = code
+
+
+class CScanner(C.Structure):
+    _fields_ = [
+        ("radius", C.c_double), ("scintillator_length", C.c_double),
+        ("n_y_pixels", C.c_int32), ("n_z_pixels", C.c_int32),
+        ("pixel_height", C.c_double), ("pixel_width", C.c_double),
+        ("sigma_z", C.c_double), ("sigma_dl", C.c_double),
+        ("center_y", C.c_double), ("center_z", C.c_double),
+    ]
+
+
+class COptions(C.Structure):
+    _fields_ = [("accumulation", C.c_int32), ("update", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+
+class CStats(C.Structure):
+    _fields_ = [
+        ("n_events", C.c_int64), ("n_box_pixels", C.c_int64),
+        ("n_ellipse_pixels", C.c_int64), ("n_zero_denominator", C.c_int64),
+        ("flop_alg", C.c_double), ("last_iteration_ms", C.c_double),
+    ]
+
+
+OUTPUT_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_float),
+                        C.c_int32, C.c_int32)
+PROGRESS_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int)
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(path)
+    vp, i32, i64, dp, fp = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_void_p
+    sig = {
+        "petstrip_abi_version": ([], C.c_int),
+        "petstrip_last_error": ([], C.c_char_p),
+        "petstrip_device_count": ([C.POINTER(C.c_int)], C.c_int),
+        "petstrip_create": ([C.POINTER(CScanner), C.c_int, C.POINTER(COptions),
+                             C.POINTER(vp)], C.c_int),
+        "petstrip_destroy": ([vp], C.c_int),
+        "petstrip_device_name": ([vp, C.c_char_p, C.c_size_t], C.c_int),
+        "petstrip_stream": ([vp, C.POINTER(vp)], C.c_int),
+        "petstrip_set_options": ([vp, C.POINTER(COptions)], C.c_int),
+        "petstrip_set_events": ([vp, fp, i64], C.c_int),
+        "petstrip_set_events_device": ([vp, vp, i64], C.c_int),
+        "petstrip_set_rho": ([vp, fp], C.c_int),
+        "petstrip_get_rho": ([vp, fp], C.c_int),
+        "petstrip_set_rho_device": ([vp, vp], C.c_int),
+        "petstrip_rho_device": ([vp, C.POINTER(vp)], C.c_int),
+        "petstrip_iterate": ([vp, C.c_int], C.c_int),
+        "petstrip_run": ([vp, fp, C.c_int, C.c_int, OUTPUT_CB, PROGRESS_CB, vp],
+                         C.c_int),
+        "petstrip_sensitivity": ([vp, C.c_int, fp], C.c_int),
+        "petstrip_stats_get": ([vp, C.POINTER(CStats)], C.c_int),
+        "petstrip_footprints": ([vp, fp, fp], C.c_int),
+        "petstrip_forward": ([vp, fp], C.c_int),
+        "petstrip_backward": ([vp, fp, fp, C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)], C.c_int),
+        "petstrip_set_sigma": ([vp, dp, dp, C.c_int], C.c_int),
+        "petstrip_phantom_events": ([C.POINTER(CScanner), fp, i32, C.c_uint64,
+                                     i64, fp, C.c_int], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise PetstripError(rc, _lib.petstrip_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Scanner:
+    """Strip scanner geometry (src/2d/strip/scanner.h:39-67)."""
+
+    def __init__(self, radius, scintillator_length, n_y_pixels, n_z_pixels,
+                 pixel_height, pixel_width, sigma_z, sigma_dl, center_y=0.0,
+                 center_z=0.0):
+        self.radius = float(radius)
+        self.scintillator_length = float(scintillator_length)
+        self.n_y_pixels = int(n_y_pixels)
+        self.n_z_pixels = int(n_z_pixels)
+        self.pixel_height = float(pixel_height)
+        self.pixel_width = float(pixel_width)
+        self.sigma_z = float(sigma_z)
+        self.sigma_dl = float(sigma_dl)
+        self.center_y = float(center_y)
+        self.center_z = float(center_z)
+
+    @classmethod
+    def square(cls, radius, length, n_pixels, pixel_size, sigma_z, sigma_dl):
+        return cls(radius, length, n_pixels, n_pixels, pixel_size, pixel_size,
+                   sigma_z, sigma_dl)
+
+    @property
+    def total_n_pixels(self):
+        return self.n_y_pixels * self.n_z_pixels
+
+    def c(self):
+        return CScanner(self.radius, self.scintillator_length, self.n_y_pixels,
+                        self.n_z_pixels, self.pixel_height, self.pixel_width,
+                        self.sigma_z, self.sigma_dl, self.center_y, self.center_z)
+
+
+def device_count():
+    lib = load_library()
+    c = C.c_int(0)
+    _check(lib.petstrip_device_count(C.byref(c)))
+    return c.value
+
+
+def phantom_events(scanner, regions, n, seed=1, n_threads=None):
+    """Seeded detected events (n x 3 float32: z_u, z_d, dl).
+
+    regions: iterable of (x, y, a, b, angle_deg, intensity); earlier regions
+    lie on top (non-additive), as in src/2d/geometry/phantom.h:193-203.
+    """
+    lib = load_library()
+    reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1, 6))
+    out = np.empty((int(n), 3), dtype=np.float32)
+    if n_threads is None:
+        n_threads = min(os.cpu_count() or 1, 16)
+    sc = scanner.c()
+    _check(lib.petstrip_phantom_events(C.byref(sc), _ptr(reg), reg.shape[0],
+                                       C.c_uint64(seed), C.c_int64(int(n)),
+                                       _ptr(out), int(n_threads)))
+    return out
+
+
+class Reconstruction:
+    """List-mode MLEM on one GPU (src/2d/strip/reconstruction.h:32-154).
+
+    rho starts at 1; responses are (z_u, z_d, dl) float32 triples.
+    """
+
+    def __init__(self, scanner, device=0, accumulation=ACCUM_FP32,
+                 update=UPDATE_CPU):
+        self.lib = load_library()
+        self.scanner = scanner
+        self._h = C.c_void_p()
+        opts = COptions(accumulation, update)
+        sc = scanner.c()
+        _check(self.lib.petstrip_create(C.byref(sc), device, C.byref(opts),
+                                        C.byref(self._h)))
+        self.n_events = 0
+
+    def close(self):
+        if self._h:
+            self.lib.petstrip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.scanner.n_y_pixels, self.scanner.n_z_pixels)
+
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        _check(self.lib.petstrip_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    def stream(self):
+        s = C.c_void_p()
+        _check(self.lib.petstrip_stream(self._h, C.byref(s)))
+        return s.value
+
+    def set_options(self, accumulation=ACCUM_FP32, update=UPDATE_CPU):
+        o = COptions(accumulation, update)
+        _check(self.lib.petstrip_set_options(self._h, C.byref(o)))
+
+    def set_events(self, responses):
+        r = np.ascontiguousarray(responses, dtype=np.float32).reshape(-1, 3)
+        _check(self.lib.petstrip_set_events(self._h, _ptr(r), C.c_int64(r.shape[0])))
+        self.n_events = r.shape[0]
+
+    def set_events_device(self, ptr, n):
+        _check(self.lib.petstrip_set_events_device(self._h, C.c_void_p(ptr),
+                                                   C.c_int64(int(n))))
+        self.n_events = int(n)
+
+    @property
+    def rho(self):
+        out = np.empty(self.shape, dtype=np.float32)
+        _check(self.lib.petstrip_get_rho(self._h, _ptr(out)))
+        return out
+
+    @rho.setter
+    def rho(self, value):
+        v = np.ascontiguousarray(value, dtype=np.float32).reshape(self.shape)
+        _check(self.lib.petstrip_set_rho(self._h, _ptr(v)))
+
+    def __call__(self, n_iterations=1):
+        """Runs n_iterations MLEM iterations (reconstruction.h:108-154)."""
+        _check(self.lib.petstrip_iterate(self._h, int(n_iterations)))
+        return self
+
+    iterate = __call__
+
+    def run(self, rho, n_blocks, n_iterations_in_block, output=None, progress=None):
+        """The reference GPU run(): output(iteration, image) at the reference's
+        output iterations (-1 = sensitivity); progress(completed, finished)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float32).reshape(self.shape)
+        W, H = self.scanner.n_z_pixels, self.scanner.n_y_pixels
+
+        def _out(user, it, img, w, h):
+            if output is None:
+                return 0
+            a = np.ctypeslib.as_array(img, shape=(h, w)).copy()
+            r = output(it, a)
+            return 1 if r else 0
+
+        def _prog(user, completed, finished):
+            if progress is not None:
+                progress(completed, bool(finished))
+
+        ocb, pcb = OUTPUT_CB(_out), PROGRESS_CB(_prog)
+        _check(self.lib.petstrip_run(self._h, _ptr(rho), int(n_blocks),
+                                     int(n_iterations_in_block), ocb, pcb, None))
+        return rho
+
+    def sensitivity(self, mode=0):
+        out = np.empty(self.shape, dtype=np.float32)
+        _check(self.lib.petstrip_sensitivity(self._h, int(mode), _ptr(out)))
+        return out
+
+    def stats(self):
+        s = CStats()
+        _check(self.lib.petstrip_stats_get(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in CStats._fields_}
+
+    def footprints(self):
+        nb = np.empty(self.n_events, dtype=np.int32)
+        ne = np.empty(self.n_events, dtype=np.int32)
+        _check(self.lib.petstrip_footprints(self._h, _ptr(nb), _ptr(ne)))
+        return nb, ne
+
+    def forward(self):
+        lam = np.empty(self.n_events, dtype=np.float32)
+        _check(self.lib.petstrip_forward(self._h, _ptr(lam)))
+        return lam
+
+    def backward(self, g=None, want_rho=True, want_sigma=True, want_loglik=False):
+        gr = np.empty(self.shape, dtype=np.float32) if want_rho else None
+        gs = (C.c_double * 2)() if want_sigma else None
+        ll = C.c_double(0.0) if want_loglik else None
+        garr = None if g is None else np.ascontiguousarray(g, dtype=np.float32)
+        _check(self.lib.petstrip_backward(
+            self._h, _ptr(garr), _ptr(gr),
+            gs if gs is not None else None,
+            C.byref(ll) if ll is not None else None))
+        return (gr, (gs[0], gs[1]) if gs is not None else None,
+                ll.value if ll is not None else None)
+
+    def set_sigma(self, sigma_z, sigma_dl, freeze_support=False):
+        _check(self.lib.petstrip_set_sigma(self._h, float(sigma_z), float(sigma_dl),
+                                           1 if freeze_support else 0))

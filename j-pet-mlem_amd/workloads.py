@@ -1,0 +1,67 @@
+"""Named strip-PET workloads: scanner geometry and seeded phantoms.
+
+Geometry: strips at y = +-0.43 m, length 0.5 m, sigma_z = 0.01 m,
+sigma_dl = 0.04 m.  Phantom regions are (x, y, a, b, angle_deg, intensity),
+earlier regions on top (src/2d/geometry/phantom.h:193-203).
+"""
+import numpy as np
+
+from .petstrip import Scanner, phantom_events
+
+R, L, SIGMA_Z, SIGMA_DL = 0.43, 0.5, 0.01, 0.04
+
+# three ellipses: cold circle, hot circle, elliptical background
+PHANTOM_3 = [
+    (-0.08, 0.03, 0.03, 0.03, 0.0, 0.0),
+    (0.08, -0.02, 0.04, 0.04, 0.0, 4.0),
+    (0.0, 0.0, 0.20, 0.15, 0.0, 1.0),
+]
+
+
+def shepp6(seed=7):
+    """Six-ellipse Shepp-like phantom, intensities 0, 0.2, 1, 1.5, 2, 4,
+    semi-axes 0.02-0.22 m, rotations drawn from `seed`."""
+    rot = np.random.default_rng(seed).uniform(0.0, 180.0, 6)
+    return [
+        (0.05, 0.06, 0.025, 0.020, rot[0], 0.0),
+        (-0.10, -0.04, 0.060, 0.040, rot[1], 4.0),
+        (0.10, -0.06, 0.050, 0.030, rot[2], 2.0),
+        (-0.06, 0.08, 0.070, 0.045, rot[3], 1.5),
+        (0.00, 0.00, 0.150, 0.110, rot[4], 0.2),
+        (0.00, 0.00, 0.220, 0.170, rot[5], 1.0),
+    ]
+
+
+def shepp10(seed=11):
+    """Ten-ellipse phantom for the 0.5 mm workload."""
+    rng = np.random.default_rng(seed)
+    rot = rng.uniform(0.0, 180.0, 10)
+    base = [
+        (0.02, 0.07, 0.020, 0.020, 0.0), (-0.12, -0.05, 0.030, 0.020, 4.0),
+        (0.12, 0.04, 0.025, 0.025, 3.0), (-0.05, -0.09, 0.040, 0.030, 0.5),
+        (0.07, -0.08, 0.050, 0.030, 2.0), (-0.10, 0.07, 0.045, 0.035, 1.5),
+        (0.15, -0.01, 0.030, 0.060, 2.5), (-0.01, 0.00, 0.090, 0.060, 0.2),
+        (0.00, 0.00, 0.160, 0.120, 1.2), (0.00, 0.00, 0.220, 0.170, 1.0),
+    ]
+    return [(x, y, a, b, rot[i], c) for i, (x, y, a, b, c) in enumerate(base)]
+
+
+WORKLOADS = {
+    # name: (n_pixels, pixel_size, n_events, iterations, phantom, seed)
+    "strip-160px-4mm-1e6ev": (160, 0.004, 1_000_000, 10, PHANTOM_3, 1),
+    "strip-512px-1mm-2e7ev": (512, 0.001, 20_000_000, 10, shepp6(), 2),
+    "strip-256px-2mm-5e6ev": (256, 0.002, 5_000_000, 1, PHANTOM_3, 4),
+    "strip-1024px-0.5mm-1e8ev": (1024, 0.0005, 100_000_000, 5, shepp10(), 5),
+}
+
+
+def workload_scanner(name, sigma_z=SIGMA_Z, sigma_dl=SIGMA_DL):
+    n_pix, size = WORKLOADS[name][:2]
+    return Scanner(R, L, n_pix, n_pix, size, size, sigma_z, sigma_dl)
+
+
+def workload_events(name, n=None, seed=None):
+    n_pix, size, n_ev, _, phantom, s = WORKLOADS[name]
+    sc = workload_scanner(name)
+    return phantom_events(sc, phantom, n_ev if n is None else n,
+                          seed=s if seed is None else seed)

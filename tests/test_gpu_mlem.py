@@ -1,0 +1,142 @@
+"""GPU MLEM parity against the CPU oracle (which is bitwise equal to the
+reference CPU build, see test_oracle_reference.py).
+
+Tolerances: discrete decisions (box, ellipse membership) must match exactly;
+per-event denominators within 2e-6 relative (kernel value evaluated by the
+device exp/sqrt instead of host libm); images within 1e-5 relative L2 per
+iteration of the FP32 oracle and within 1e-4 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) /
+                 max(np.linalg.norm(np.asarray(b, np.float64)), 1e-300))
+
+
+def small(gpu_pkg, n_pix=64, size=0.008, n=20000, seed=3, phantom=None):
+    pkg = gpu_pkg
+    sc = pkg.Scanner(0.43, 0.5, n_pix, n_pix, size, size, 0.01, 0.04)
+    ev = pkg.phantom_events(sc, phantom or pkg.workloads.PHANTOM_3, n, seed=seed)
+    return sc, ev
+
+
+def test_footprints_exact(gpu_pkg):
+    sc, ev = small(gpu_pkg, n=30000)
+    with gpu_pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        nb, ne = rec.footprints()
+        st = rec.stats()
+    onb, one = ol.footprint(sc, ev)
+    print(f"case: 64x64, 3e4 events, box {onb.sum()} ellipse {one.sum()}", flush=True)
+    assert np.array_equal(nb, onb)
+    assert np.array_equal(ne, one)
+    assert st["n_box_pixels"] == int(onb.sum())
+    assert st["n_ellipse_pixels"] == int(one.sum())
+
+
+def test_denominators(gpu_pkg):
+    sc, ev = small(gpu_pkg, n=20000)
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0.5, 2.0, (sc.n_y_pixels, sc.n_z_pixels)).astype(np.float32)
+    with gpu_pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        rec.rho = rho
+        lam = rec.forward()
+    ref = ol.denominators_f32(sc, ev, rho)
+    pos = ref > 0
+    assert np.array_equal(lam > 0, pos)
+    err = np.abs(lam[pos] - ref[pos]) / ref[pos]
+    print(f"denominator max rel err {err.max():.3e}", flush=True)
+    assert err.max() < 2e-6
+
+
+@pytest.mark.parametrize("n_pix,size", [(64, 0.008), (41, 0.0125), (100, 0.005)])
+def test_mlem_small(gpu_pkg, n_pix, size):
+    sc, ev = small(gpu_pkg, n_pix=n_pix, size=size)
+    print(f"case: {n_pix}x{n_pix} @ {size} m, {len(ev)} events", flush=True)
+    with gpu_pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        outs = []
+        for _ in range(5):
+            rec(1)
+            outs.append(rec.rho)
+    ref32 = ol.mlem_f32(sc, ev, 5, n_threads=8)
+    ref64 = ol.mlem_f64(sc, ev, 5, n_threads=8)
+    for i in range(5):
+        e32, e64 = rel_l2(outs[i], ref32[i]), rel_l2(outs[i], ref64[i])
+        print(f"iter {i + 1}: rel L2 vs f32 {e32:.2e} vs f64 {e64:.2e}", flush=True)
+        assert e32 < 1e-5
+        assert e64 < 1e-4
+    # invariant: sum rho' = number of events with a positive denominator
+    lam = ol.denominators_f32(sc, ev, ref32[3])
+    assert abs(outs[4].sum(dtype=np.float64) - np.count_nonzero(lam > 0)) < 1e-3 * len(ev)
+    # the last row and column are never visited (exclusive clamped box)
+    assert np.all(outs[0][-1, :] == 0) and np.all(outs[0][:, -1] == 0)
+
+
+def test_nonsquare_and_offsets(gpu_pkg):
+    pkg = gpu_pkg
+    sc = pkg.Scanner(0.45, 0.4, 70, 90, 0.006, 0.005, 0.012, 0.05, 0.01, -0.02)
+    ev = pkg.phantom_events(sc, pkg.workloads.PHANTOM_3, 15000, seed=9)
+    with pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        rec(3)
+        out = rec.rho
+    osc = ol.SimpleScanner(0.45, 0.4, 70, 90, 0.006, 0.005, 0.012, 0.05, 0.01, -0.02)
+    ref = ol.mlem_f32(osc, ev, 3, n_threads=8)[-1]
+    assert out.shape == (70, 90)
+    assert rel_l2(out, ref) < 1e-5
+
+
+def test_edge_events(gpu_pkg):
+    """Empty input, events outside the field of view and non-finite events
+    contribute nothing and produce no NaN (reconstruction.h:290)."""
+    pkg = gpu_pkg
+    sc = pkg.Scanner(0.43, 0.5, 32, 32, 0.01, 0.01, 0.01, 0.04)
+    with pkg.Reconstruction(sc) as rec:
+        rec.set_events(np.zeros((0, 3), np.float32))
+        rec(2)
+        assert np.all(rec.rho == 0)
+        bad = np.array([[5.0, 5.0, 0.0], [-7.0, -7.0, 0.1], [np.nan, 0.0, 0.0],
+                        [np.inf, 0.0, 0.0], [0.0, 0.0, np.nan], [0.01, 0.02, 0.03]],
+                       np.float32)
+        rec.rho = np.ones((32, 32), np.float32)
+        rec.set_events(bad)
+        rec(1)
+        out = rec.rho
+        ref = ol.mlem_f32(ol.SimpleScanner(0.43, 0.5, 32, 32, 0.01, 0.01, 0.01, 0.04),
+                          bad, 1)[0]
+        assert np.all(np.isfinite(out))
+        assert abs(out.sum() - 1.0) < 1e-5  # only the last event is inside
+        assert rel_l2(out, ref) < 1e-6
+
+
+def test_run_callbacks(gpu_pkg):
+    sc, ev = small(gpu_pkg, n=5000)
+    seen, prog = [], []
+    with gpu_pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        rho = np.ones((64, 64), np.float32)
+        final = rec.run(rho, 2, 7, output=lambda it, img: seen.append((it, img.sum())),
+                        progress=lambda c, f: prog.append((c, f)))
+    its = [s[0] for s in seen]
+    # sensitivity, first block 1-5, end of block 7, end of block 14
+    assert its == [-1, 1, 2, 3, 4, 5, 7, 14]
+    ref = ol.mlem_f32(sc, ev, 14, n_threads=8)
+    assert rel_l2(final, ref[-1]) < 2e-5
+    assert prog[-1] == (14, False)
+
+
+def test_sensitivity(gpu_pkg):
+    sc, _ = small(gpu_pkg, n=10)
+    with gpu_pkg.Reconstruction(sc) as rec:
+        for mode in (0, 1):
+            s = rec.sensitivity(mode)
+            ref = ol.sensitivity_f32(sc, mode)
+            assert np.max(np.abs(s - ref)) < 2e-6
