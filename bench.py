@@ -1,0 +1,149 @@
+"""Benchmark: list-mode MLEM events/s/iteration on one MI355X.
+
+Workload (default): strip scanner R = 0.43 m, L = 0.5 m, sigma_z = 0.01 m,
+sigma_dl = 0.04 m; image 512 x 512 pixels of 1 mm; 2e7 seeded events from a
+six-ellipse phantom; one step = one MLEM iteration over all events with the
+events and the image resident in device memory.
+
+  python bench.py [--steps K] [--warmup W] [--workload NAME] [--gpus 1]
+
+Prints one JSON line.  value = events * K / (time of K iterations).
+roofline: the fused iteration kernel is FP32-VALU bound (no contraction to
+put on MFMA, image and events are a few hundred MB against ~5e12 flop);
+achieved = algorithmic flop per launch (sum_e 15 N_box + 41 N_ell + 50) /
+mean launch time measured with HIP events on the library's stream.
+cpu_baseline: the reference CPU algorithm (the reference's own sources
+compiled by oracle/Makefile when present, else the oracle's restatement) on a
+bounded prefix of the same events, with 16 threads.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("MLEM events/s/iter at 512², 2e7 ev vs host CPU; grad ms; "
+          "rel L2 err vs FP64")
+VALU_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="strip-512px-1mm-2e7ev")
+    p.add_argument("--accumulation", default="fp32",
+                   choices=["fp32", "compensated", "fixed64"])
+    p.add_argument("--cpu-sample", type=float, default=2e6,
+                   help="events in the CPU baseline sample")
+    p.add_argument("--cpu-threads", type=int, default=16)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-grad", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(sc, ev, n_sample, threads):
+    """Reference CPU path on a prefix of the events, one iteration."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    import oracle_lib as ol
+    sample = np.ascontiguousarray(ev[:int(n_sample)])
+    kind = "reference" if ol.have_ref() else "port"
+    fn = ol.ref_mlem if kind == "reference" else ol.mlem_f32
+    fn(sc, sample[:min(len(sample), 20000)], 1, n_threads=threads)  # warm
+    t0 = time.perf_counter()
+    fn(sc, sample, 1, n_threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": len(sample) / dt, "unit": "events/s/iter", "cores": threads,
+            "kind": kind,
+            "sample": f"first {len(sample)} events of the same event list, "
+                      f"1 iteration from rho=1, {threads} OpenMP threads, "
+                      f"{dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
+              "supported (the path would shard events across GPUs and all-reduce "
+              "the image, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from __graft_entry__ import load_package
+    pkg = load_package()
+    name = args.workload
+    n_pix, size, n_ev, _, _, _ = pkg.WORKLOADS[name]
+    sc = pkg.workload_scanner(name)
+    t0 = time.perf_counter()
+    ev = pkg.workload_events(name)
+    t_gen = time.perf_counter() - t0
+    acc = {"fp32": pkg.ACCUM_FP32, "compensated": pkg.ACCUM_COMPENSATED,
+           "fixed64": pkg.ACCUM_FIXED64}[args.accumulation]
+    rec = pkg.Reconstruction(sc, device=0, accumulation=acc)
+    t0 = time.perf_counter()
+    rec.set_events(ev)
+    t_setup = time.perf_counter() - t0
+    st = rec.stats()
+    stream = torch.cuda.ExternalStream(rec.stream())
+    rec(args.warmup)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    kernel_ms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        rec(1)
+        ends[i].record(stream)
+        kernel_ms.append(rec.stats()["last_kernel_ms"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    iter_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    value = n_ev * args.steps / elapsed
+    k_ms = float(np.mean(kernel_ms))
+    achieved = st["flop_alg"] / (k_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "events/s/iter", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": name, "image": f"{n_pix}x{n_pix} @ {size * 1e3:g} mm",
+                   "events": n_ev, "accumulation": args.accumulation,
+                   "device": rec.device_name()},
+        "roofline": {"bound": "valu", "achieved": achieved,
+                     "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / VALU_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_mlem_fused", "kernel_ms": k_ms,
+                     "flop_per_launch": st["flop_alg"]},
+        "iteration_ms": {"median": float(np.median(iter_ms)),
+                         "min": float(np.min(iter_ms))},
+        "footprint": {"box_px_per_event": st["n_box_pixels"] / max(n_ev, 1),
+                      "ellipse_px_per_event": st["n_ellipse_pixels"] / max(n_ev, 1)},
+        "setup_s": {"generate": t_gen, "upload_and_bin": t_setup},
+    }
+    if not args.no_grad:
+        torch.cuda.synchronize()
+        g0 = time.perf_counter()
+        rec.forward()
+        rec.backward()
+        torch.cuda.synchronize()
+        out["grad_ms"] = (time.perf_counter() - g0) * 1e3
+    rec.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, ev, args.cpu_sample, args.cpu_threads)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

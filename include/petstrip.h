@@ -81,10 +81,17 @@ enum {
   PETSTRIP_UPDATE_SENSITIVITY_DENOMINATOR = 1
 };
 
+/* Which kernels run an iteration. */
+enum {
+  PETSTRIP_PATH_AUTO = 0,      /* fused binned kernel when it applies       */
+  PETSTRIP_PATH_PER_EVENT = 1  /* one wave per event, global atomics        */
+};
+
 typedef struct petstrip_options {
   int32_t accumulation; /* PETSTRIP_ACCUM_*                                */
   int32_t update;       /* PETSTRIP_UPDATE_*                               */
-  int32_t reserved[6];  /* zero                                            */
+  int32_t path;         /* PETSTRIP_PATH_*                                 */
+  int32_t reserved[5];  /* zero                                            */
 } petstrip_options;
 
 typedef struct petstrip_handle petstrip_handle;
@@ -97,6 +104,10 @@ typedef struct petstrip_stats {
   int64_t n_zero_denominator;/* events whose denominator was not > 0     */
   double flop_alg;           /* sum_e 15 N_box + 41 N_ell + 50           */
   double last_iteration_ms;  /* device time of the last iteration        */
+  double last_kernel_ms;     /* device time of its event kernel(s)       */
+  int32_t fused;             /* 1 if iterations use the binned kernel    */
+  int32_t n_work_items;      /* workgroups of the binned kernel          */
+  int64_t lds_bytes;         /* LDS per workgroup of the binned kernel   */
 } petstrip_stats;
 
 /* Called after an output is ready.  iteration >= 1 for images, -1 for the

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/petstrip.h"
+#include "strip_fused.h"
 #include "strip_geometry.h"
 #include "strip_kernels.h"
 
@@ -103,10 +104,14 @@ void dfree(T*& p) {
 
 }  // namespace
 
+namespace petstrip {
+int set_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace petstrip
+
 struct petstrip_handle {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
   petstrip_scanner scanner{};
   petstrip_options opts{};
   StripConst sg{};  // constants the support (ellipse, box) was built with
@@ -127,6 +132,19 @@ struct petstrip_handle {
   double* d_scalars = nullptr;
   bool lambda_valid = false;
   petstrip_stats stats{};
+  // spatially binned events for the fused iteration
+  size_t lds_limit = 65536;
+  int32_t* d_bin = nullptr;
+  int32_t* d_count = nullptr;
+  EventGeom* d_sgeom = nullptr;
+  int32_t* d_sidx = nullptr;
+  petstrip::WorkItem* d_items = nullptr;
+  size_t cap_items = 0;
+  int n_items = 0;
+  int max_rows = 1;
+  size_t lds_bytes = 0;
+  bool fused_ok = false;
+  void* d_acc = nullptr;  // W*H accumulator, 8 bytes per pixel
 };
 
 namespace {
@@ -145,6 +163,9 @@ int reserve_events(petstrip_handle* h, int64_t n) {
   dfree(h->d_inv);
   dfree(h->d_nbox);
   dfree(h->d_nell);
+  dfree(h->d_bin);
+  dfree(h->d_sgeom);
+  dfree(h->d_sidx);
   h->cap_events = 0;
   size_t c = (size_t)std::max<int64_t>(n, 1);
   int rc;
@@ -154,7 +175,95 @@ int reserve_events(petstrip_handle* h, int64_t n) {
   if ((rc = dalloc(&h->d_inv, c))) return rc;
   if ((rc = dalloc(&h->d_nbox, c))) return rc;
   if ((rc = dalloc(&h->d_nell, c))) return rc;
+  if ((rc = dalloc(&h->d_bin, c))) return rc;
+  if ((rc = dalloc(&h->d_sgeom, c))) return rc;
+  if ((rc = dalloc(&h->d_sidx, c))) return rc;
   h->cap_events = c;
+  return PETSTRIP_OK;
+}
+
+constexpr int kBin = 16;          // spatial bin edge in pixels
+constexpr int kChunk = 1024;      // events per work item
+
+// Along a box row the reference's float ellipse test has no holes when the
+// quadratic's column-to-column curvature C*pw^2 dominates the rounding of q
+// (|dq/dz| * ulp(z) + ulp(9)); then each row's ellipse pixels are one
+// interval, which the fused kernel relies on.
+bool rows_are_intervals(const StripConst& s) {
+  double C = 2.0 * s.isz;
+  double pw = s.pw;
+  double zmax = std::max(std::fabs((double)s.tl_z), std::fabs((double)s.tl_z) + s.W * pw) + 1.0;
+  double ulp_z = zmax * 1.2e-7;
+  double dq = 2.0 * std::sqrt(9.0 * C) + 1.0;
+  double err = dq * ulp_z + 9.0 * 2.4e-7;
+  return C * pw * pw > 50.0 * err;
+}
+
+// Sorts events into kBin x kBin bins of their box centre, cuts each bin into
+// chunks of kChunk events and sizes each chunk's LDS window.
+int prepare_fused(petstrip_handle* h) {
+  h->fused_ok = false;
+  h->n_items = 0;
+  const int64_t n = h->n_events;
+  const int W = h->sg.W, H = h->sg.H;
+  const int nbx = (W + kBin - 1) / kBin, nby = (H + kBin - 1) / kBin;
+  const int nb = nbx * nby;
+  dfree(h->d_count);
+  int rc;
+  if ((rc = dalloc(&h->d_count, (size_t)nb * 2))) return rc;
+  HIP_TRY(hipMemsetAsync(h->d_count, 0, (size_t)nb * 2 * sizeof(int32_t), h->stream));
+  HIP_TRY(petstrip::launch_bin_count(h->stream, h->d_geom, n, kBin, kBin, nbx,
+                                     h->d_bin, h->d_count));
+  std::vector<int32_t> cnt((size_t)nb), off((size_t)nb);
+  HIP_TRY(hipMemcpyAsync(cnt.data(), h->d_count, nb * sizeof(int32_t),
+                         hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  int64_t acc = 0;
+  std::vector<petstrip::WorkItem> items;
+  for (int b = 0; b < nb; ++b) {
+    off[(size_t)b] = (int32_t)acc;
+    for (int64_t s0 = acc; s0 < acc + cnt[(size_t)b]; s0 += kChunk) {
+      petstrip::WorkItem it{};
+      it.begin = (int32_t)s0;
+      it.end = (int32_t)std::min<int64_t>(s0 + kChunk, acc + cnt[(size_t)b]);
+      items.push_back(it);
+    }
+    acc += cnt[(size_t)b];
+  }
+  int32_t* d_cursor = h->d_count + nb;
+  HIP_TRY(hipMemcpyAsync(d_cursor, off.data(), nb * sizeof(int32_t),
+                         hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(petstrip::launch_bin_scatter(h->stream, h->d_geom, n, h->d_bin, d_cursor,
+                                       h->d_sgeom, h->d_sidx));
+  h->n_items = (int)items.size();
+  if (items.size() > h->cap_items) {
+    dfree(h->d_items);
+    if ((rc = dalloc(&h->d_items, items.size()))) return rc;
+    h->cap_items = items.size();
+  }
+  if (!items.empty()) {
+    HIP_TRY(hipMemcpyAsync(h->d_items, items.data(),
+                           items.size() * sizeof(petstrip::WorkItem),
+                           hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(petstrip::launch_item_windows(h->stream, h->d_sgeom, h->d_items,
+                                          h->n_items));
+    HIP_TRY(hipMemcpyAsync(items.data(), h->d_items,
+                           items.size() * sizeof(petstrip::WorkItem),
+                           hipMemcpyDeviceToHost, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  int max_px = 1, max_rows = 1;
+  for (auto& it : items) {
+    max_px = std::max(max_px, it.ww * it.wh);
+    max_rows = std::max(max_rows, it.max_rows);
+  }
+  h->max_rows = max_rows;
+  int mode = h->opts.accumulation == PETSTRIP_ACCUM_FIXED64 ? petstrip::ACC_LDS_FIXED64
+                                                            : petstrip::ACC_LDS_F32;
+  h->lds_bytes = petstrip::fused_lds_bytes(max_px, mode, max_rows);
+  h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
+                h->opts.update == PETSTRIP_UPDATE_CPU &&
+                h->opts.path == PETSTRIP_PATH_AUTO;
   return PETSTRIP_OK;
 }
 
@@ -183,7 +292,7 @@ int prepare_events(petstrip_handle* h) {
   h->stats.n_ellipse_pixels = se;
   h->stats.flop_alg = 15.0 * (double)sb + 41.0 * (double)se + 50.0 * (double)n;
   h->lambda_valid = false;
-  return PETSTRIP_OK;
+  return prepare_fused(h);
 }
 
 const float* denominator_weight(petstrip_handle* h) {
@@ -195,12 +304,34 @@ const float* denominator_weight(petstrip_handle* h) {
 // reference form out[p] += K rho[p] inv_e).
 int one_iteration(petstrip_handle* h) {
   const int64_t n = h->n_events;
+  if (h->fused_ok) {
+    const int accm = h->opts.accumulation;
+    const int kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
+                     : accm == PETSTRIP_ACCUM_COMPENSATED ? petstrip::GLOBAL_F64
+                                                           : petstrip::GLOBAL_F32;
+    const int mode = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::ACC_LDS_FIXED64
+                                                    : petstrip::ACC_LDS_F32;
+    HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
+    HIP_TRY(hipEventRecord(h->evk0, h->stream));
+    HIP_TRY(petstrip::launch_mlem_fused(
+        h->stream, h->sg, h->sk, h->d_sgeom, h->d_sidx, h->d_items, h->n_items,
+        h->lds_bytes, h->d_rho, h->d_lambda, h->d_acc, kind, mode, h->max_rows,
+        accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0));
+    HIP_TRY(hipEventRecord(h->evk1, h->stream));
+    HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, kind, h->d_out,
+                                      (int64_t)h->n_pix));
+    std::swap(h->d_rho, h->d_out);
+    h->lambda_valid = false;
+    return PETSTRIP_OK;
+  }
+  HIP_TRY(hipEventRecord(h->evk0, h->stream));
   HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
                                        h->d_rho, denominator_weight(h),
                                        h->d_lambda, h->d_inv));
   HIP_TRY(petstrip::launch_fill(h->stream, h->d_out, 0.0f, (int64_t)h->n_pix));
   HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n,
                                        h->d_rho, h->d_inv, h->d_out));
+  HIP_TRY(hipEventRecord(h->evk1, h->stream));
   std::swap(h->d_rho, h->d_out);
   h->lambda_valid = false;
   return PETSTRIP_OK;
@@ -229,7 +360,8 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   int rc = check_scanner(scanner);
   if (rc) return rc;
   if (opts && (opts->accumulation < 0 || opts->accumulation > 2 ||
-               opts->update < 0 || opts->update > 1))
+               opts->update < 0 || opts->update > 1 || opts->path < 0 ||
+      opts->path > 1))
     return fail(PETSTRIP_EINVAL, "invalid options");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -256,12 +388,24 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   h->name = prop.name;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipStreamCreate failed"));
-  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+      hipEventCreate(&h->evk0) != hipSuccess || hipEventCreate(&h->evk1) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipEventCreate failed"));
   if ((rc = dalloc(&h->d_rho, h->n_pix))) return bail(rc);
   if ((rc = dalloc(&h->d_out, h->n_pix))) return bail(rc);
   if ((rc = dalloc(&h->d_sens, h->n_pix))) return bail(rc);
   if ((rc = dalloc(&h->d_scalars, 8))) return bail(rc);
+  if ((rc = dalloc((double**)&h->d_acc, h->n_pix))) return bail(rc);
+  if (hipMemsetAsync(h->d_acc, 0, h->n_pix * sizeof(double), h->stream) != hipSuccess)
+    return bail(fail(PETSTRIP_EDEVICE, "hipMemset failed"));
+  {
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin,
+                              device) == hipSuccess && optin > 0)
+      h->lds_limit = (size_t)optin;
+    else
+      h->lds_limit = prop.sharedMemPerBlock;
+  }
   if ((rc = reserve_events(h, 1))) return bail(rc);
   // rho = 1 (reconstruction.h:59), 5-point sensitivity for the --gpu variant
   if (petstrip::launch_fill(h->stream, h->d_rho, 1.0f, (int64_t)h->n_pix) ||
@@ -286,7 +430,16 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_out);
   dfree(h->d_sens);
   dfree(h->d_scalars);
+  dfree(h->d_bin);
+  dfree(h->d_count);
+  dfree(h->d_sgeom);
+  dfree(h->d_sidx);
+  dfree(h->d_items);
+  if (h->d_acc) (void)hipFree(h->d_acc);
+  h->d_acc = nullptr;
   if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->evk0) (void)hipEventDestroy(h->evk0);
+  if (h->evk1) (void)hipEventDestroy(h->evk1);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -312,7 +465,9 @@ int petstrip_set_options(petstrip_handle* h, const petstrip_options* opts) {
     return fail(PETSTRIP_EINVAL, "invalid options");
   h->opts = *opts;
   h->lambda_valid = false;
-  return PETSTRIP_OK;
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  return prepare_fused(h);
 }
 
 int petstrip_set_events(petstrip_handle* h, const float* responses, int64_t n) {
@@ -398,6 +553,8 @@ int petstrip_iterate(petstrip_handle* h, int n_iterations) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.last_iteration_ms = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, h->evk0, h->evk1));
+    h->stats.last_kernel_ms = ms;
   }
   return PETSTRIP_OK;
 }
@@ -458,6 +615,9 @@ int petstrip_sensitivity(petstrip_handle* h, int mode, float* out) {
 int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
   if (!h || !out) return fail(PETSTRIP_EINVAL, "bad argument");
   *out = h->stats;
+  out->fused = h->fused_ok ? 1 : 0;
+  out->n_work_items = h->n_items;
+  out->lds_bytes = (int64_t)h->lds_bytes;
   return PETSTRIP_OK;
 }
 

@@ -23,6 +23,10 @@
 
 #include "../../include/petstrip.h"
 
+namespace petstrip {
+int set_error(int code, const char* msg);
+}
+
 namespace {
 
 struct Philox {
@@ -70,8 +74,9 @@ extern "C" int petstrip_phantom_events(const petstrip_scanner* sc,
                                        const double* regions, int32_t n_regions,
                                        uint64_t seed, int64_t n, float* out,
                                        int n_threads) {
+  using petstrip::set_error;
   if (!sc || n < 0 || (n > 0 && !out) || n_regions <= 0 || !regions)
-    return PETSTRIP_EINVAL;
+    return set_error(PETSTRIP_EINVAL, "bad phantom arguments");
   std::vector<Region> reg((size_t)n_regions);
   double total = 0;
   for (int i = 0; i < n_regions; ++i) {
@@ -81,7 +86,9 @@ extern "C" int petstrip_phantom_events(const petstrip_scanner* sc,
     g.y = r[1];
     g.a = r[2];
     g.b = r[3];
-    if (!(g.a > 0) || !(g.b > 0) || !(r[5] >= 0)) return PETSTRIP_EINVAL;
+    if (!(g.a > 0) || !(g.b > 0) || !(r[5] >= 0))
+      return set_error(PETSTRIP_EINVAL,
+                       "phantom region needs positive axes and intensity >= 0");
     double ang = r[4] * M_PI / 180;
     g.s = sin(ang);
     g.c = cos(ang);
@@ -91,7 +98,8 @@ extern "C" int petstrip_phantom_events(const petstrip_scanner* sc,
     total += r[5] * M_PI * g.a * g.b;  // weight = intensity * area
     g.cdf = total;
   }
-  if (!(total > 0)) return PETSTRIP_EINVAL;
+  if (!(total > 0))
+    return set_error(PETSTRIP_EINVAL, "phantom has zero total intensity");
   for (auto& g : reg) g.cdf /= total;
   const double R = sc->radius, half_L = sc->scintillator_length / 2;
   const double sz = sc->sigma_z, sdl = sc->sigma_dl;
@@ -153,6 +161,8 @@ extern "C" int petstrip_phantom_events(const petstrip_scanner* sc,
   work(0);
   for (auto& t : th) t.join();
   for (int s : status)
-    if (s) return PETSTRIP_ESTATE;
+    if (s)
+      return set_error(PETSTRIP_ESTATE,
+                       "no detected emission after 2^24 draws in a slot");
   return PETSTRIP_OK;
 }

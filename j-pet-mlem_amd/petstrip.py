@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libpetstrip_hip.so")
 
 ACCUM_FP32, ACCUM_COMPENSATED, ACCUM_FIXED64 = 0, 1, 2
 UPDATE_CPU, UPDATE_SENSITIVITY_DENOMINATOR = 0, 1
+PATH_AUTO, PATH_PER_EVENT = 0, 1
 
 
 class PetstripError(RuntimeError):
@@ -41,7 +42,7 @@ class CScanner(C.Structure):
 
 class COptions(C.Structure):
     _fields_ = [("accumulation", C.c_int32), ("update", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("path", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class CStats(C.Structure):
@@ -49,6 +50,8 @@ class CStats(C.Structure):
         ("n_events", C.c_int64), ("n_box_pixels", C.c_int64),
         ("n_ellipse_pixels", C.c_int64), ("n_zero_denominator", C.c_int64),
         ("flop_alg", C.c_double), ("last_iteration_ms", C.c_double),
+        ("last_kernel_ms", C.c_double), ("fused", C.c_int32),
+        ("n_work_items", C.c_int32), ("lds_bytes", C.c_int64),
     ]
 
 
@@ -178,11 +181,11 @@ class Reconstruction:
     """
 
     def __init__(self, scanner, device=0, accumulation=ACCUM_FP32,
-                 update=UPDATE_CPU):
+                 update=UPDATE_CPU, path=PATH_AUTO):
         self.lib = load_library()
         self.scanner = scanner
         self._h = C.c_void_p()
-        opts = COptions(accumulation, update)
+        opts = COptions(accumulation, update, path)
         sc = scanner.c()
         _check(self.lib.petstrip_create(C.byref(sc), device, C.byref(opts),
                                         C.byref(self._h)))
@@ -219,8 +222,8 @@ class Reconstruction:
         _check(self.lib.petstrip_stream(self._h, C.byref(s)))
         return s.value
 
-    def set_options(self, accumulation=ACCUM_FP32, update=UPDATE_CPU):
-        o = COptions(accumulation, update)
+    def set_options(self, accumulation=ACCUM_FP32, update=UPDATE_CPU, path=PATH_AUTO):
+        o = COptions(accumulation, update, path)
         _check(self.lib.petstrip_set_options(self._h, C.byref(o)))
 
     def set_events(self, responses):

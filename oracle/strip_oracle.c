@@ -513,6 +513,22 @@ void oracle_pixel_center_f32(const oracle_scanner* sc, int32_t x, int32_t y,
   out[1] = fmaf(-(float)y, s.ph, s.tl_y_half_h);
 }
 
+void oracle_pixel_at_f64(const oracle_scanner* sc, double z, double y,
+                         int32_t out[2]) {
+  s64 s;
+  make_s64(sc, &s, sc->sigma_z, sc->sigma_dl);
+  out[0] = wrap_i16(cvt_i32_d((z - s.tl_z) / s.pw));
+  out[1] = wrap_i16(cvt_i32_d((s.tl_y - y) / s.ph));
+}
+
+void oracle_pixel_center_f64(const oracle_scanner* sc, int32_t x, int32_t y,
+                             double out[2]) {
+  s64 s;
+  make_s64(sc, &s, sc->sigma_z, sc->sigma_dl);
+  out[0] = fma((double)x, s.pw, s.tl_z_half_w);
+  out[1] = fma(-(double)y, s.ph, s.tl_y_half_h);
+}
+
 /* Scanner::sensitivity (scanner.h:147-171). */
 static float sens_point_f(const s32* s, float z, float y) {
   float L_plus = s->half_L + z;

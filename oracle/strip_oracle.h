@@ -99,6 +99,10 @@ void oracle_pixel_at_f32(const oracle_scanner* s, float z, float y,
                          int32_t out[2]);
 void oracle_pixel_center_f32(const oracle_scanner* s, int32_t x, int32_t y,
                              float out[2]);
+void oracle_pixel_at_f64(const oracle_scanner* s, double z, double y,
+                         int32_t out[2]);
+void oracle_pixel_center_f64(const oracle_scanner* s, int32_t x, int32_t y,
+                             double out[2]);
 
 /* Sensitivity image: mode 0 = Scanner::sensitivity at the pixel centre
  * (reconstruction.h:67-73), mode 1 = Scanner::pixel_sensitivity 5-point

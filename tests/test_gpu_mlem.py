@@ -56,11 +56,23 @@ def test_denominators(gpu_pkg):
     assert err.max() < 2e-6
 
 
+MODES = [("auto", "fp32"), ("auto", "compensated"), ("auto", "fixed64"),
+         ("per_event", "fp32")]
+
+
+def opts(pkg, path, acc):
+    return dict(path={"auto": pkg.PATH_AUTO, "per_event": pkg.PATH_PER_EVENT}[path],
+                accumulation={"fp32": pkg.ACCUM_FP32,
+                              "compensated": pkg.ACCUM_COMPENSATED,
+                              "fixed64": pkg.ACCUM_FIXED64}[acc])
+
+
+@pytest.mark.parametrize("path,acc", MODES)
 @pytest.mark.parametrize("n_pix,size", [(64, 0.008), (41, 0.0125), (100, 0.005)])
-def test_mlem_small(gpu_pkg, n_pix, size):
+def test_mlem_small(gpu_pkg, n_pix, size, path, acc):
     sc, ev = small(gpu_pkg, n_pix=n_pix, size=size)
-    print(f"case: {n_pix}x{n_pix} @ {size} m, {len(ev)} events", flush=True)
-    with gpu_pkg.Reconstruction(sc) as rec:
+    print(f"case: {n_pix}x{n_pix} @ {size} m, {len(ev)} events, {path}/{acc}", flush=True)
+    with gpu_pkg.Reconstruction(sc, **opts(gpu_pkg, path, acc)) as rec:
         rec.set_events(ev)
         outs = []
         for _ in range(5):
@@ -140,3 +152,52 @@ def test_sensitivity(gpu_pkg):
             s = rec.sensitivity(mode)
             ref = ol.sensitivity_f32(sc, mode)
             assert np.max(np.abs(s - ref)) < 2e-6
+
+
+def test_fixed64_bit_reproducible(gpu_pkg):
+    sc, ev = small(gpu_pkg, n=30000)
+    outs = []
+    for _ in range(2):
+        with gpu_pkg.Reconstruction(sc, accumulation=gpu_pkg.ACCUM_FIXED64) as rec:
+            rec.set_events(ev)
+            rec(3)
+            outs.append(rec.rho)
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_lambda_of_fused_iteration(gpu_pkg):
+    """The fused iteration's denominators equal the forward projector's."""
+    sc, ev = small(gpu_pkg, n=20000)
+    with gpu_pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        rec(2)
+        lam = rec.forward()
+    ref = ol.denominators_f32(sc, ev, ol.mlem_f32(sc, ev, 2, n_threads=8)[-1])
+    pos = ref > 0
+    err = np.abs(lam[pos] - ref[pos]) / ref[pos]
+    assert err.max() < 2e-5
+
+
+@pytest.mark.parametrize("acc", ["fp32", "fixed64"])
+def test_workload_160px_1e6(gpu_pkg, acc):
+    """Strip scanner R=0.43 m, L=0.5 m, 160x160 pixels of 4 mm, 1e6 events
+    from the three-ellipse phantom, 10 iterations: every iteration within
+    1e-5 relative L2 of the FP32 oracle and 1e-4 of the FP64 oracle."""
+    pkg = gpu_pkg
+    name = "strip-160px-4mm-1e6ev"
+    sc = pkg.workload_scanner(name)
+    ev = pkg.workload_events(name)
+    print(f"case: {name} {acc}", flush=True)
+    with pkg.Reconstruction(sc, **opts(pkg, "auto", acc)) as rec:
+        rec.set_events(ev)
+        outs = []
+        for _ in range(10):
+            rec(1)
+            outs.append(rec.rho)
+    ref32 = ol.mlem_f32(sc, ev, 10, n_threads=16)
+    ref64 = ol.mlem_f64(sc, ev, 10, n_threads=16)
+    for i in range(10):
+        e32, e64 = rel_l2(outs[i], ref32[i]), rel_l2(outs[i], ref64[i])
+        print(f"iter {i + 1}: rel L2 vs f32 {e32:.2e} vs f64 {e64:.2e}", flush=True)
+        assert e32 < 1e-5
+        assert e64 < 1e-4
