@@ -1,0 +1,142 @@
+"""The 2d_strip_reconstruction command line (j-pet-mlem_amd/cli): argument
+handling on the CPU, end-to-end runs and output files on the GPU.
+
+File conventions checked against the reference's writers:
+src/util/png_writer.h:31-54 (inverted 8-bit PNG), nrrd_writer.h:18-27,
+cmdline_types.h:35-55 (index suffix), reconstruction_cmd.cpp:232-291.
+"""
+import os
+import struct
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "j-pet-mlem_amd", "cli", "2d_strip_reconstruction")
+
+needs_cli = pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+
+
+def run(args, **kw):
+    return subprocess.run([CLI] + args, capture_output=True, text=True, timeout=120, **kw)
+
+
+def read_png(path):
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w = 8, b"", None
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        typ = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + n]
+        crc = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0]
+        assert zlib.crc32(typ + body) & 0xffffffff == crc
+        if typ == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert depth == 8 and ctype == 0
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = zlib.decompress(idat)
+    rows = np.frombuffer(raw, np.uint8).reshape(h, w + 1)
+    assert np.all(rows[:, 0] == 0)
+    return rows[:, 1:]
+
+
+@needs_cli
+def test_usage_and_errors():
+    r = run([])
+    assert r.returncode == 0 and "usage" in r.stderr
+    r = run(["/nonexistent/events.bin", "-i", "1"])
+    assert r.returncode == 1 and "cannot open" in r.stderr
+    r = run(["--bogus-option", "x.bin"])
+    assert r.returncode == 1 and "undefined option" in r.stderr
+    r = run(["-n", "12x", "x.bin"])
+    assert r.returncode == 1
+
+
+@needs_cli
+def test_no_device_is_an_error_not_a_crash(tmp_path, pkg):
+    if pkg.device_count() > 0:
+        pytest.skip("a device is visible")
+    ev = tmp_path / "ev.bin"
+    np.zeros((4, 3), np.float32).tofile(ev)
+    r = run([str(ev), "-i", "1", "-n", "16", "-p", "0.01"])
+    assert r.returncode == 1 and "device" in r.stderr
+
+
+def make_input(tmp_path, pkg, n=20000, txt=False):
+    sc = pkg.Scanner(0.43, 0.5, 64, 64, 0.008, 0.008, 0.01, 0.04)
+    ev = pkg.phantom_events(sc, pkg.workloads.PHANTOM_3, n, seed=3)
+    if txt:
+        path = tmp_path / "p_strip.txt"
+        np.savetxt(path, ev, fmt="%.9g")
+    else:
+        path = tmp_path / "p_strip.bin"
+        ev.tofile(path)
+    # accompanying config, as 2d_strip_phantom writes it
+    (tmp_path / "p_strip.cfg").write_text(
+        "--radius=0.43\n--length=0.5\n--s-pixel=0.008\n--n-z-pixels=64\n"
+        "--n-y-pixels=64\n# comment\n--s-z=0.01\n--s-dl=0.04\n")
+    return sc, ev, path
+
+
+@needs_cli
+@pytest.mark.gpu
+def test_gpu_branch_outputs(tmp_path, gpu_pkg):
+    sc, ev, path = make_input(tmp_path, gpu_pkg)
+    out = tmp_path / "r.bin"
+    r = run([str(path), "-G", "-i", "2", "-I", "3", "-o", str(out), "-v", "-n", "64",
+             "-p", "0.008", "-r", "0.43", "-l", "0.5", "--s-z", "0.01", "--s-dl", "0.04"])
+    assert r.returncode == 0, r.stderr
+    assert "# read 20000 responsess" in r.stderr and "# device:" in r.stderr
+    names = sorted(os.listdir(tmp_path))
+    for stem in ["r_sensitivity", "r_1", "r_2", "r_3", "r_6"]:
+        for e in [".png", ".bin", ".nrrd"]:
+            assert stem + e in names, (stem, e, names)
+    assert "r_4.bin" not in names  # first block stops outputs before its end
+    img = np.fromfile(tmp_path / "r_6.bin", np.float32).reshape(64, 64)
+    ref = ol.mlem_f32(sc, ev, 6, n_threads=8)[-1]
+    assert np.linalg.norm(img - ref) / np.linalg.norm(ref) < 1e-5
+    nrrd = (tmp_path / "r_6.nrrd").read_text().splitlines()
+    assert nrrd == ["NRRD0001", "type: float", "endian: little", "dimension: 2",
+                    "sizes: 64 64", "data file: ./r_6.bin", "encoding: raw"]
+    png = read_png(tmp_path / "r_6.png")
+    want = np.maximum(0.0, 255.0 - (255.0 / img.max()) * img.astype(np.float64))
+    assert np.max(np.abs(png.astype(int) - want.astype(np.uint8).astype(int))) <= 1
+
+
+@needs_cli
+@pytest.mark.gpu
+def test_cpu_branch_outputs_and_config(tmp_path, gpu_pkg):
+    sc, ev, path = make_input(tmp_path, gpu_pkg, txt=True)
+    out = tmp_path / "o.txt"
+    # geometry comes from the accompanying p_strip.cfg
+    r = run([str(path), "-i", "2", "-I", "5", "-o", str(out)])
+    assert r.returncode == 0, r.stderr
+    names = sorted(os.listdir(tmp_path))
+    assert "o_sensitivity.png" in names and "o_05.txt" in names and "o_10.txt" in names
+    tup = np.loadtxt(tmp_path / "o_10.txt")
+    img = np.zeros((64, 64))
+    img[tup[:, 1].astype(int), tup[:, 0].astype(int)] = tup[:, 2]
+    ev32 = np.loadtxt(path, dtype=np.float64).astype(np.float32)
+    ref = ol.mlem_f32(sc, ev32, 10, n_threads=8)[-1]
+    assert np.linalg.norm(img - ref) / np.linalg.norm(ref) < 1e-5
+
+
+@needs_cli
+@pytest.mark.gpu
+def test_rho_resume(tmp_path, gpu_pkg):
+    sc, ev, path = make_input(tmp_path, gpu_pkg)
+    r = run([str(path), "-G", "-i", "1", "-I", "2", "-o", str(tmp_path / "a.bin")])
+    assert r.returncode == 0, r.stderr
+    r = run([str(path), "-G", "-i", "1", "-I", "1", "--rho", str(tmp_path / "a_2.bin"),
+             "-o", str(tmp_path / "b.bin")])
+    assert r.returncode == 0, r.stderr
+    img = np.fromfile(tmp_path / "b_1.bin", np.float32).reshape(64, 64)
+    ref = ol.mlem_f32(sc, ev, 3, n_threads=8)[-1]
+    assert np.linalg.norm(img - ref) / np.linalg.norm(ref) < 1e-5
