@@ -19,7 +19,8 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PETSTRIP_ARCH", "gfx950")
 
-SOURCES = ["strip_kernels.hip", "strip_fused.hip", "petstrip_capi.hip", "phantom.hip"]
+SOURCES = ["strip_kernels.hip", "strip_bins.hip", "strip_fused.hip", "petstrip_capi.hip",
+           "phantom.hip"]
 
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",

@@ -143,6 +143,7 @@ struct petstrip_handle {
   int n_items = 0;
   int max_rows = 1;
   size_t lds_bytes = 0;
+  int fused_waves = 4;
   bool fused_ok = false;
   void* d_acc = nullptr;  // W*H accumulator, 8 bytes per pixel
 };
@@ -254,13 +255,26 @@ int prepare_fused(petstrip_handle* h) {
   HIP_TRY(hipStreamSynchronize(h->stream));
   int max_px = 1, max_rows = 1;
   for (auto& it : items) {
-    max_px = std::max(max_px, it.ww * it.wh);
+    max_px = std::max(max_px, (it.ww | 1) * it.wh);  // odd LDS pitch
     max_rows = std::max(max_rows, it.max_rows);
   }
   h->max_rows = max_rows;
-  int mode = h->opts.accumulation == PETSTRIP_ACCUM_FIXED64 ? petstrip::ACC_LDS_FIXED64
-                                                            : petstrip::ACC_LDS_F32;
-  h->lds_bytes = petstrip::fused_lds_bytes(max_px, mode, max_rows);
+  // every mode accumulates the LDS window in exact 2^-40 fixed point:
+  // ds_add_u64 runs at several times the rate of ds_add_f32 on gfx950, and
+  // the window sums become exact; the mode only picks the global format
+  const int mode = petstrip::ACC_LDS_FIXED64;
+  // the most waves per workgroup whose row tables still fit next to the
+  // windows: a workgroup holds one CU's LDS, so its waves set the occupancy
+  h->fused_waves = 4;
+  h->lds_bytes = petstrip::fused_lds_bytes(max_px, mode, max_rows, 4);
+  for (int waves : {16, 8}) {
+    size_t b = petstrip::fused_lds_bytes(max_px, mode, max_rows, waves);
+    if (b <= h->lds_limit) {
+      h->fused_waves = waves;
+      h->lds_bytes = b;
+      break;
+    }
+  }
   h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
                 h->opts.update == PETSTRIP_UPDATE_CPU &&
                 h->opts.path == PETSTRIP_PATH_AUTO;
@@ -309,14 +323,13 @@ int one_iteration(petstrip_handle* h) {
     const int kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
                      : accm == PETSTRIP_ACCUM_COMPENSATED ? petstrip::GLOBAL_F64
                                                            : petstrip::GLOBAL_F32;
-    const int mode = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::ACC_LDS_FIXED64
-                                                    : petstrip::ACC_LDS_F32;
+    const int mode = petstrip::ACC_LDS_FIXED64;
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
     HIP_TRY(petstrip::launch_mlem_fused(
         h->stream, h->sg, h->sk, h->d_sgeom, h->d_sidx, h->d_items, h->n_items,
         h->lds_bytes, h->d_rho, h->d_lambda, h->d_acc, kind, mode, h->max_rows,
-        accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0));
+        accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, h->fused_waves));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, kind, h->d_out,
                                       (int64_t)h->n_pix));

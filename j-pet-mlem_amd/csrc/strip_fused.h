@@ -28,14 +28,15 @@ hipError_t launch_bin_scatter(hipStream_t st, const EventGeom* geom, int64_t n,
                               EventGeom* sgeom, int32_t* sidx);
 hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
                                WorkItem* items, int n_items);
-size_t fused_lds_bytes(int max_window_px, int acc_mode, int max_rows);
+// LDS bytes per workgroup of `waves` waves (4, 8 or 16).
+size_t fused_lds_bytes(int max_window_px, int acc_mode, int max_rows, int waves);
 hipError_t launch_mlem_fused(hipStream_t st, const StripConst& sg,
                              const StripConst& sk, const EventGeom* sgeom,
                              const int32_t* sidx, const WorkItem* items,
                              int n_items, size_t lds_bytes, const float* rho,
                              float* lambda, void* acc_global,
                              int acc_global_kind, int acc_mode, int max_rows,
-                             int compensated);
+                             int compensated, int waves);
 hipError_t launch_finalize(hipStream_t st, void* acc, int kind, float* out,
                            int64_t n);
 
