@@ -44,7 +44,24 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=16)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-grad", action="store_true")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="also run one device copy of MIB MiB (for HBM counter "
+                        "calibration under rocprofv3 --pmc)")
     return p.parse_args()
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the fused kernel from the committed
+    rocprofv3 --pmc summary (profiles/traffic.json), if it covers this
+    workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = t.get(workload)
+    return None if entry is None else entry.get("bytes_per_launch")
 
 
 def cpu_baseline(sc, ev, n_sample, threads):
@@ -122,7 +139,8 @@ def main():
                    "device": rec.device_name()},
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / VALU_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / VALU_PEAK_TFLOPS,
+                     "traffic": measured_traffic(name),
                      "kernel": "k_mlem_fused", "kernel_ms": k_ms,
                      "flop_per_launch": st["flop_alg"]},
         "iteration_ms": {"median": float(np.median(iter_ms)),
@@ -139,6 +157,15 @@ def main():
         torch.cuda.synchronize()
         out["grad_ms"] = (time.perf_counter() - g0) * 1e3
     rec.close()
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = 2 * n * 4
+        del a, b
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, ev, args.cpu_sample, args.cpu_threads)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
