@@ -150,12 +150,19 @@ def main():
         "setup_s": {"generate": t_gen, "upload_and_bin": t_setup},
     }
     if not args.no_grad:
-        torch.cuda.synchronize()
-        g0 = time.perf_counter()
-        rec.forward()
+        # full image + (sigma_z, sigma_dl) gradient of the Poisson list-mode
+        # log-likelihood at the current image: one forward pass (lambda, sigma
+        # derivative) and one adjoint pass, kernels recomputed; includes the
+        # copy of the W*H gradient to the host.  Median of 3.
         rec.backward()
-        torch.cuda.synchronize()
-        out["grad_ms"] = (time.perf_counter() - g0) * 1e3
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            g0 = time.perf_counter()
+            rec.backward()
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - g0) * 1e3)
+        out["grad_ms"] = float(np.median(times))
     rec.close()
     if args.calibrate_copy > 0:
         n = args.calibrate_copy * (1 << 20) // 4

@@ -143,7 +143,9 @@ struct petstrip_handle {
   int n_items = 0;
   int max_rows = 1;
   size_t lds_bytes = 0;
-  int fused_waves = 4;
+  int fused_waves = 16;
+  int max_px = 1;
+  int* d_flag = nullptr;  // device overflow flag of the fixed-point image
   bool fused_ok = false;
   void* d_acc = nullptr;  // W*H accumulator, 8 bytes per pixel
 };
@@ -259,22 +261,13 @@ int prepare_fused(petstrip_handle* h) {
     max_rows = std::max(max_rows, it.max_rows);
   }
   h->max_rows = max_rows;
-  // every mode accumulates the LDS window in exact 2^-40 fixed point:
-  // ds_add_u64 runs at several times the rate of ds_add_f32 on gfx950, and
-  // the window sums become exact; the mode only picks the global format
-  const int mode = petstrip::ACC_LDS_FIXED64;
-  // the most waves per workgroup whose row tables still fit next to the
-  // windows: a workgroup holds one CU's LDS, so its waves set the occupancy
-  h->fused_waves = 4;
-  h->lds_bytes = petstrip::fused_lds_bytes(max_px, mode, max_rows, 4);
-  for (int waves : {16, 8}) {
-    size_t b = petstrip::fused_lds_bytes(max_px, mode, max_rows, waves);
-    if (b <= h->lds_limit) {
-      h->fused_waves = waves;
-      h->lds_bytes = b;
-      break;
-    }
-  }
+  h->max_px = max_px;
+  // Every mode accumulates its LDS window in exact fixed point: ds_add_u64
+  // runs at several times the rate of ds_add_f32 on gfx950 and the window
+  // sums become exact; the accumulation mode only picks the global format.
+  // A workgroup of 16 waves holds one CU's LDS for the larger windows.
+  h->fused_waves = 16;
+  h->lds_bytes = petstrip::fused_lds_bytes(max_px, petstrip::MODE_MLEM);
   h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
                 h->opts.update == PETSTRIP_UPDATE_CPU &&
                 h->opts.path == PETSTRIP_PATH_AUTO;
@@ -316,23 +309,42 @@ const float* denominator_weight(petstrip_handle* h) {
 
 // One MLEM iteration: rho <- rho * backprojection(inv) (kept as the
 // reference form out[p] += K rho[p] inv_e).
+petstrip::FusedArgs fused_args(petstrip_handle* h) {
+  petstrip::FusedArgs a{};
+  a.sg = h->sg;
+  a.sk = h->sk;
+  a.sgeom = h->d_sgeom;
+  a.sidx = h->d_sidx;
+  a.items = h->d_items;
+  a.rho = h->d_rho;
+  a.lambda = h->d_lambda;
+  a.g = nullptr;
+  a.acc = h->d_acc;
+  a.acc_kind = petstrip::GLOBAL_F32;
+  a.fixed_scale = 1.0f;
+  a.inv_fixed_scale = 1.0 / 1099511627776.0;  // 2^-40
+  a.scalars = h->d_scalars;
+  return a;
+}
+
 int one_iteration(petstrip_handle* h) {
   const int64_t n = h->n_events;
   if (h->fused_ok) {
     const int accm = h->opts.accumulation;
-    const int kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
-                     : accm == PETSTRIP_ACCUM_COMPENSATED ? petstrip::GLOBAL_F64
-                                                           : petstrip::GLOBAL_F32;
-    const int mode = petstrip::ACC_LDS_FIXED64;
+    petstrip::FusedArgs a = fused_args(h);
+    a.acc_kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
+                 : accm == PETSTRIP_ACCUM_COMPENSATED ? petstrip::GLOBAL_F64
+                                                       : petstrip::GLOBAL_F32;
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
-    HIP_TRY(petstrip::launch_mlem_fused(
-        h->stream, h->sg, h->sk, h->d_sgeom, h->d_sidx, h->d_items, h->n_items,
-        h->lds_bytes, h->d_rho, h->d_lambda, h->d_acc, kind, mode, h->max_rows,
-        accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, h->fused_waves));
+    HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_MLEM, h->fused_waves,
+                                         accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
+                                         h->n_items, h->lds_bytes));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
-    HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, kind, h->d_out,
-                                      (int64_t)h->n_pix));
+    // float accumulators already hold image units; the u64 one holds 2^-40
+    double unit = a.acc_kind == petstrip::GLOBAL_U64 ? a.inv_fixed_scale : 1.0;
+    HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, a.acc_kind, unit, 0.0f, 1,
+                                      h->d_flag, h->d_out, (int64_t)h->n_pix));
     std::swap(h->d_rho, h->d_out);
     h->lambda_valid = false;
     return PETSTRIP_OK;
@@ -409,6 +421,9 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   if ((rc = dalloc(&h->d_sens, h->n_pix))) return bail(rc);
   if ((rc = dalloc(&h->d_scalars, 8))) return bail(rc);
   if ((rc = dalloc((double**)&h->d_acc, h->n_pix))) return bail(rc);
+  if ((rc = dalloc(&h->d_flag, 1))) return bail(rc);
+  if (hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream) != hipSuccess)
+    return bail(fail(PETSTRIP_EDEVICE, "hipMemset failed"));
   if (hipMemsetAsync(h->d_acc, 0, h->n_pix * sizeof(double), h->stream) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipMemset failed"));
   {
@@ -448,6 +463,7 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_sgeom);
   dfree(h->d_sidx);
   dfree(h->d_items);
+  dfree(h->d_flag);
   if (h->d_acc) (void)hipFree(h->d_acc);
   h->d_acc = nullptr;
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -568,6 +584,14 @@ int petstrip_iterate(petstrip_handle* h, int n_iterations) {
     h->stats.last_iteration_ms = ms;
     HIP_TRY(hipEventElapsedTime(&ms, h->evk0, h->evk1));
     h->stats.last_kernel_ms = ms;
+    int flag = 0;
+    HIP_TRY(hipMemcpy(&flag, h->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+      HIP_TRY(hipMemset(h->d_flag, 0, sizeof(int)));
+      return fail(PETSTRIP_ESTATE,
+                  "fixed-point image overflow: a pixel exceeded 2^23 in 2^-40 units "
+                  "(use the fp32 or compensated accumulation)");
+    }
   }
   return PETSTRIP_OK;
 }
@@ -653,47 +677,56 @@ int petstrip_forward(petstrip_handle* h, float* lambda) {
   if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
   int rc;
   if ((rc = ensure_device(h))) return rc;
-  HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom,
-                                       h->n_events, h->d_rho, nullptr,
-                                       h->d_lambda, h->d_inv));
-  if (lambda && h->n_events > 0)
-    HIP_TRY(hipMemcpyAsync(lambda, h->d_lambda, h->n_events * sizeof(float),
+  const int64_t n = h->n_events;
+  if (h->fused_ok) {
+    petstrip::FusedArgs a = fused_args(h);
+    HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
+    HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
+    HIP_TRY(petstrip::launch_strip_fused(
+        h->stream, petstrip::MODE_FORWARD, h->fused_waves, 0, a, h->n_items,
+        petstrip::fused_lds_bytes(h->max_px, petstrip::MODE_FORWARD)));
+  } else {
+    HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
+                                         h->d_rho, nullptr, h->d_lambda, h->d_inv));
+  }
+  if (lambda && n > 0)
+    HIP_TRY(hipMemcpyAsync(lambda, h->d_lambda, n * sizeof(float),
                            hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->lambda_valid = true;
   return PETSTRIP_OK;
 }
 
-int petstrip_backward(petstrip_handle* h, const float* g, float* grad_rho,
-                      double grad_sigma[2], double* loglik) {
-  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
-  int rc;
-  if ((rc = ensure_device(h))) return rc;
+namespace {
+
+// Per-event-kernel form of backward (windows too large for LDS, or the
+// per-event path forced).
+int backward_per_event(petstrip_handle* h, const float* g, float* grad_rho,
+                       double grad_sigma[2], double* ll_terms) {
   const int64_t n = h->n_events;
+  int rc;
   if (!h->lambda_valid && (rc = petstrip_forward(h, nullptr))) return rc;
-  // weights into d_inv: upstream g, or 1/lambda (log-likelihood)
   HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
   if (g) {
     if (n > 0)
-      HIP_TRY(hipMemcpyAsync(h->d_inv, g, n * sizeof(float),
-                             hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipMemcpyAsync(h->d_inv, g, n * sizeof(float), hipMemcpyHostToDevice,
+                             h->stream));
   } else {
     HIP_TRY(petstrip::launch_loglik_weights(h->stream, h->d_lambda, n, h->d_inv,
                                             h->d_scalars + 2));
   }
   if (grad_rho) {
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_out, 0.0f, (int64_t)h->n_pix));
-    HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n,
-                                         nullptr, h->d_inv, h->d_out));
+    HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n, nullptr,
+                                         h->d_inv, h->d_out));
     if (!g)
-      HIP_TRY(petstrip::launch_add_scalar(h->stream, h->d_out, -1.0f,
-                                          (int64_t)h->n_pix));
+      HIP_TRY(petstrip::launch_add_scalar(h->stream, h->d_out, -1.0f, (int64_t)h->n_pix));
     HIP_TRY(hipMemcpyAsync(grad_rho, h->d_out, h->n_pix * sizeof(float),
                            hipMemcpyDeviceToHost, h->stream));
   }
   if (grad_sigma)
-    HIP_TRY(petstrip::launch_sigma_grad(h->stream, h->sg, h->sk, h->d_geom, n,
-                                        h->d_rho, h->d_inv, h->d_scalars));
+    HIP_TRY(petstrip::launch_sigma_grad(h->stream, h->sg, h->sk, h->d_geom, n, h->d_rho,
+                                        h->d_inv, h->d_scalars));
   double scal[8];
   HIP_TRY(hipMemcpyAsync(scal, h->d_scalars, sizeof(scal), hipMemcpyDeviceToHost,
                          h->stream));
@@ -702,13 +735,82 @@ int petstrip_backward(petstrip_handle* h, const float* g, float* grad_rho,
     grad_sigma[0] = scal[0];
     grad_sigma[1] = scal[1];
   }
+  *ll_terms = scal[2];
+  return PETSTRIP_OK;
+}
+
+// Fused form: one FORWARD pass (lambda, sigma derivative, log-likelihood
+// terms, largest |g| K) and one BACKWARD pass (K g_e into the image in
+// signed fixed point with a scale that keeps every sum below 2^62).
+int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
+                   double grad_sigma[2], double* ll_terms) {
+  const int64_t n = h->n_events;
+  petstrip::FusedArgs a = fused_args(h);
+  if (g) {
+    if (n > 0)
+      HIP_TRY(hipMemcpyAsync(h->d_inv, g, n * sizeof(float), hipMemcpyHostToDevice,
+                             h->stream));
+    a.g = h->d_inv;
+  }
+  HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
+  HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
+  HIP_TRY(petstrip::launch_strip_fused(
+      h->stream, petstrip::MODE_FORWARD, h->fused_waves, 0, a, h->n_items,
+      petstrip::fused_lds_bytes(h->max_px, petstrip::MODE_FORWARD)));
+  double scal[8];
+  HIP_TRY(hipMemcpyAsync(scal, h->d_scalars, sizeof(scal), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->lambda_valid = true;
+  if (grad_sigma) {
+    grad_sigma[0] = scal[1];
+    grad_sigma[1] = scal[2];
+  }
+  *ll_terms = scal[0];
+  if (!grad_rho) return PETSTRIP_OK;
+  const float bias = g ? 0.0f : -1.0f;
+  const double bound = (double)std::max<int64_t>(n, 1) * scal[3];
+  if (!(bound > 0.0)) {
+    HIP_TRY(petstrip::launch_fill(h->stream, h->d_out, bias, (int64_t)h->n_pix));
+  } else {
+    int k = 62 - (int)std::ceil(std::log2(bound));
+    k = std::max(-100, std::min(100, k));
+    a.fixed_scale = std::ldexp(1.0f, k);
+    a.inv_fixed_scale = std::ldexp(1.0, -k);
+    a.acc_kind = petstrip::GLOBAL_U64;
+    HIP_TRY(hipEventRecord(h->evk0, h->stream));
+    HIP_TRY(petstrip::launch_strip_fused(
+        h->stream, petstrip::MODE_BACKWARD, h->fused_waves, 0, a, h->n_items,
+        petstrip::fused_lds_bytes(h->max_px, petstrip::MODE_BACKWARD)));
+    HIP_TRY(hipEventRecord(h->evk1, h->stream));
+    HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_U64,
+                                      a.inv_fixed_scale, bias, 0, nullptr, h->d_out,
+                                      (int64_t)h->n_pix));
+  }
+  HIP_TRY(hipMemcpyAsync(grad_rho, h->d_out, h->n_pix * sizeof(float),
+                         hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return PETSTRIP_OK;
+}
+
+}  // namespace
+
+int petstrip_backward(petstrip_handle* h, const float* g, float* grad_rho,
+                      double grad_sigma[2], double* loglik) {
+  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
+  if (loglik && g) return fail(PETSTRIP_EINVAL, "loglik needs g == NULL");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  double ll_terms = 0.0;
+  rc = h->fused_ok ? backward_fused(h, g, grad_rho, grad_sigma, &ll_terms)
+                   : backward_per_event(h, g, grad_rho, grad_sigma, &ll_terms);
+  if (rc) return rc;
   if (loglik) {
-    if (g) return fail(PETSTRIP_EINVAL, "loglik needs g == NULL");
     std::vector<float> rho(h->n_pix);
     if ((rc = petstrip_get_rho(h, rho.data()))) return rc;
     double srho = 0;
     for (float v : rho) srho += v;
-    *loglik = scal[2] - srho;
+    *loglik = ll_terms - srho;
   }
   return PETSTRIP_OK;
 }

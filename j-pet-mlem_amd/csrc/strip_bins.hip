@@ -88,22 +88,25 @@ __global__ __launch_bounds__(256) void k_item_windows(const EventGeom* __restric
 }
 
 // rho' from the accumulator; the accumulator is cleared for the next use.
-__global__ void k_finalize(void* __restrict__ acc, int kind, float* __restrict__ out,
-                           int64_t n) {
+__global__ void k_finalize(void* __restrict__ acc, int kind, double unit, float bias,
+                           int check_sign, int* __restrict__ overflow,
+                           float* __restrict__ out, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (kind == GLOBAL_F32) {
     float* a = reinterpret_cast<float*>(acc);
-    out[i] = a[i];
+    out[i] = (float)((double)a[i] * unit) + bias;
     a[i] = 0.0f;
   } else if (kind == GLOBAL_F64) {
     double* a = reinterpret_cast<double*>(acc);
-    out[i] = (float)a[i];
+    out[i] = (float)(a[i] * unit) + bias;
     a[i] = 0.0;
   } else {
-    unsigned long long* a = reinterpret_cast<unsigned long long*>(acc);
-    out[i] = (float)((double)a[i] * (1.0 / 1099511627776.0));
-    a[i] = 0ull;
+    long long* a = reinterpret_cast<long long*>(acc);
+    long long v = a[i];
+    if (check_sign && v < 0 && overflow) *overflow = 1;
+    out[i] = (float)((double)v * unit) + bias;
+    a[i] = 0;
   }
 }
 
@@ -136,10 +139,11 @@ hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(hipStream_t st, void* acc, int kind, float* out,
+hipError_t launch_finalize(hipStream_t st, void* acc, int kind, double unit,
+                           float bias, int check_sign, int* overflow, float* out,
                            int64_t n) {
   hipLaunchKernelGGL(k_finalize, dim3(nblk(n, 256)), dim3(256), 0, st, acc, kind,
-                     out, n);
+                     unit, bias, check_sign, overflow, out, n);
   return hipGetLastError();
 }
 

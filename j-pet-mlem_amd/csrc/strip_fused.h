@@ -1,4 +1,5 @@
-// Fused MLEM iteration over spatially binned events (strip_fused.hip).
+// Fused passes over spatially binned events (strip_fused.hip) and the
+// binning that feeds them (strip_bins.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,8 +18,24 @@ struct WorkItem {
   int32_t pad;
 };
 
-enum { ACC_LDS_F32 = 0, ACC_LDS_FIXED64 = 1 };
 enum { GLOBAL_F32 = 0, GLOBAL_F64 = 1, GLOBAL_U64 = 2 };
+enum { MODE_MLEM = 0, MODE_FORWARD = 1, MODE_BACKWARD = 2 };
+
+struct FusedArgs {
+  StripConst sg;             // constants the ellipse/box support was built with
+  StripConst sk;             // constants of the kernel value
+  const EventGeom* sgeom;    // sorted event geometry
+  const int32_t* sidx;       // original index of each sorted event
+  const WorkItem* items;
+  const float* rho;          // W*H image (MLEM, FORWARD)
+  float* lambda;             // per-event denominators (written by MLEM/FORWARD)
+  const float* g;            // upstream weights per event, or null for 1/lambda
+  void* acc;                 // W*H global accumulator (MLEM, BACKWARD)
+  int acc_kind;              // GLOBAL_*
+  float fixed_scale;         // BACKWARD: fixed-point scale of K g_e
+  double inv_fixed_scale;    // fixed-point unit when flushing to floats
+  double* scalars;           // FORWARD: {sum log lambda, dl/dsz, dl/dsdl, max |g| K}
+};
 
 hipError_t launch_bin_count(hipStream_t st, const EventGeom* geom, int64_t n,
                             int bin_w, int bin_h, int nbx, int32_t* bin,
@@ -28,16 +45,17 @@ hipError_t launch_bin_scatter(hipStream_t st, const EventGeom* geom, int64_t n,
                               EventGeom* sgeom, int32_t* sidx);
 hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
                                WorkItem* items, int n_items);
-// LDS bytes per workgroup of `waves` waves (4, 8 or 16).
-size_t fused_lds_bytes(int max_window_px, int acc_mode, int max_rows, int waves);
-hipError_t launch_mlem_fused(hipStream_t st, const StripConst& sg,
-                             const StripConst& sk, const EventGeom* sgeom,
-                             const int32_t* sidx, const WorkItem* items,
-                             int n_items, size_t lds_bytes, const float* rho,
-                             float* lambda, void* acc_global,
-                             int acc_global_kind, int acc_mode, int max_rows,
-                             int compensated, int waves);
-hipError_t launch_finalize(hipStream_t st, void* acc, int kind, float* out,
+
+// LDS bytes per workgroup for a window of max_window_px (pitch included).
+size_t fused_lds_bytes(int max_window_px, int mode);
+hipError_t launch_strip_fused(hipStream_t st, int mode, int waves, int compensated,
+                              const FusedArgs& args, int n_items, size_t lds_bytes);
+
+// out = acc * unit + bias (acc read as kind; 64-bit values as signed), the
+// accumulator cleared; *overflow (nullable) set to 1 if a 64-bit value is
+// negative where only non-negative sums are expected (check_sign).
+hipError_t launch_finalize(hipStream_t st, void* acc, int kind, double unit,
+                           float bias, int check_sign, int* overflow, float* out,
                            int64_t n);
 
 }  // namespace petstrip

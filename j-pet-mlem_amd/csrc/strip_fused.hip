@@ -1,5 +1,5 @@
-// Fused MLEM iteration for gfx950: events sorted into spatial bins, one
-// workgroup per chunk of a bin, rho and the backprojection held in LDS.
+// Fused passes over spatially binned events for gfx950: one workgroup per
+// chunk of a bin, the chunk's image window held in LDS.
 //
 // Reference semantics: Reconstruction::bb_update
 // (src/2d/strip/reconstruction.h:220-298) per event, Jacobi over events
@@ -9,14 +9,21 @@
 //      reference's float operation order, gaussian_kernel.h:131-137) and
 //      the row factor of the kernel;
 //   2. pass 1: walks the interval summing K * rho read from the LDS window;
-//      the DPP wave sum is the denominator lambda_e;
-//   3. pass 2: walks it again adding K * rho / lambda_e into the LDS
-//      accumulation window (ds_add), kernel values recomputed.
+//      the DPP wave sum is the denominator lambda_e (the forward projection);
+//   3. pass 2: walks it again adding K * w_e (* rho) into the LDS
+//      accumulation window (ds_add_u64 of fixed-point values).
+// Modes:
+//   MLEM      pass 1 + pass 2 with w_e = 1/lambda_e, values K rho / lambda_e
+//             in 2^-40 fixed point (each <= 1)
+//   FORWARD   pass 1 only, plus the sigma derivative of lambda_e and the
+//             log-likelihood terms, reduced per workgroup into doubles
+//   BACKWARD  pass 2 only with upstream weights g_e, values K g_e in signed
+//             fixed point with a scale chosen from the largest K g_e
 // The inner walks are branch-free and unrolled by four so that each lane
 // keeps several LDS reads and exponentials in flight.  The window pitch is
 // odd so that lanes on neighbouring rows of a tilted ellipse fall in
-// different LDS banks.  At the end the window is added to the global image
-// with one atomic per touched pixel.
+// different LDS banks.  At the end the window is added to the global
+// accumulator with one atomic per touched pixel.
 //
 // The kernel value follows GaussianKernel::operator()
 // (gaussian_kernel.h:51-79) with its exponent split in a row part and a
@@ -53,9 +60,19 @@ __device__ __forceinline__ float wave_sum(float v) {
   return (r0 + r1) + (r2 + r3);
 }
 
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __device__ __forceinline__ int wave_max(int v) {
   for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
   return uniform(v);
+}
+
+__device__ __forceinline__ float wave_max_f(float v) {
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
 }
 
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
@@ -89,13 +106,16 @@ __device__ __forceinline__ bool exact_in(const StripConst& s, const EventGeom& g
   return q <= 9.0f;
 }
 
-// One box row of an event: its ellipse interval and kernel row factor.
+// One box row of an event: its ellipse interval and kernel row factors.
 struct RowSeg {
-  int lo, len;  // first ellipse column, ellipse pixels in the row
-  float ndy;    // -(pixel_center.y - y_e)
-  float scale;  // c / sqrt(a^T S a) * exp(-isdl br^2 / 2)
+  int lo, len;   // first ellipse column, ellipse pixels in the row
+  float ndy;     // -(pixel_center.y - y_e)
+  float scale;   // c / sqrt(a^T S a) * exp(-isdl br^2 / 2)
+  float dz_row;  // row part of d lnK / d sigma_z
+  float ddl_row; // d lnK / d sigma_dl (no pixel part)
 };
 
+template <bool SIGMA>
 __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
                                             const EventGeom& g, int iy) {
   RowSeg s;
@@ -106,6 +126,8 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   s.len = 0;
   s.lo = g.x0;
   s.scale = 0.0f;
+  s.dz_row = 0.0f;
+  s.ddl_row = 0.0f;
   // Superset of the interval from the real roots of C dz^2 + Bdy dz + Ady2 =
   // 9.05, then the exact test trims both ends.  Along a row the exact test
   // has no holes: adjacent columns differ in q by at least C pw^2 (checked
@@ -133,226 +155,254 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   float ar = ((yp * -2.0f) * g.sec) * g.t;
   float br = (dy * -2.0f) * g.sec;
   float norm = fmaf(ar * sk.isdl, ar, fmaf(ap * sk.isz, ap, (aq * sk.isz) * aq));
-  s.scale = (sk.c / sqrtf(norm)) * expf(-0.5f * ((br * sk.isdl) * br));
+  float brb = (br * sk.isdl) * br;
+  s.scale = (sk.c / sqrtf(norm)) * expf(-0.5f * brb);
+  if constexpr (SIGMA) {
+    // ln K = ln c - ln(norm)/2 - bSb/2 with c ~ 1/(sz^2 sdl), norm and bSb
+    // linear in 1/sz^2 and 1/sdl^2:
+    //   d lnK/d sz  = (-2 + isz (ap^2 + aq^2)/norm + 2 isz pq^2) / sz
+    //   d lnK/d sdl = (-1 + isdl ar^2/norm + isdl br^2) / sdl
+    float apq = fmaf(ap, ap, aq * aq);
+    s.dz_row = (-2.0f + sk.isz * apq / norm) / sk.sigma_z;
+    s.ddl_row = (-1.0f + sk.isdl * (ar * ar) / norm + brb) / sk.sigma_dl;
+  }
   return s;
 }
 
-template <int ACC>
-struct AccType;
-template <>
-struct AccType<ACC_LDS_F32> {
-  using T = float;
-};
-template <>
-struct AccType<ACC_LDS_FIXED64> {
-  using T = unsigned long long;
-};
-
 constexpr float kFixedScale = 1099511627776.0f;  // 2^40
 constexpr int kUnroll = 4;
+constexpr float kLog2e = 1.44269504088896341f;
 
-// exp(-isz pq^2) for the pixel at column izf of a row (reference pq).
+// exp(-isz pq^2) for the pixel at column izf of a row (reference pq);
+// uu receives isz log2(e) pq^2.
 __device__ __forceinline__ float pixel_gauss(const StripConst& sg, const EventGeom& g,
-                                             float izf, float ndy, float s_pix) {
+                                             float izf, float ndy, float s_pix,
+                                             float& uu) {
   const float dz = fmaf(izf, sg.pw, sg.tl_z_half_w) - g.z;
   const float u = fmaf(ndy, g.t, dz) * s_pix;
-  return __builtin_amdgcn_exp2f(-(u * u));
+  uu = u * u;
+  return __builtin_amdgcn_exp2f(-uu);
 }
 
 }  // namespace
 
 // One workgroup per work item (a chunk of events sharing a spatial bin).
-template <int ACC, int WAVES, bool COMP>
-__global__ __launch_bounds__(64 * WAVES) void k_mlem_fused(
-    StripConst sg, StripConst sk, const EventGeom* __restrict__ sgeom,
-    const int32_t* __restrict__ sidx, const WorkItem* __restrict__ items,
-    const float* __restrict__ rho, float* __restrict__ lambda,
-    void* __restrict__ acc_global, int acc_global_kind) {
-  using A = typename AccType<ACC>::T;
+template <int MODE, int WAVES, bool COMP>
+__global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
+  constexpr bool kPass1 = MODE != MODE_BACKWARD;
+  constexpr bool kPass2 = MODE != MODE_FORWARD;
+  constexpr bool kRho = MODE != MODE_BACKWARD;  // rho window in LDS
   constexpr int kThreads = 64 * WAVES;
+  using U = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem[];
-  const WorkItem it = items[blockIdx.x];
+  const StripConst& sg = a.sg;
+  const StripConst& sk = a.sk;
+  const WorkItem it = a.items[blockIdx.x];
   const int ww = it.ww, wh = it.wh;
   const int pitch = ww | 1;  // odd pitch: rows of a tilted ellipse spread over banks
   const int npx = pitch * wh;
   float* rho_w = reinterpret_cast<float*>(smem);
-  A* acc_w = reinterpret_cast<A*>(smem + (((size_t)npx * 4 + 15) & ~(size_t)15));
+  U* acc_w = reinterpret_cast<U*>(smem + (kRho ? (((size_t)npx * 4 + 15) & ~(size_t)15) : 0));
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = __lane_id();
 
-  // load the rho window, clear the accumulation window
-  for (int i = threadIdx.x; i < ww * wh; i += kThreads) {
-    int r = i / ww, c = i - r * ww;
-    rho_w[r * pitch + c] = rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
+  if constexpr (kRho) {
+    for (int i = threadIdx.x; i < ww * wh; i += kThreads) {
+      int r = i / ww, c = i - r * ww;
+      rho_w[r * pitch + c] = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
+    }
   }
-  for (int i = threadIdx.x; i < npx; i += kThreads) acc_w[i] = (A)0;
+  if constexpr (kPass2) {
+    for (int i = threadIdx.x; i < npx; i += kThreads) acc_w[i] = 0ull;
+  }
   __syncthreads();
 
-  const float s_pix = sqrtf(sk.isz * 1.44269504088896341f);  // sqrt(isz log2 e)
+  const float s_pix = sqrtf(sk.isz * kLog2e);  // sqrt(isz log2 e)
+  const float cz_pix = 2.0f / (kLog2e * sk.sigma_z);  // 2 isz pq^2 / sz = cz uu
+  double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0;
 
   for (int e = it.begin + wave; e < it.end; e += WAVES) {
-    const EventGeom g = sgeom[e];
+    const EventGeom g = a.sgeom[e];
     const int nrows = g.y1 - g.y0;
     const int row0 = (g.y0 - it.wy0) * pitch - it.wx0;  // window index of (g.y0, 0)
+    const int orig = a.sidx[e];
 
-    // ---- pass 1: denominator -------------------------------------------------
-    float acc = 0.0f, comp = 0.0f;
-    for (int base = 0; base < nrows; base += 64) {
-      const int r = base + lane;
-      RowSeg s{0, 0, 0.0f, 0.0f};
-      if (r < nrows) s = row_setup(sg, sk, g, g.y0 + r);
-      const int w0 = s.len > 0 ? row0 + r * pitch + s.lo : 0;
-      const int maxlen = wave_max(s.len);
-      const float lof = (float)s.lo;
-      for (int k = 0; k < maxlen; k += kUnroll) {
+    // ---- pass 1: denominator (and sigma sums) -------------------------------
+    float lam = 0.0f, kmax = 0.0f;
+    if constexpr (kPass1) {
+      constexpr bool SIG = MODE == MODE_FORWARD;
+      float acc = 0.0f, comp = 0.0f, sz = 0.0f, sdl = 0.0f;
+      for (int base = 0; base < nrows; base += 64) {
+        const int r = base + lane;
+        RowSeg s{0, 0, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (r < nrows) s = row_setup<SIG>(sg, sk, g, g.y0 + r);
+        const int w0 = s.len > 0 ? row0 + r * pitch + s.lo : 0;
+        const int maxlen = wave_max(s.len);
+        const float lof = (float)s.lo;
+        float s0 = 0.0f, s1 = 0.0f;
+        for (int k = 0; k < maxlen; k += kUnroll) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          const int kk = k + u;
-          const bool on = kk < s.len;
-          const float gv = pixel_gauss(sg, g, lof + (float)kk, s.ndy, s_pix);
-          const float kr = gv * s.scale * rho_w[on ? w0 + kk : w0];
-          if constexpr (COMP) {
-            float sum, err;
-            two_sum(acc, on ? kr : 0.0f, sum, err);
-            acc = sum;
-            comp += err;
-          } else {
-            acc += on ? kr : 0.0f;
+          for (int u = 0; u < kUnroll; ++u) {
+            const int kk = k + u;
+            const bool on = kk < s.len;
+            float uu;
+            const float gv = pixel_gauss(sg, g, lof + (float)kk, s.ndy, s_pix, uu);
+            const float kr = on ? gv * s.scale * rho_w[on ? w0 + kk : w0] : 0.0f;
+            if constexpr (COMP) {
+              float sum, err;
+              two_sum(s0, kr, sum, err);
+              s0 = sum;
+              comp += err;
+            } else {
+              s0 += kr;
+            }
+            if constexpr (SIG) s1 = fmaf(kr, uu, s1);
           }
+        }
+        if constexpr (COMP) {
+          float sum, err;
+          two_sum(acc, s0, sum, err);
+          acc = sum;
+          comp += err;
+        } else {
+          acc += s0;
+        }
+        if constexpr (SIG) {
+          sz += fmaf(s.dz_row, s0, cz_pix * s1);
+          sdl += s.ddl_row * s0;
+          kmax = fmaxf(kmax, s.scale);
+        }
+      }
+      lam = COMP ? wave_sum_comp(acc, comp) : wave_sum(acc);
+      if (lane == 0) a.lambda[orig] = lam;
+      if constexpr (SIG) {
+        // weight of this event: upstream g_e or 1/lambda_e (log-likelihood)
+        float ge = a.g ? a.g[orig] : (lam > 0.0f ? 1.0f / lam : 0.0f);
+        double wsz = wave_sum_d((double)sz), wsdl = wave_sum_d((double)sdl);
+        kmax = wave_max_f(kmax);
+        if (lane == 0) {
+          if (lam > 0.0f) ll += log((double)lam);
+          gsz += (double)ge * wsz;
+          gsdl += (double)ge * wsdl;
+          mmax = fmax(mmax, (double)fabsf(ge) * (double)kmax);
         }
       }
     }
-    const float lam = COMP ? wave_sum_comp(acc, comp) : wave_sum(acc);
-    if (lane == 0) lambda[sidx[e]] = lam;
-    if (!(lam > 0.0f)) continue;
-    const float inv = 1.0f / lam;
 
     // ---- pass 2: backprojection into the LDS window --------------------------
-    for (int base = 0; base < nrows; base += 64) {
-      const int r = base + lane;
-      RowSeg s{0, 0, 0.0f, 0.0f};
-      if (r < nrows) s = row_setup(sg, sk, g, g.y0 + r);
-      const int w0 = s.len > 0 ? row0 + r * pitch + s.lo : 0;
-      const int maxlen = wave_max(s.len);
-      const float lof = (float)s.lo;
-      const float sc = s.scale * inv;
-      for (int k = 0; k < maxlen; k += kUnroll) {
+    if constexpr (kPass2) {
+      float we;  // multiplier of the row scale
+      if constexpr (MODE == MODE_MLEM) {
+        if (!(lam > 0.0f)) continue;
+        we = kFixedScale / lam;
+      } else {
+        float ge = a.g ? a.g[orig] : a.lambda[orig];
+        if (!a.g) ge = ge > 0.0f ? 1.0f / ge : 0.0f;
+        if (ge == 0.0f) continue;
+        we = ge * a.fixed_scale;
+      }
+      for (int base = 0; base < nrows; base += 64) {
+        const int r = base + lane;
+        RowSeg s{0, 0, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (r < nrows) s = row_setup<false>(sg, sk, g, g.y0 + r);
+        const int w0 = s.len > 0 ? row0 + r * pitch + s.lo : 0;
+        const int maxlen = wave_max(s.len);
+        const float lof = (float)s.lo;
+        const float sc = s.scale * we;
+        for (int k = 0; k < maxlen; k += kUnroll) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          const int kk = k + u;
-          if (kk < s.len) {
-            const float gv = pixel_gauss(sg, g, lof + (float)kk, s.ndy, s_pix);
-            const float v = gv * sc * rho_w[w0 + kk];
-            if constexpr (ACC == ACC_LDS_F32) {
-              atomicAdd(&acc_w[w0 + kk], v);
-            } else {
-              atomicAdd(&acc_w[w0 + kk],
-                        (unsigned long long)__float2ull_rn(v * kFixedScale));
+          for (int u = 0; u < kUnroll; ++u) {
+            const int kk = k + u;
+            if (kk < s.len) {
+              float uu;
+              const float gv = pixel_gauss(sg, g, lof + (float)kk, s.ndy, s_pix, uu);
+              float v = gv * sc;
+              if constexpr (kRho) v *= rho_w[w0 + kk];
+              // two's complement adds: signed values accumulate exactly
+              atomicAdd(&acc_w[w0 + kk], (U)__float2ll_rn(v));
             }
           }
         }
       }
     }
   }
-  __syncthreads();
 
-  // ---- flush the window into the global accumulator ------------------------
-  for (int i = threadIdx.x; i < ww * wh; i += kThreads) {
-    int r = i / ww, c = i - r * ww;
-    A v = acc_w[r * pitch + c];
-    if (v == (A)0) continue;
-    size_t p = (size_t)(it.wy0 + r) * sg.W + (it.wx0 + c);
-    if constexpr (ACC == ACC_LDS_F32) {
-      if (acc_global_kind == GLOBAL_F64)
-        unsafeAtomicAdd(reinterpret_cast<double*>(acc_global) + p, (double)v);
+  if constexpr (MODE == MODE_FORWARD) {
+    // per-workgroup reduction of the scalars: lane 0 of each wave holds them
+    if (lane == 0) {
+      unsafeAtomicAdd(&a.scalars[0], ll);
+      unsafeAtomicAdd(&a.scalars[1], gsz);
+      unsafeAtomicAdd(&a.scalars[2], gsdl);
+      // positive doubles order like their bit patterns
+      atomicMax(reinterpret_cast<unsigned long long*>(&a.scalars[3]),
+                (U)__double_as_longlong(mmax));
+    }
+  }
+
+  if constexpr (kPass2) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < ww * wh; i += kThreads) {
+      int r = i / ww, c = i - r * ww;
+      U v = acc_w[r * pitch + c];
+      if (v == 0ull) continue;
+      size_t p = (size_t)(it.wy0 + r) * sg.W + (it.wx0 + c);
+      if (a.acc_kind == GLOBAL_U64)
+        atomicAdd(reinterpret_cast<U*>(a.acc) + p, v);
+      else if (a.acc_kind == GLOBAL_F64)
+        unsafeAtomicAdd(reinterpret_cast<double*>(a.acc) + p,
+                        (double)(long long)v * a.inv_fixed_scale);
       else
-        unsafeAtomicAdd(reinterpret_cast<float*>(acc_global) + p, v);
-    } else {
-      // the window holds exact 2^-40 fixed-point sums
-      if (acc_global_kind == GLOBAL_U64)
-        atomicAdd(reinterpret_cast<unsigned long long*>(acc_global) + p, v);
-      else if (acc_global_kind == GLOBAL_F64)
-        unsafeAtomicAdd(reinterpret_cast<double*>(acc_global) + p,
-                        (double)v * (1.0 / 1099511627776.0));
-      else
-        unsafeAtomicAdd(reinterpret_cast<float*>(acc_global) + p,
-                        (float)((double)v * (1.0 / 1099511627776.0)));
+        unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + p,
+                        (float)((double)(long long)v * a.inv_fixed_scale));
     }
   }
 }
 
 // ---------------------------------------------------------------------------
 
-size_t fused_lds_bytes(int max_window_px, int acc_mode, int max_rows, int waves) {
-  (void)max_rows;
-  (void)waves;
+size_t fused_lds_bytes(int max_window_px, int mode) {
   size_t npx = (size_t)max_window_px;
-  size_t accb = acc_mode == ACC_LDS_F32 ? 4 : 8;
-  return ((npx * 4 + 15) & ~(size_t)15) + npx * accb;
+  size_t rho = mode == MODE_BACKWARD ? 0 : ((npx * 4 + 15) & ~(size_t)15);
+  size_t acc = mode == MODE_FORWARD ? 0 : npx * 8;
+  return rho + acc + 16;
 }
 
 namespace {
 
-template <int ACC, int WAVES, bool COMP>
-hipError_t launch_t(hipStream_t st, const StripConst& sg, const StripConst& sk,
-                    const EventGeom* sgeom, const int32_t* sidx, const WorkItem* items,
-                    int n_items, size_t lds_bytes, const float* rho, float* lambda,
-                    void* acc_global, int acc_global_kind) {
-  auto kern = k_mlem_fused<ACC, WAVES, COMP>;
+template <int MODE, int WAVES, bool COMP>
+hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds) {
+  auto kern = k_strip_fused<MODE, WAVES, COMP>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds_bytes);
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(n_items), dim3(64 * WAVES), lds_bytes, st, sg, sk,
-                     sgeom, sidx, items, rho, lambda, acc_global, acc_global_kind);
+  hipLaunchKernelGGL(kern, dim3(n_items), dim3(64 * WAVES), lds, st, a);
   return hipGetLastError();
 }
 
-template <int ACC, bool COMP>
-hipError_t launch_w(int waves, hipStream_t st, const StripConst& sg,
-                    const StripConst& sk, const EventGeom* sgeom, const int32_t* sidx,
-                    const WorkItem* items, int n_items, size_t lds_bytes,
-                    const float* rho, float* lambda, void* acc_global,
-                    int acc_global_kind) {
+template <int MODE, bool COMP>
+hipError_t launch_w(int waves, hipStream_t st, const FusedArgs& a, int n_items, size_t lds) {
   switch (waves) {
-    case 16:
-      return launch_t<ACC, 16, COMP>(st, sg, sk, sgeom, sidx, items, n_items, lds_bytes,
-                                     rho, lambda, acc_global, acc_global_kind);
-    case 8:
-      return launch_t<ACC, 8, COMP>(st, sg, sk, sgeom, sidx, items, n_items, lds_bytes,
-                                    rho, lambda, acc_global, acc_global_kind);
-    default:
-      return launch_t<ACC, 4, COMP>(st, sg, sk, sgeom, sidx, items, n_items, lds_bytes,
-                                    rho, lambda, acc_global, acc_global_kind);
+    case 16: return launch_t<MODE, 16, COMP>(st, a, n_items, lds);
+    case 8: return launch_t<MODE, 8, COMP>(st, a, n_items, lds);
+    default: return launch_t<MODE, 4, COMP>(st, a, n_items, lds);
   }
 }
 
 }  // namespace
 
-hipError_t launch_mlem_fused(hipStream_t st, const StripConst& sg, const StripConst& sk,
-                             const EventGeom* sgeom, const int32_t* sidx,
-                             const WorkItem* items, int n_items, size_t lds_bytes,
-                             const float* rho, float* lambda, void* acc_global,
-                             int acc_global_kind, int acc_mode, int max_rows,
-                             int compensated, int waves) {
-  (void)max_rows;
+hipError_t launch_strip_fused(hipStream_t st, int mode, int waves, int compensated,
+                              const FusedArgs& a, int n_items, size_t lds_bytes) {
   if (n_items <= 0) return hipSuccess;
-  if (acc_mode == ACC_LDS_F32) {
-    if (compensated)
-      return launch_w<ACC_LDS_F32, true>(waves, st, sg, sk, sgeom, sidx, items, n_items,
-                                         lds_bytes, rho, lambda, acc_global,
-                                         acc_global_kind);
-    return launch_w<ACC_LDS_F32, false>(waves, st, sg, sk, sgeom, sidx, items, n_items,
-                                        lds_bytes, rho, lambda, acc_global,
-                                        acc_global_kind);
+  switch (mode) {
+    case MODE_MLEM:
+      return compensated ? launch_w<MODE_MLEM, true>(waves, st, a, n_items, lds_bytes)
+                         : launch_w<MODE_MLEM, false>(waves, st, a, n_items, lds_bytes);
+    case MODE_FORWARD:
+      return compensated ? launch_w<MODE_FORWARD, true>(waves, st, a, n_items, lds_bytes)
+                         : launch_w<MODE_FORWARD, false>(waves, st, a, n_items, lds_bytes);
+    default:
+      return launch_w<MODE_BACKWARD, false>(waves, st, a, n_items, lds_bytes);
   }
-  if (compensated)
-    return launch_w<ACC_LDS_FIXED64, true>(waves, st, sg, sk, sgeom, sidx, items, n_items,
-                                           lds_bytes, rho, lambda, acc_global,
-                                           acc_global_kind);
-  return launch_w<ACC_LDS_FIXED64, false>(waves, st, sg, sk, sgeom, sidx, items, n_items,
-                                          lds_bytes, rho, lambda, acc_global,
-                                          acc_global_kind);
 }
 
 }  // namespace petstrip
