@@ -145,6 +145,7 @@ struct petstrip_handle {
   size_t lds_bytes = 0;
   int fused_waves = 16;
   int max_px = 1;
+  int max_pitch = 1;
   int* d_flag = nullptr;  // device overflow flag of the fixed-point image
   bool fused_ok = false;
   void* d_acc = nullptr;  // W*H accumulator, 8 bytes per pixel
@@ -255,19 +256,21 @@ int prepare_fused(petstrip_handle* h) {
                            hipMemcpyDeviceToHost, h->stream));
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
-  int max_px = 1, max_rows = 1;
+  int max_px = 1, max_rows = 1, max_pitch = 1;
   for (auto& it : items) {
     max_px = std::max(max_px, (it.ww | 1) * it.wh);  // odd LDS pitch
+    max_pitch = std::max(max_pitch, it.ww | 1);
     max_rows = std::max(max_rows, it.max_rows);
   }
   h->max_rows = max_rows;
   h->max_px = max_px;
+  h->max_pitch = max_pitch;
   // Every mode accumulates its LDS window in exact fixed point: ds_add_u64
   // runs at several times the rate of ds_add_f32 on gfx950 and the window
   // sums become exact; the accumulation mode only picks the global format.
   // A workgroup of 16 waves holds one CU's LDS for the larger windows.
   h->fused_waves = 16;
-  h->lds_bytes = petstrip::fused_lds_bytes(max_px, petstrip::MODE_MLEM);
+  h->lds_bytes = petstrip::fused_lds_bytes(max_px, max_pitch, petstrip::MODE_MLEM);
   h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
                 h->opts.update == PETSTRIP_UPDATE_CPU &&
                 h->opts.path == PETSTRIP_PATH_AUTO;
@@ -684,7 +687,7 @@ int petstrip_forward(petstrip_handle* h, float* lambda) {
     HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
     HIP_TRY(petstrip::launch_strip_fused(
         h->stream, petstrip::MODE_FORWARD, h->fused_waves, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_px, petstrip::MODE_FORWARD)));
+        petstrip::fused_lds_bytes(h->max_px, h->max_pitch, petstrip::MODE_FORWARD)));
   } else {
     HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
                                          h->d_rho, nullptr, h->d_lambda, h->d_inv));
@@ -756,7 +759,7 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
   HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
   HIP_TRY(petstrip::launch_strip_fused(
       h->stream, petstrip::MODE_FORWARD, h->fused_waves, 0, a, h->n_items,
-      petstrip::fused_lds_bytes(h->max_px, petstrip::MODE_FORWARD)));
+      petstrip::fused_lds_bytes(h->max_px, h->max_pitch, petstrip::MODE_FORWARD)));
   double scal[8];
   HIP_TRY(hipMemcpyAsync(scal, h->d_scalars, sizeof(scal), hipMemcpyDeviceToHost,
                          h->stream));
@@ -781,7 +784,7 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
     HIP_TRY(petstrip::launch_strip_fused(
         h->stream, petstrip::MODE_BACKWARD, h->fused_waves, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_px, petstrip::MODE_BACKWARD)));
+        petstrip::fused_lds_bytes(h->max_px, h->max_pitch, petstrip::MODE_BACKWARD)));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_U64,
                                       a.inv_fixed_scale, bias, 0, nullptr, h->d_out,

@@ -46,8 +46,9 @@ hipError_t launch_bin_scatter(hipStream_t st, const EventGeom* geom, int64_t n,
 hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
                                WorkItem* items, int n_items);
 
-// LDS bytes per workgroup for a window of max_window_px (pitch included).
-size_t fused_lds_bytes(int max_window_px, int mode);
+// LDS bytes per workgroup for windows of at most max_window_px pixels
+// (pitch included) and pitch max_pitch.
+size_t fused_lds_bytes(int max_window_px, int max_pitch, int mode);
 hipError_t launch_strip_fused(hipStream_t st, int mode, int waves, int compensated,
                               const FusedArgs& args, int n_items, size_t lds_bytes);
 

@@ -19,17 +19,18 @@
 //             log-likelihood terms, reduced per workgroup into doubles
 //   BACKWARD  pass 2 only with upstream weights g_e, values K g_e in signed
 //             fixed point with a scale chosen from the largest K g_e
-// The inner walks are branch-free and unrolled by four so that each lane
-// keeps several LDS reads and exponentials in flight.  The window pitch is
-// odd so that lanes on neighbouring rows of a tilted ellipse fall in
-// different LDS banks.  At the end the window is added to the global
-// accumulator with one atomic per touched pixel.
+// The inner walks are branch-free where it matters and unrolled by four so
+// that each lane keeps several LDS reads and exponentials in flight.  The
+// window pitch is odd so that lanes on neighbouring rows of a tilted
+// ellipse fall in different LDS banks.  At the end the window is added to
+// the global accumulator with one atomic per touched pixel.
 //
-// The kernel value follows GaussianKernel::operator()
-// (gaussian_kernel.h:51-79) with its exponent split in a row part and a
-// pixel part: K = [c / sqrt(a^T S a) * exp(-isdl br^2 / 2)] * exp(-isz pq^2),
-// the pixel part evaluated as exp2(-(pq * sqrt(isz log2 e))^2) with pq
-// computed from the pixel centre exactly as the reference does.
+// Kernel value (GaussianKernel::operator(), gaussian_kernel.h:51-79): the
+// exponent splits into a row part and a pixel part,
+//   K = c/sqrt(a^T S a) exp(-isdl br^2/2) * exp(-isz pq^2)
+//     = exp2(lsc_row - u^2),   u = pq sqrt(isz log2 e),
+// with pq = dz - dy tan computed exactly as the reference at the row's first
+// ellipse pixel and advanced by one pixel width per column (u = u0 + k du).
 #include <hip/hip_runtime.h>
 
 #include "strip_fused.h"
@@ -106,28 +107,26 @@ __device__ __forceinline__ bool exact_in(const StripConst& s, const EventGeom& g
   return q <= 9.0f;
 }
 
-// One box row of an event: its ellipse interval and kernel row factors.
+constexpr float kLog2e = 1.44269504088896341f;
+
+// One box row of an event as a lane walks it.
 struct RowSeg {
-  int lo, len;   // first ellipse column, ellipse pixels in the row
-  float ndy;     // -(pixel_center.y - y_e)
-  float scale;   // c / sqrt(a^T S a) * exp(-isdl br^2 / 2)
-  float dz_row;  // row part of d lnK / d sigma_z
-  float ddl_row; // d lnK / d sigma_dl (no pixel part)
+  int len;        // ellipse pixels in the row (0: nothing to do)
+  int w0;         // LDS window index of the first ellipse pixel
+  float u0;       // u of the first ellipse pixel
+  float lsc;      // log2 of the row factor
+  float dz_row;   // row part of d lnK / d sigma_z (FORWARD)
+  float ddl_row;  // d lnK / d sigma_dl (FORWARD)
 };
 
 template <bool SIGMA>
 __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
-                                            const EventGeom& g, int iy) {
-  RowSeg s;
+                                            const EventGeom& g, int iy, int row_w,
+                                            float s_pix) {
+  RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
   float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
   float Ady2 = (dy * g.A) * dy;
   float Bdy = dy * g.B;
-  s.ndy = -dy;
-  s.len = 0;
-  s.lo = g.x0;
-  s.scale = 0.0f;
-  s.dz_row = 0.0f;
-  s.ddl_row = 0.0f;
   // Superset of the interval from the real roots of C dz^2 + Bdy dz + Ady2 =
   // 9.05, then the exact test trims both ends.  Along a row the exact test
   // has no holes: adjacent columns differ in q by at least C pw^2 (checked
@@ -145,8 +144,11 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   while (a <= b && !exact_in(sg, g, Ady2, Bdy, a)) ++a;
   while (b >= a && !exact_in(sg, g, Ady2, Bdy, b)) --b;
   if (a > b) return s;
-  s.lo = a;
   s.len = b - a + 1;
+  s.w0 = row_w + a;
+  // pq at the first pixel, with the reference's rounding (gaussian_kernel.h:64)
+  float dz = fmaf((float)a, sg.pw, sg.tl_z_half_w) - g.z;
+  s.u0 = fmaf(-dy, g.t, dz) * s_pix;
   // kernel row factor (gaussian_kernel.h:58-78)
   float sec_sq = g.sec * g.sec;
   float yp = dy + g.y;
@@ -156,7 +158,7 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   float br = (dy * -2.0f) * g.sec;
   float norm = fmaf(ar * sk.isdl, ar, fmaf(ap * sk.isz, ap, (aq * sk.isz) * aq));
   float brb = (br * sk.isdl) * br;
-  s.scale = (sk.c / sqrtf(norm)) * expf(-0.5f * brb);
+  s.lsc = log2f(sk.c) - 0.5f * log2f(norm) - (0.5f * kLog2e) * brb;
   if constexpr (SIGMA) {
     // ln K = ln c - ln(norm)/2 - bSb/2 with c ~ 1/(sz^2 sdl), norm and bSb
     // linear in 1/sz^2 and 1/sdl^2:
@@ -169,19 +171,15 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   return s;
 }
 
-constexpr float kFixedScale = 1099511627776.0f;  // 2^40
 constexpr int kUnroll = 4;
-constexpr float kLog2e = 1.44269504088896341f;
+constexpr int kCachedGroups = 4;  // row groups kept in registers between passes
 
-// exp(-isz pq^2) for the pixel at column izf of a row (reference pq);
-// uu receives isz log2(e) pq^2.
-__device__ __forceinline__ float pixel_gauss(const StripConst& sg, const EventGeom& g,
-                                             float izf, float ndy, float s_pix,
-                                             float& uu) {
-  const float dz = fmaf(izf, sg.pw, sg.tl_z_half_w) - g.z;
-  const float u = fmaf(ndy, g.t, dz) * s_pix;
-  uu = u * u;
-  return __builtin_amdgcn_exp2f(-uu);
+// x in [0, 2^63): the two 32-bit halves of round-toward-zero(x), x given as
+// w = x * 2^-32 (five VALU operations instead of a generic float->i64).
+__device__ __forceinline__ unsigned long long split_u64(float w) {
+  float hi = floorf(w);
+  float lo = (w - hi) * 4294967296.0f;
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
 }
 
 }  // namespace
@@ -192,6 +190,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
   constexpr bool kPass1 = MODE != MODE_BACKWARD;
   constexpr bool kPass2 = MODE != MODE_FORWARD;
   constexpr bool kRho = MODE != MODE_BACKWARD;  // rho window in LDS
+  constexpr bool kSigma = MODE == MODE_FORWARD;
   constexpr int kThreads = 64 * WAVES;
   using U = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -201,15 +200,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
   const int ww = it.ww, wh = it.wh;
   const int pitch = ww | 1;  // odd pitch: rows of a tilted ellipse spread over banks
   const int npx = pitch * wh;
+  // the rho window has spare space: unrolled lanes past their row's end read
+  // up to maxlen + 3 elements beyond it (values discarded)
+  const int nrho = npx + 2 * pitch + 8;
   float* rho_w = reinterpret_cast<float*>(smem);
-  U* acc_w = reinterpret_cast<U*>(smem + (kRho ? (((size_t)npx * 4 + 15) & ~(size_t)15) : 0));
+  U* acc_w = reinterpret_cast<U*>(
+      smem + (kRho ? (((size_t)nrho * 4 + 15) & ~(size_t)15) : 0));
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = __lane_id();
 
   if constexpr (kRho) {
-    for (int i = threadIdx.x; i < ww * wh; i += kThreads) {
-      int r = i / ww, c = i - r * ww;
-      rho_w[r * pitch + c] = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
+    for (int i = threadIdx.x; i < nrho; i += kThreads) {
+      int r = i / pitch, c = i - r * pitch;
+      float v = 0.0f;
+      if (r < wh && c < ww) v = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
+      rho_w[i] = v;
     }
   }
   if constexpr (kPass2) {
@@ -218,68 +223,74 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
   __syncthreads();
 
   const float s_pix = sqrtf(sk.isz * kLog2e);  // sqrt(isz log2 e)
-  const float cz_pix = 2.0f / (kLog2e * sk.sigma_z);  // 2 isz pq^2 / sz = cz uu
+  const float du = sg.pw * s_pix;               // u step per column
+  const float cz_pix = 2.0f / (kLog2e * sk.sigma_z);  // 2 isz pq^2 / sz = cz u^2
   double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0;
 
   for (int e = it.begin + wave; e < it.end; e += WAVES) {
     const EventGeom g = a.sgeom[e];
     const int nrows = g.y1 - g.y0;
+    const int ngroups = (nrows + 63) >> 6;
     const int row0 = (g.y0 - it.wy0) * pitch - it.wx0;  // window index of (g.y0, 0)
     const int orig = a.sidx[e];
+    RowSeg cache[kCachedGroups];
 
     // ---- pass 1: denominator (and sigma sums) -------------------------------
-    float lam = 0.0f, kmax = 0.0f;
+    float lam = 0.0f;
     if constexpr (kPass1) {
-      constexpr bool SIG = MODE == MODE_FORWARD;
-      float acc = 0.0f, comp = 0.0f, sz = 0.0f, sdl = 0.0f;
-      for (int base = 0; base < nrows; base += 64) {
-        const int r = base + lane;
-        RowSeg s{0, 0, 0.0f, 0.0f, 0.0f, 0.0f};
-        if (r < nrows) s = row_setup<SIG>(sg, sk, g, g.y0 + r);
-        const int w0 = s.len > 0 ? row0 + r * pitch + s.lo : 0;
-        const int maxlen = wave_max(s.len);
-        const float lof = (float)s.lo;
-        float s0 = 0.0f, s1 = 0.0f;
-        for (int k = 0; k < maxlen; k += kUnroll) {
+      float acc = 0.0f, comp = 0.0f, sz = 0.0f, sdl = 0.0f, lmax = -1e30f;
 #pragma unroll
-          for (int u = 0; u < kUnroll; ++u) {
-            const int kk = k + u;
-            const bool on = kk < s.len;
-            float uu;
-            const float gv = pixel_gauss(sg, g, lof + (float)kk, s.ndy, s_pix, uu);
-            const float kr = on ? gv * s.scale * rho_w[on ? w0 + kk : w0] : 0.0f;
-            if constexpr (COMP) {
-              float sum, err;
-              two_sum(s0, kr, sum, err);
-              s0 = sum;
-              comp += err;
-            } else {
-              s0 += kr;
+      for (int gi = 0; gi < kCachedGroups + 1; ++gi) {
+        // groups beyond the cached ones share the last iteration as a loop
+        for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
+          if (gg >= ngroups) break;
+          const int r = gg * 64 + lane;
+          RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+          if (r < nrows) s = row_setup<kSigma>(sg, sk, g, g.y0 + r, row0 + r * pitch, s_pix);
+          if (gi < kCachedGroups) cache[gi] = s;
+          const int maxlen = wave_max(s.len);
+          float s0 = 0.0f, s1 = 0.0f;
+          for (int k = 0; k < maxlen; k += kUnroll) {
+            const float kf = (float)k;
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+              const float uu = fmaf(kf + (float)u, du, s.u0);
+              const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc));
+              float kr = kv * rho_w[s.w0 + k + u];
+              kr = (k + u < s.len) ? kr : 0.0f;
+              if constexpr (COMP) {
+                float sum, err;
+                two_sum(s0, kr, sum, err);
+                s0 = sum;
+                comp += err;
+              } else {
+                s0 += kr;
+              }
+              if constexpr (kSigma) s1 = fmaf(kr, uu * uu, s1);
             }
-            if constexpr (SIG) s1 = fmaf(kr, uu, s1);
           }
-        }
-        if constexpr (COMP) {
-          float sum, err;
-          two_sum(acc, s0, sum, err);
-          acc = sum;
-          comp += err;
-        } else {
-          acc += s0;
-        }
-        if constexpr (SIG) {
-          sz += fmaf(s.dz_row, s0, cz_pix * s1);
-          sdl += s.ddl_row * s0;
-          kmax = fmaxf(kmax, s.scale);
+          if constexpr (COMP) {
+            float sum, err;
+            two_sum(acc, s0, sum, err);
+            acc = sum;
+            comp += err;
+          } else {
+            acc += s0;
+          }
+          if constexpr (kSigma) {
+            sz += fmaf(s.dz_row, s0, cz_pix * s1);
+            sdl += s.ddl_row * s0;
+            lmax = fmaxf(lmax, s.lsc);
+          }
         }
       }
       lam = COMP ? wave_sum_comp(acc, comp) : wave_sum(acc);
       if (lane == 0) a.lambda[orig] = lam;
-      if constexpr (SIG) {
+      if constexpr (kSigma) {
         // weight of this event: upstream g_e or 1/lambda_e (log-likelihood)
         float ge = a.g ? a.g[orig] : (lam > 0.0f ? 1.0f / lam : 0.0f);
         double wsz = wave_sum_d((double)sz), wsdl = wave_sum_d((double)sdl);
-        kmax = wave_max_f(kmax);
+        float kmax = __builtin_amdgcn_exp2f(wave_max_f(lmax));
         if (lane == 0) {
           if (lam > 0.0f) ll += log((double)lam);
           gsz += (double)ge * wsz;
@@ -291,51 +302,60 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
 
     // ---- pass 2: backprojection into the LDS window --------------------------
     if constexpr (kPass2) {
-      float we;  // multiplier of the row scale
+      // value multiplier, including the fixed-point unit / 2^32 (split_u64)
+      float we;
       if constexpr (MODE == MODE_MLEM) {
         if (!(lam > 0.0f)) continue;
-        we = kFixedScale / lam;
+        we = 256.0f / lam;  // 2^40 / 2^32
       } else {
         float ge = a.g ? a.g[orig] : a.lambda[orig];
         if (!a.g) ge = ge > 0.0f ? 1.0f / ge : 0.0f;
         if (ge == 0.0f) continue;
-        we = ge * a.fixed_scale;
+        we = ge * a.fixed_scale * 2.3283064365386963e-10f;  // / 2^32
       }
-      for (int base = 0; base < nrows; base += 64) {
-        const int r = base + lane;
-        RowSeg s{0, 0, 0.0f, 0.0f, 0.0f, 0.0f};
-        if (r < nrows) s = row_setup<false>(sg, sk, g, g.y0 + r);
-        const int w0 = s.len > 0 ? row0 + r * pitch + s.lo : 0;
-        const int maxlen = wave_max(s.len);
-        const float lof = (float)s.lo;
-        const float sc = s.scale * we;
-        for (int k = 0; k < maxlen; k += kUnroll) {
+      const bool neg = we < 0.0f;  // uniform: one sign per event
+      const float wabs = fabsf(we);
 #pragma unroll
-          for (int u = 0; u < kUnroll; ++u) {
-            const int kk = k + u;
-            if (kk < s.len) {
-              float uu;
-              const float gv = pixel_gauss(sg, g, lof + (float)kk, s.ndy, s_pix, uu);
-              float v = gv * sc;
-              if constexpr (kRho) v *= rho_w[w0 + kk];
-              // two's complement adds: signed values accumulate exactly
-              atomicAdd(&acc_w[w0 + kk], (U)__float2ll_rn(v));
+      for (int gi = 0; gi < kCachedGroups + 1; ++gi) {
+        for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
+          if (gg >= ngroups) break;
+          RowSeg s;
+          if (kPass1 && gi < kCachedGroups) {
+            s = cache[gi];
+          } else {
+            const int r = gg * 64 + lane;
+            s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+            if (r < nrows) s = row_setup<false>(sg, sk, g, g.y0 + r, row0 + r * pitch, s_pix);
+          }
+          const int maxlen = wave_max(s.len);
+          for (int k = 0; k < maxlen; k += kUnroll) {
+            const float kf = (float)k;
+            U v[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+              const float uu = fmaf(kf + (float)u, du, s.u0);
+              float w = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * wabs;
+              if constexpr (kRho) w *= rho_w[s.w0 + k + u];
+              v[u] = split_u64(w);
+              if (neg) v[u] = 0ull - v[u];  // two's complement: signed sums stay exact
             }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u)
+              if (k + u < s.len) atomicAdd(&acc_w[s.w0 + k + u], v[u]);
           }
         }
       }
     }
   }
 
-  if constexpr (MODE == MODE_FORWARD) {
-    // per-workgroup reduction of the scalars: lane 0 of each wave holds them
+  if constexpr (kSigma) {
+    // lane 0 of each wave holds the wave's scalars
     if (lane == 0) {
       unsafeAtomicAdd(&a.scalars[0], ll);
       unsafeAtomicAdd(&a.scalars[1], gsz);
       unsafeAtomicAdd(&a.scalars[2], gsdl);
-      // positive doubles order like their bit patterns
-      atomicMax(reinterpret_cast<unsigned long long*>(&a.scalars[3]),
-                (U)__double_as_longlong(mmax));
+      // non-negative doubles order like their bit patterns
+      atomicMax(reinterpret_cast<U*>(&a.scalars[3]), (U)__double_as_longlong(mmax));
     }
   }
 
@@ -360,9 +380,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
 
 // ---------------------------------------------------------------------------
 
-size_t fused_lds_bytes(int max_window_px, int mode) {
+size_t fused_lds_bytes(int max_window_px, int max_pitch, int mode) {
   size_t npx = (size_t)max_window_px;
-  size_t rho = mode == MODE_BACKWARD ? 0 : ((npx * 4 + 15) & ~(size_t)15);
+  size_t rho = mode == MODE_BACKWARD
+                   ? 0
+                   : (((npx + 2 * (size_t)max_pitch + 8) * 4 + 15) & ~(size_t)15);
   size_t acc = mode == MODE_FORWARD ? 0 : npx * 8;
   return rho + acc + 16;
 }
