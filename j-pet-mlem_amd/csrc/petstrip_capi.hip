@@ -188,6 +188,8 @@ int reserve_events(petstrip_handle* h, int64_t n) {
 
 constexpr int kBin = 16;          // spatial bin edge in pixels
 constexpr int kChunk = 1024;      // events per work item
+// MLEM backprojection values K rho / lambda <= 1 are summed in 2^-31 units
+constexpr int kMlemFixedBits = 31;
 
 // Along a box row the reference's float ellipse test has no holes when the
 // quadratic's column-to-column curvature C*pw^2 dominates the rounding of q
@@ -324,8 +326,9 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.g = nullptr;
   a.acc = h->d_acc;
   a.acc_kind = petstrip::GLOBAL_F32;
-  a.fixed_scale = 1.0f;
-  a.inv_fixed_scale = 1.0 / 1099511627776.0;  // 2^-40
+  a.log2_scale = (float)kMlemFixedBits;
+  a.split = 0;
+  a.inv_fixed_scale = std::ldexp(1.0, -kMlemFixedBits);
   a.scalars = h->d_scalars;
   return a;
 }
@@ -344,7 +347,7 @@ int one_iteration(petstrip_handle* h) {
                                          accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
                                          h->n_items, h->lds_bytes));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
-    // float accumulators already hold image units; the u64 one holds 2^-40
+    // float accumulators already hold image units; the u64 one holds 2^-31
     double unit = a.acc_kind == petstrip::GLOBAL_U64 ? a.inv_fixed_scale : 1.0;
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, a.acc_kind, unit, 0.0f, 1,
                                       h->d_flag, h->d_out, (int64_t)h->n_pix));
@@ -592,8 +595,8 @@ int petstrip_iterate(petstrip_handle* h, int n_iterations) {
     if (flag) {
       HIP_TRY(hipMemset(h->d_flag, 0, sizeof(int)));
       return fail(PETSTRIP_ESTATE,
-                  "fixed-point image overflow: a pixel exceeded 2^23 in 2^-40 units "
-                  "(use the fp32 or compensated accumulation)");
+                  "fixed-point image overflow: a pixel sum exceeded 2^32 in 2^-31 "
+                  "units");
     }
   }
   return PETSTRIP_OK;
@@ -743,17 +746,24 @@ int backward_per_event(petstrip_handle* h, const float* g, float* grad_rho,
 }
 
 // Fused form: one FORWARD pass (lambda, sigma derivative, log-likelihood
-// terms, largest |g| K) and one BACKWARD pass (K g_e into the image in
-// signed fixed point with a scale that keeps every sum below 2^62).
+// terms, largest and mean |g| K) and one BACKWARD pass (K g_e into the image
+// in signed fixed point).  For non-negative weights whose largest |g| K is
+// within a factor 256 of the mean, the unit 2^-k puts the largest value just
+// below 2^31 (one conversion per value; every sum is positive, so the
+// absolute rounding is small against it).  Otherwise (weights of both
+// signs, whose sums cancel, or an outlier) values may reach 2^62 / n_events
+// (split conversion).  Either way no per-pixel sum can exceed 2^63.
 int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
                    double grad_sigma[2], double* ll_terms) {
   const int64_t n = h->n_events;
   petstrip::FusedArgs a = fused_args(h);
+  bool mixed_signs = false;
   if (g) {
     if (n > 0)
       HIP_TRY(hipMemcpyAsync(h->d_inv, g, n * sizeof(float), hipMemcpyHostToDevice,
                              h->stream));
     a.g = h->d_inv;
+    for (int64_t i = 0; i < n && !mixed_signs; ++i) mixed_signs = g[i] < 0.0f;
   }
   HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
   HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
@@ -772,13 +782,16 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
   *ll_terms = scal[0];
   if (!grad_rho) return PETSTRIP_OK;
   const float bias = g ? 0.0f : -1.0f;
-  const double bound = (double)std::max<int64_t>(n, 1) * scal[3];
-  if (!(bound > 0.0)) {
+  const double mmax = scal[3], mmean = scal[4] / (double)std::max<int64_t>(n, 1);
+  if (!(mmax > 0.0)) {
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_out, bias, (int64_t)h->n_pix));
   } else {
-    int k = 62 - (int)std::ceil(std::log2(bound));
+    const bool split = mixed_signs || mmax > 256.0 * mmean || n >= ((int64_t)1 << 31);
+    int k = split ? 62 - (int)std::ceil(std::log2((double)n * mmax))
+                  : 31 - (int)std::ceil(std::log2(mmax));
     k = std::max(-100, std::min(100, k));
-    a.fixed_scale = std::ldexp(1.0f, k);
+    a.log2_scale = (float)k;
+    a.split = split ? 1 : 0;
     a.inv_fixed_scale = std::ldexp(1.0, -k);
     a.acc_kind = petstrip::GLOBAL_U64;
     HIP_TRY(hipEventRecord(h->evk0, h->stream));

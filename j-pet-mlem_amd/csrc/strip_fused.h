@@ -32,9 +32,12 @@ struct FusedArgs {
   const float* g;            // upstream weights per event, or null for 1/lambda
   void* acc;                 // W*H global accumulator (MLEM, BACKWARD)
   int acc_kind;              // GLOBAL_*
-  float fixed_scale;         // BACKWARD: fixed-point scale of K g_e
-  double inv_fixed_scale;    // fixed-point unit when flushing to floats
-  double* scalars;           // FORWARD: {sum log lambda, dl/dsz, dl/dsdl, max |g| K}
+  float log2_scale;          // k: pass-2 values are added in units 2^-k
+  int split;                 // BACKWARD: values up to 2^62 (else below 2^31)
+  double inv_fixed_scale;    // 2^-k, the unit when flushing to floats
+  double* scalars;           // FORWARD: {sum log lambda, dl/dsz, dl/dsdl,
+                             //           max_e |g_e| K_e, sum_e |g_e| K_e}
+                             // with K_e the event's largest kernel value
 };
 
 hipError_t launch_bin_count(hipStream_t st, const EventGeom* geom, int64_t n,

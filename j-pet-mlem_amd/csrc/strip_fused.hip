@@ -4,26 +4,37 @@
 // Reference semantics: Reconstruction::bb_update
 // (src/2d/strip/reconstruction.h:220-298) per event, Jacobi over events
 // (reconstruction.h:121-149).  One wave handles one event at a time; its
-// lanes take the event's box rows (64 at a time).  Each lane
-//   1. finds its row's exact ellipse column interval (in_ellipse with the
-//      reference's float operation order, gaussian_kernel.h:131-137) and
-//      the row factor of the kernel;
+// lanes take the event's box rows, 64 at a time, the 64 rows around the
+// ellipse centre first (the longest rows share a group, so the short ones
+// do not pad it).  Each lane
+//   1. finds its row's exact ellipse column interval: the roots of the
+//      row's quadratic give the two boundary columns to within a small
+//      fraction of a pixel, and the reference's own float test
+//      (in_ellipse, gaussian_kernel.h:131-137) decides each boundary column
+//      and its outer neighbour; rows the estimate cannot settle take a
+//      scanning path;
 //   2. pass 1: walks the interval summing K * rho read from the LDS window;
 //      the DPP wave sum is the denominator lambda_e (the forward projection);
 //   3. pass 2: walks it again adding K * w_e (* rho) into the LDS
-//      accumulation window (ds_add_u64 of fixed-point values).
+//      accumulation window as 64-bit fixed point.
 // Modes:
-//   MLEM      pass 1 + pass 2 with w_e = 1/lambda_e, values K rho / lambda_e
-//             in 2^-40 fixed point (each <= 1)
+//   MLEM      pass 1 + pass 2 with w_e = 1/lambda_e; each value K rho /
+//             lambda_e <= 1 is added in 2^-31 units
 //   FORWARD   pass 1 only, plus the sigma derivative of lambda_e and the
 //             log-likelihood terms, reduced per workgroup into doubles
-//   BACKWARD  pass 2 only with upstream weights g_e, values K g_e in signed
-//             fixed point with a scale chosen from the largest K g_e
-// The inner walks are branch-free where it matters and unrolled by four so
-// that each lane keeps several LDS reads and exponentials in flight.  The
-// window pitch is odd so that lanes on neighbouring rows of a tilted
-// ellipse fall in different LDS banks.  At the end the window is added to
-// the global accumulator with one atomic per touched pixel.
+//   BACKWARD  pass 2 only with upstream weights g_e, values |K g_e| in
+//             units 2^-k chosen on the host, added or subtracted by the
+//             sign of g_e: each value below 2^31 when the largest |g_e| K
+//             is within a factor 256 of the mean, else (split) each below
+//             2^62 / n_events
+// A value x becomes the integer round(x) with one fused multiply-add of 0.5
+// and one float->u32 conversion; the event weight is folded into the
+// exponent of K (below), so the pixel step of pass 2 is four VALU
+// operations plus the LDS read and the LDS atomic.  The inner walks are
+// unrolled by four so that each lane keeps several LDS reads and
+// exponentials in flight.  The window pitch is odd so that lanes on
+// neighbouring rows fall in different LDS banks.  At the end the window is
+// added to the global accumulator with one atomic per touched pixel.
 //
 // Kernel value (GaussianKernel::operator(), gaussian_kernel.h:51-79): the
 // exponent splits into a row part and a pixel part,
@@ -61,14 +72,20 @@ __device__ __forceinline__ float wave_sum(float v) {
   return (r0 + r1) + (r2 + r3);
 }
 
+// Wave64 maximum of a non-negative int (same DPP pattern as wave_sum).
+__device__ __forceinline__ int wave_max_i(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x124, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x128, 0xf, 0xf, false));
+  int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+  int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+  return max(max(r0, r1), max(r2, r3));
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
-}
-
-__device__ __forceinline__ int wave_max(int v) {
-  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return uniform(v);
 }
 
 __device__ __forceinline__ float wave_max_f(float v) {
@@ -109,6 +126,14 @@ __device__ __forceinline__ bool exact_in(const StripConst& s, const EventGeom& g
 
 constexpr float kLog2e = 1.44269504088896341f;
 
+// Per-event values every row of the event uses.
+struct EventRows {
+  float zc;    // ellipse centre in column coordinates, (z - tl_z_half_w) / pw
+  float kcol;  // 1 / (2 C pw): root offset in metres -> columns
+  float c4;    // 4 C
+  float c005;  // 4 C * 0.05
+};
+
 // One box row of an event as a lane walks it.
 struct RowSeg {
   int len;        // ellipse pixels in the row (0: nothing to do)
@@ -121,50 +146,74 @@ struct RowSeg {
 
 template <bool SIGMA>
 __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
-                                            const EventGeom& g, int iy, int row_w,
-                                            float s_pix) {
+                                            const EventGeom& g, const EventRows& ev,
+                                            int iy, int row_w, float s_pix) {
   RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
-  float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
-  float Ady2 = (dy * g.A) * dy;
-  float Bdy = dy * g.B;
-  // Superset of the interval from the real roots of C dz^2 + Bdy dz + Ady2 =
-  // 9.05, then the exact test trims both ends.  Along a row the exact test
-  // has no holes: adjacent columns differ in q by at least C pw^2 (checked
-  // on the host), far above the rounding of q.
-  float disc = Bdy * Bdy - 4.0f * g.C * (Ady2 - 9.05f);
-  if (!(disc >= 0.0f)) return s;
-  float sq = sqrtf(disc);
-  float inv2c = 0.5f / g.C;
-  float zlo = g.z + (-Bdy - sq) * inv2c;
-  float zhi = g.z + (-Bdy + sq) * inv2c;
-  float flo = floorf((zlo - sg.tl_z_half_w) / sg.pw) - 1.0f;
-  float fhi = ceilf((zhi - sg.tl_z_half_w) / sg.pw) + 1.0f;
-  int a = (flo > (float)g.x0) ? (int)flo : g.x0;
-  int b = (fhi < (float)(g.x1 - 1)) ? (int)fhi : g.x1 - 1;
-  while (a <= b && !exact_in(sg, g, Ady2, Bdy, a)) ++a;
-  while (b >= a && !exact_in(sg, g, Ady2, Bdy, b)) --b;
+  const float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
+  const float Ady2 = (dy * g.A) * dy;
+  const float Bdy = dy * g.B;
+  // C dz^2 + Bdy dz + Ady2 - 9: discriminant at the threshold; at 9.05 it
+  // bounds what the float test can accept (its rounding is ~1e-6)
+  const float d9 = fmaf(Bdy, Bdy, -ev.c4 * (Ady2 - 9.0f));
+  if (!(d9 + ev.c005 >= 0.0f)) return s;
+  const float sq = __builtin_amdgcn_sqrtf(fmaxf(d9, 0.0f));
+  const float clo = fmaf(-(Bdy + sq), ev.kcol, ev.zc);  // first column, estimated
+  const float chi = fmaf(sq - Bdy, ev.kcol, ev.zc);     // last column, estimated
+  const int r = (int)floorf(clo + 0.5f);
+  const int q = (int)floorf(chi + 0.5f);
+  // Rows are hole-free intervals under the float test (checked on the host
+  // from the constants), so a boundary is settled once a column and its
+  // outer neighbour disagree.
+  const bool in_r = exact_in(sg, g, Ady2, Bdy, r);
+  const bool in_q = exact_in(sg, g, Ady2, Bdy, q);
+  const bool in_r2 = exact_in(sg, g, Ady2, Bdy, in_r ? r - 1 : r + 1);
+  const bool in_q2 = exact_in(sg, g, Ady2, Bdy, in_q ? q + 1 : q - 1);
+  int a = in_r ? r : r + 1;
+  int b = in_q ? q : q - 1;
+  const bool left = in_r != in_r2, right = in_q != in_q2;
+  // no column of [r-1, q+1] passes: the row is empty when the estimated
+  // interval is shorter than two columns (every column in it was tested)
+  const bool none = !(in_r || in_r2 || in_q || in_q2) && (chi - clo < 2.0f);
+  if (!((left && right) || none)) {
+    // the estimate did not settle the row: scan a superset (roots at 9.05
+    // widened by a column) from both ends
+    const float sq2 = __builtin_amdgcn_sqrtf(d9 + ev.c005);
+    const float flo = floorf(fmaf(-(Bdy + sq2), ev.kcol, ev.zc)) - 1.0f;
+    const float fhi = ceilf(fmaf(sq2 - Bdy, ev.kcol, ev.zc)) + 1.0f;
+    a = (flo > (float)g.x0) ? (int)flo : g.x0;
+    b = (fhi < (float)(g.x1 - 1)) ? (int)fhi : g.x1 - 1;
+#pragma nounroll
+    while (a <= b && !exact_in(sg, g, Ady2, Bdy, a)) ++a;
+#pragma nounroll
+    while (b >= a && !exact_in(sg, g, Ady2, Bdy, b)) --b;
+  } else if (none) {
+    return s;
+  }
+  a = max(a, g.x0);
+  b = min(b, g.x1 - 1);
   if (a > b) return s;
   s.len = b - a + 1;
   s.w0 = row_w + a;
   // pq at the first pixel, with the reference's rounding (gaussian_kernel.h:64)
-  float dz = fmaf((float)a, sg.pw, sg.tl_z_half_w) - g.z;
+  const float dz = fmaf((float)a, sg.pw, sg.tl_z_half_w) - g.z;
   s.u0 = fmaf(-dy, g.t, dz) * s_pix;
   // kernel row factor (gaussian_kernel.h:58-78)
-  float sec_sq = g.sec * g.sec;
-  float yp = dy + g.y;
-  float ap = -(yp - sk.R) * sec_sq;
-  float aq = -(sk.R + yp) * sec_sq;
-  float ar = ((yp * -2.0f) * g.sec) * g.t;
-  float br = (dy * -2.0f) * g.sec;
-  float norm = fmaf(ar * sk.isdl, ar, fmaf(ap * sk.isz, ap, (aq * sk.isz) * aq));
-  float brb = (br * sk.isdl) * br;
-  s.lsc = log2f(sk.c) - 0.5f * log2f(norm) - (0.5f * kLog2e) * brb;
+  const float sec_sq = g.sec * g.sec;
+  const float yp = dy + g.y;
+  const float ap = -(yp - sk.R) * sec_sq;
+  const float aq = -(sk.R + yp) * sec_sq;
+  const float ar = ((yp * -2.0f) * g.sec) * g.t;
+  const float br = (dy * -2.0f) * g.sec;
+  const float norm = fmaf(ar * sk.isdl, ar, fmaf(ap * sk.isz, ap, (aq * sk.isz) * aq));
+  const float brb = (br * sk.isdl) * br;
+  s.lsc = __builtin_amdgcn_logf(sk.c) - 0.5f * __builtin_amdgcn_logf(norm) -
+          (0.5f * kLog2e) * brb;
   if constexpr (SIGMA) {
     // ln K = ln c - ln(norm)/2 - bSb/2 with c ~ 1/(sz^2 sdl), norm and bSb
     // linear in 1/sz^2 and 1/sdl^2:
     //   d lnK/d sz  = (-2 + isz (ap^2 + aq^2)/norm + 2 isz pq^2) / sz
     //   d lnK/d sdl = (-1 + isdl ar^2/norm + isdl br^2) / sdl
-    float apq = fmaf(ap, ap, aq * aq);
+    const float apq = fmaf(ap, ap, aq * aq);
     s.dz_row = (-2.0f + sk.isz * apq / norm) / sk.sigma_z;
     s.ddl_row = (-1.0f + sk.isdl * (ar * ar) / norm + brb) / sk.sigma_dl;
   }
@@ -174,12 +223,76 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
 constexpr int kUnroll = 4;
 constexpr int kCachedGroups = 4;  // row groups kept in registers between passes
 
-// x in [0, 2^63): the two 32-bit halves of round-toward-zero(x), x given as
-// w = x * 2^-32 (five VALU operations instead of a generic float->i64).
-__device__ __forceinline__ unsigned long long split_u64(float w) {
-  float hi = floorf(w);
-  float lo = (w - hi) * 4294967296.0f;
-  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+// k + sgn * log2(x) for x > 0 and integer k, formed as (k + sgn e) + sgn
+// log2(m) with x = m 2^e: scaling x by a power of two while k moves the
+// other way gives the same float, so pass-2 values scale exactly.
+__device__ __forceinline__ float log2_shift(float k, float x, float sgn) {
+  const int e = __builtin_amdgcn_frexp_expf(x);
+  const float m = __builtin_amdgcn_frexp_mantf(x);
+  return fmaf(sgn, (float)e, k) + sgn * __builtin_amdgcn_logf(m);
+}
+
+// Pass 2 over one row group.  FOLD: the exponent already carries log2 of the
+// event multiplier (lscw = lsc + lw).  Otherwise (MLEM events whose folded
+// exponent could overflow) x = (K rho / lambda) 2^bits, computed without
+// forming the large multiplier.  SPLIT: values reach 2^62; the exponent
+// carries a further 2^-32 and the two 32-bit halves are converted apart.
+template <bool RHO, bool FOLD, bool SPLIT, bool SUB>
+__device__ __forceinline__ void backproject_rows(const RowSeg& s, int maxlen, float du,
+                                                 float lscw, float lam, float unit_scale,
+                                                 const float* rho_w,
+                                                 unsigned long long* acc_w) {
+  const float* rp = rho_w + s.w0;
+  unsigned long long* ap = acc_w + s.w0;
+  const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
+  float kf = 0.0f;
+  for (int k = 0; k < maxlen; k += kUnroll, kf += (float)kUnroll) {
+    unsigned lo[kUnroll], hi[kUnroll];
+    float rv[kUnroll];
+    if constexpr (RHO) {
+      // issue the four LDS reads together
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) rv[u] = rp[k + u];
+      asm volatile("" : "+v"(rv[0]), "+v"(rv[1]), "+v"(rv[2]), "+v"(rv[3]));
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const float uu = fmaf(kf, du, uo[u]);
+      const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, lscw));
+      if constexpr (SPLIT) {
+        // kv = x 2^-32 < 2^30
+        hi[u] = (unsigned)kv;
+        lo[u] = (unsigned)fmaf(__builtin_amdgcn_fractf(kv), 4294967296.0f, 0.5f);
+      } else {
+        float x;
+        if constexpr (FOLD) {
+          if constexpr (RHO)
+            x = fmaf(kv, rv[u], 0.5f);
+          else
+            x = kv + 0.5f;
+        } else {
+          x = fmaf((kv * rv[u]) / lam, unit_scale, 0.5f);
+        }
+        lo[u] = (unsigned)x;  // round(x), x in [0.5, 2^32)
+        hi[u] = 0u;
+      }
+      // keep the four values (and their LDS reads) ahead of the masked
+      // atomics instead of letting each sink into its own branch
+      asm volatile("" : "+v"(lo[u]));
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const unsigned long long v = ((unsigned long long)hi[u] << 32) | lo[u];
+      if (k + u < s.len) {
+        if constexpr (SUB)
+          __hip_atomic_fetch_sub(ap + k + u, v, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+          __hip_atomic_fetch_add(ap + k + u, v, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -224,8 +337,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
 
   const float s_pix = sqrtf(sk.isz * kLog2e);  // sqrt(isz log2 e)
   const float du = sg.pw * s_pix;               // u step per column
+  const float inv_pw = 1.0f / sg.pw;
+  const float inv_ph = 1.0f / sg.ph;
   const float cz_pix = 2.0f / (kLog2e * sk.sigma_z);  // 2 isz pq^2 / sz = cz u^2
-  double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0;
+  const float unit_scale = __builtin_amdgcn_exp2f(a.log2_scale);
+  double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0, msum = 0.0;
 
   for (int e = it.begin + wave; e < it.end; e += WAVES) {
     const EventGeom g = a.sgeom[e];
@@ -233,6 +349,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
     const int ngroups = (nrows + 63) >> 6;
     const int row0 = (g.y0 - it.wy0) * pitch - it.wx0;  // window index of (g.y0, 0)
     const int orig = a.sidx[e];
+    EventRows ev;
+    ev.c4 = 4.0f * g.C;
+    ev.c005 = ev.c4 * 0.05f;
+    ev.kcol = 0.5f * inv_pw * __builtin_amdgcn_rcpf(g.C);
+    ev.zc = (g.z - sg.tl_z_half_w) * inv_pw;
+    // group 0 takes the 64 rows around the ellipse centre, later groups the
+    // rows above and below it in order
+    const int mid = (int)((sg.tl_y - g.y) * inv_ph) - g.y0;
+    const int m0 = nrows > 64 ? min(max(mid - 32, 0), nrows - 64) : 0;
     RowSeg cache[kCachedGroups];
 
     // ---- pass 1: denominator (and sigma sums) -------------------------------
@@ -244,29 +369,38 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
         // groups beyond the cached ones share the last iteration as a loop
         for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
           if (gg >= ngroups) break;
-          const int r = gg * 64 + lane;
+          const int j = gg * 64 + lane - 64;
+          const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
-          if (r < nrows) s = row_setup<kSigma>(sg, sk, g, g.y0 + r, row0 + r * pitch, s_pix);
+          if (gg * 64 + lane < nrows)
+            s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r * pitch, s_pix);
           if (gi < kCachedGroups) cache[gi] = s;
-          const int maxlen = wave_max(s.len);
-          float s0 = 0.0f, s1 = 0.0f;
-          for (int k = 0; k < maxlen; k += kUnroll) {
-            const float kf = (float)k;
+          const int maxlen = wave_max_i(s.len);
+          const float* rp = rho_w + s.w0;
+          const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0),
+                                     fmaf(3.0f, du, s.u0)};
+          float s0 = 0.0f, s1 = 0.0f, kf = 0.0f;
+          for (int k = 0; k < maxlen; k += kUnroll, kf += (float)kUnroll) {
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-              const float uu = fmaf(kf + (float)u, du, s.u0);
-              const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc));
-              float kr = kv * rho_w[s.w0 + k + u];
-              kr = (k + u < s.len) ? kr : 0.0f;
-              if constexpr (COMP) {
-                float sum, err;
-                two_sum(s0, kr, sum, err);
-                s0 = sum;
-                comp += err;
+              const float uu = fmaf(kf, du, uo[u]);
+              float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc));
+              kv = (k + u < s.len) ? kv : 0.0f;
+              const float rho = rp[k + u];
+              if constexpr (COMP || kSigma) {
+                const float kr = kv * rho;
+                if constexpr (COMP) {
+                  float sum, err;
+                  two_sum(s0, kr, sum, err);
+                  s0 = sum;
+                  comp += err;
+                } else {
+                  s0 += kr;
+                }
+                if constexpr (kSigma) s1 = fmaf(kr, uu * uu, s1);
               } else {
-                s0 += kr;
+                s0 = fmaf(kv, rho, s0);
               }
-              if constexpr (kSigma) s1 = fmaf(kr, uu * uu, s1);
             }
           }
           if constexpr (COMP) {
@@ -295,26 +429,31 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
           if (lam > 0.0f) ll += log((double)lam);
           gsz += (double)ge * wsz;
           gsdl += (double)ge * wsdl;
-          mmax = fmax(mmax, (double)fabsf(ge) * (double)kmax);
+          const double m = (double)fabsf(ge) * (double)kmax;
+          mmax = fmax(mmax, m);
+          msum += m;
         }
       }
     }
 
     // ---- pass 2: backprojection into the LDS window --------------------------
     if constexpr (kPass2) {
-      // value multiplier, including the fixed-point unit / 2^32 (split_u64)
-      float we;
+      // lw = log2 of the event multiplier in fixed-point units
+      float lw;
+      bool neg = false;
       if constexpr (MODE == MODE_MLEM) {
         if (!(lam > 0.0f)) continue;
-        we = 256.0f / lam;  // 2^40 / 2^32
-      } else {
-        float ge = a.g ? a.g[orig] : a.lambda[orig];
-        if (!a.g) ge = ge > 0.0f ? 1.0f / ge : 0.0f;
+        lw = log2_shift(a.log2_scale, lam, -1.0f);
+      } else if (a.g) {
+        const float ge = a.g[orig];
         if (ge == 0.0f) continue;
-        we = ge * a.fixed_scale * 2.3283064365386963e-10f;  // / 2^32
+        neg = ge < 0.0f;  // uniform: one sign per event
+        lw = log2_shift(a.log2_scale, fabsf(ge), 1.0f);
+      } else {
+        const float l = a.lambda[orig];
+        if (!(l > 0.0f)) continue;
+        lw = log2_shift(a.log2_scale, l, -1.0f);
       }
-      const bool neg = we < 0.0f;  // uniform: one sign per event
-      const float wabs = fabsf(we);
 #pragma unroll
       for (int gi = 0; gi < kCachedGroups + 1; ++gi) {
         for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
@@ -323,25 +462,40 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
           if (kPass1 && gi < kCachedGroups) {
             s = cache[gi];
           } else {
-            const int r = gg * 64 + lane;
+            const int j = gg * 64 + lane - 64;
+            const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
             s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
-            if (r < nrows) s = row_setup<false>(sg, sk, g, g.y0 + r, row0 + r * pitch, s_pix);
+            if (gg * 64 + lane < nrows)
+              s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r * pitch, s_pix);
           }
-          const int maxlen = wave_max(s.len);
-          for (int k = 0; k < maxlen; k += kUnroll) {
-            const float kf = (float)k;
-            U v[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-              const float uu = fmaf(kf + (float)u, du, s.u0);
-              float w = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * wabs;
-              if constexpr (kRho) w *= rho_w[s.w0 + k + u];
-              v[u] = split_u64(w);
-              if (neg) v[u] = 0ull - v[u];  // two's complement: signed sums stay exact
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u)
-              if (k + u < s.len) atomicAdd(&acc_w[s.w0 + k + u], v[u]);
+          const int maxlen = wave_max_i(s.len);
+          const float lscw = s.lsc + lw;
+          if constexpr (MODE == MODE_MLEM) {
+            // K rho 2^bits / lambda <= 2^bits, but K 2^bits / lambda alone
+            // overflows a float when lambda is tiny
+            if (__any(lscw > 120.0f))
+              backproject_rows<true, false, false, false>(s, maxlen, du, s.lsc, lam,
+                                                          unit_scale, rho_w, acc_w);
+            else
+              backproject_rows<true, true, false, false>(s, maxlen, du, lscw, lam,
+                                                         unit_scale, rho_w, acc_w);
+          } else if (a.split) {
+            // |g| K 2^k <= 2^62 / n_events by the choice of k
+            const float l32 = lscw - 32.0f;
+            if (neg)
+              backproject_rows<false, true, true, true>(s, maxlen, du, l32, 0.0f,
+                                                        unit_scale, rho_w, acc_w);
+            else
+              backproject_rows<false, true, true, false>(s, maxlen, du, l32, 0.0f,
+                                                         unit_scale, rho_w, acc_w);
+          } else {
+            // |g| K 2^k <= 2^31 by the choice of k
+            if (neg)
+              backproject_rows<false, true, false, true>(s, maxlen, du, lscw, 0.0f,
+                                                         unit_scale, rho_w, acc_w);
+            else
+              backproject_rows<false, true, false, false>(s, maxlen, du, lscw, 0.0f,
+                                                          unit_scale, rho_w, acc_w);
           }
         }
       }
@@ -356,6 +510,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
       unsafeAtomicAdd(&a.scalars[2], gsdl);
       // non-negative doubles order like their bit patterns
       atomicMax(reinterpret_cast<U*>(&a.scalars[3]), (U)__double_as_longlong(mmax));
+      unsafeAtomicAdd(&a.scalars[4], msum);
     }
   }
 
