@@ -232,65 +232,174 @@ __device__ __forceinline__ float log2_shift(float k, float x, float sgn) {
   return fmaf(sgn, (float)e, k) + sgn * __builtin_amdgcn_logf(m);
 }
 
-// Pass 2 over one row group.  FOLD: the exponent already carries log2 of the
-// event multiplier (lscw = lsc + lw).  Otherwise (MLEM events whose folded
-// exponent could overflow) x = (K rho / lambda) 2^bits, computed without
-// forming the large multiplier.  SPLIT: values reach 2^62; the exponent
-// carries a further 2^-32 and the two 32-bit halves are converted apart.
+// Pass-1 sums over the pixels rp[0..3] (u = uo[i] + kf du): t0 = sum K rho,
+// t1 = sum K rho u^2 (SIGMA); TAIL keeps only the first rem pixels.
+template <bool SIGMA, bool TAIL>
+__device__ __forceinline__ void forward_values(const float* rp, float kf, float du,
+                                               const float (&uo)[kUnroll], float lsc,
+                                               int rem, float& t0, float& t1) {
+  float rv[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u];
+  t0 = 0.0f;
+  t1 = 0.0f;
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const float uu = fmaf(kf, du, uo[u]);
+    float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, lsc));
+    if constexpr (TAIL) kv = (u < rem) ? kv : 0.0f;
+    if constexpr (SIGMA) {
+      const float kr = kv * rv[u];
+      t0 += kr;
+      t1 = fmaf(kr, uu * uu, t1);
+    } else {
+      t0 = fmaf(kv, rv[u], t0);
+    }
+  }
+}
+
+// Pass 1 over one row group: the lane's full groups of four pixels, each
+// group's sum masked once, then its last len % 4 pixels at its own offset.
+// s0 += sum K rho (two_sum error into comp when COMP), s1 += sum K rho u^2.
+template <bool COMP, bool SIGMA>
+__device__ __forceinline__ void forward_rows(const RowSeg& s, float du, const float* rho_w,
+                                             float& s0, float& comp, float& s1) {
+  const float* rp = rho_w + s.w0;
+  const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
+  const int nfull = s.len >> 2;
+  const int maxfull = uniform(wave_max_i(nfull));
+  auto add = [&](float t0, float t1) {
+    if constexpr (COMP) {
+      float sum, err;
+      two_sum(s0, t0, sum, err);
+      s0 = sum;
+      comp += err;
+    } else {
+      s0 += t0;
+    }
+    if constexpr (SIGMA) s1 += t1;
+  };
+  float kf = 0.0f;
+  for (int gk = 0; gk < maxfull; ++gk, kf += (float)kUnroll) {
+    float t0, t1;
+    forward_values<SIGMA, false>(rp + gk * kUnroll, kf, du, uo, s.lsc, 0, t0, t1);
+    const bool on = gk < nfull;
+    add(on ? t0 : 0.0f, on ? t1 : 0.0f);
+  }
+  const int rem = s.len & (kUnroll - 1);
+  if (__any(rem > 0)) {
+    const int t = nfull * kUnroll;
+    float t0, t1;
+    forward_values<SIGMA, true>(rp + t, (float)t, du, uo, s.lsc, rem, t0, t1);
+    add(t0, t1);
+  }
+}
+
+// The four pass-2 values of the pixels rp[0..3] (u = uo[i] + kf du): K * the
+// event multiplier (* rho), rounded to integers and split into 32-bit
+// halves.  FOLD: the exponent already carries log2 of the event multiplier
+// (lscw = lsc + lw).  Otherwise (MLEM events whose folded exponent could
+// overflow) x = (K rho / lambda) 2^bits is formed without the large
+// multiplier.  SPLIT: values reach 2^62; the exponent carries a further
+// 2^-32 and the halves are converted apart.
+template <bool RHO, bool FOLD, bool SPLIT>
+__device__ __forceinline__ void backproject_values(const float* rp, float kf, float du,
+                                                   const float (&uo)[kUnroll], float lscw,
+                                                   float lam, float unit_scale,
+                                                   unsigned (&lo)[kUnroll],
+                                                   unsigned (&hi)[kUnroll]) {
+  float rv[kUnroll];
+  if constexpr (RHO) {
+    // issue the four LDS reads together
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u];
+    asm volatile("" : "+v"(rv[0]), "+v"(rv[1]), "+v"(rv[2]), "+v"(rv[3]));
+  }
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const float uu = fmaf(kf, du, uo[u]);
+    const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, lscw));
+    if constexpr (SPLIT) {
+      // kv = x 2^-32 < 2^30
+      hi[u] = (unsigned)kv;
+      lo[u] = (unsigned)fmaf(__builtin_amdgcn_fractf(kv), 4294967296.0f, 0.5f);
+    } else {
+      float x;
+      if constexpr (FOLD) {
+        if constexpr (RHO)
+          x = fmaf(kv, rv[u], 0.5f);
+        else
+          x = kv + 0.5f;
+      } else {
+        x = fmaf((kv * rv[u]) / lam, unit_scale, 0.5f);
+      }
+      lo[u] = (unsigned)x;  // round(x), x in [0.5, 2^32)
+      hi[u] = 0u;
+    }
+    // keep the values (and their LDS reads) ahead of the masked atomics
+    // instead of letting each sink into its own branch
+    asm volatile("" : "+v"(lo[u]));
+  }
+}
+
+template <bool SUB>
+__device__ __forceinline__ void lds_accumulate(unsigned long long* p, unsigned long long v) {
+  if constexpr (SUB)
+    __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// (lo, hi) into a 64-bit register pair that lives across the loop: when hi
+// is the constant 0 the pair's upper register keeps it, instead of a copy
+// of 0 per value.
+__device__ __forceinline__ void set_pair(unsigned long long& v, unsigned lo, unsigned hi,
+                                         bool hi_zero) {
+  if (hi_zero)
+    v = (v & 0xffffffff00000000ull) | lo;
+  else
+    v = ((unsigned long long)hi << 32) | lo;
+}
+
+// Pass 2 over one row group: the lane's full groups of four pixels under
+// one mask per group, then its last len % 4 pixels at its own offset.
 template <bool RHO, bool FOLD, bool SPLIT, bool SUB>
-__device__ __forceinline__ void backproject_rows(const RowSeg& s, int maxlen, float du,
-                                                 float lscw, float lam, float unit_scale,
+__device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, float lscw,
+                                                 float lam, float unit_scale,
                                                  const float* rho_w,
                                                  unsigned long long* acc_w) {
   const float* rp = rho_w + s.w0;
   unsigned long long* ap = acc_w + s.w0;
   const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
+  const int nfull = s.len >> 2;
+  const int maxfull = uniform(wave_max_i(nfull));
+  unsigned lo[kUnroll], hi[kUnroll];
+  unsigned long long pv[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    pv[u] = 0ull;
+    asm volatile("" : "+v"(pv[u]));
+  }
   float kf = 0.0f;
-  for (int k = 0; k < maxlen; k += kUnroll, kf += (float)kUnroll) {
-    unsigned lo[kUnroll], hi[kUnroll];
-    float rv[kUnroll];
-    if constexpr (RHO) {
-      // issue the four LDS reads together
+  for (int gk = 0; gk < maxfull; ++gk, kf += (float)kUnroll) {
+    backproject_values<RHO, FOLD, SPLIT>(rp + gk * kUnroll, kf, du, uo, lscw, lam,
+                                         unit_scale, lo, hi);
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) rv[u] = rp[k + u];
-      asm volatile("" : "+v"(rv[0]), "+v"(rv[1]), "+v"(rv[2]), "+v"(rv[3]));
+    for (int u = 0; u < kUnroll; ++u) set_pair(pv[u], lo[u], hi[u], !SPLIT);
+    if (gk < nfull) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) lds_accumulate<SUB>(ap + gk * kUnroll + u, pv[u]);
     }
+  }
+  const int rem = s.len & (kUnroll - 1);
+  if (__any(rem > 0)) {
+    const int t0 = nfull * kUnroll;
+    backproject_values<RHO, FOLD, SPLIT>(rp + t0, (float)t0, du, uo, lscw, lam,
+                                         unit_scale, lo, hi);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const float uu = fmaf(kf, du, uo[u]);
-      const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, lscw));
-      if constexpr (SPLIT) {
-        // kv = x 2^-32 < 2^30
-        hi[u] = (unsigned)kv;
-        lo[u] = (unsigned)fmaf(__builtin_amdgcn_fractf(kv), 4294967296.0f, 0.5f);
-      } else {
-        float x;
-        if constexpr (FOLD) {
-          if constexpr (RHO)
-            x = fmaf(kv, rv[u], 0.5f);
-          else
-            x = kv + 0.5f;
-        } else {
-          x = fmaf((kv * rv[u]) / lam, unit_scale, 0.5f);
-        }
-        lo[u] = (unsigned)x;  // round(x), x in [0.5, 2^32)
-        hi[u] = 0u;
-      }
-      // keep the four values (and their LDS reads) ahead of the masked
-      // atomics instead of letting each sink into its own branch
-      asm volatile("" : "+v"(lo[u]));
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const unsigned long long v = ((unsigned long long)hi[u] << 32) | lo[u];
-      if (k + u < s.len) {
-        if constexpr (SUB)
-          __hip_atomic_fetch_sub(ap + k + u, v, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-        else
-          __hip_atomic_fetch_add(ap + k + u, v, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
+    for (int u = 0; u < kUnroll - 1; ++u) {
+      set_pair(pv[u], lo[u], hi[u], !SPLIT);
+      if (u < rem) lds_accumulate<SUB>(ap + t0 + u, pv[u]);
     }
   }
 }
@@ -375,34 +484,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
           if (gg * 64 + lane < nrows)
             s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r * pitch, s_pix);
           if (gi < kCachedGroups) cache[gi] = s;
-          const int maxlen = wave_max_i(s.len);
-          const float* rp = rho_w + s.w0;
-          const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0),
-                                     fmaf(3.0f, du, s.u0)};
-          float s0 = 0.0f, s1 = 0.0f, kf = 0.0f;
-          for (int k = 0; k < maxlen; k += kUnroll, kf += (float)kUnroll) {
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-              const float uu = fmaf(kf, du, uo[u]);
-              float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc));
-              kv = (k + u < s.len) ? kv : 0.0f;
-              const float rho = rp[k + u];
-              if constexpr (COMP || kSigma) {
-                const float kr = kv * rho;
-                if constexpr (COMP) {
-                  float sum, err;
-                  two_sum(s0, kr, sum, err);
-                  s0 = sum;
-                  comp += err;
-                } else {
-                  s0 += kr;
-                }
-                if constexpr (kSigma) s1 = fmaf(kr, uu * uu, s1);
-              } else {
-                s0 = fmaf(kv, rho, s0);
-              }
-            }
-          }
+          float s0 = 0.0f, s1 = 0.0f;
+          forward_rows<COMP, kSigma>(s, du, rho_w, s0, comp, s1);
           if constexpr (COMP) {
             float sum, err;
             two_sum(acc, s0, sum, err);
@@ -468,33 +551,32 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
             if (gg * 64 + lane < nrows)
               s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r * pitch, s_pix);
           }
-          const int maxlen = wave_max_i(s.len);
           const float lscw = s.lsc + lw;
           if constexpr (MODE == MODE_MLEM) {
             // K rho 2^bits / lambda <= 2^bits, but K 2^bits / lambda alone
             // overflows a float when lambda is tiny
             if (__any(lscw > 120.0f))
-              backproject_rows<true, false, false, false>(s, maxlen, du, s.lsc, lam,
+              backproject_rows<true, false, false, false>(s, du, s.lsc, lam,
                                                           unit_scale, rho_w, acc_w);
             else
-              backproject_rows<true, true, false, false>(s, maxlen, du, lscw, lam,
+              backproject_rows<true, true, false, false>(s, du, lscw, lam,
                                                          unit_scale, rho_w, acc_w);
           } else if (a.split) {
             // |g| K 2^k <= 2^62 / n_events by the choice of k
             const float l32 = lscw - 32.0f;
             if (neg)
-              backproject_rows<false, true, true, true>(s, maxlen, du, l32, 0.0f,
+              backproject_rows<false, true, true, true>(s, du, l32, 0.0f,
                                                         unit_scale, rho_w, acc_w);
             else
-              backproject_rows<false, true, true, false>(s, maxlen, du, l32, 0.0f,
+              backproject_rows<false, true, true, false>(s, du, l32, 0.0f,
                                                          unit_scale, rho_w, acc_w);
           } else {
             // |g| K 2^k <= 2^31 by the choice of k
             if (neg)
-              backproject_rows<false, true, false, true>(s, maxlen, du, lscw, 0.0f,
+              backproject_rows<false, true, false, true>(s, du, lscw, 0.0f,
                                                          unit_scale, rho_w, acc_w);
             else
-              backproject_rows<false, true, false, false>(s, maxlen, du, lscw, 0.0f,
+              backproject_rows<false, true, false, false>(s, du, lscw, 0.0f,
                                                           unit_scale, rho_w, acc_w);
           }
         }
