@@ -143,9 +143,8 @@ struct petstrip_handle {
   int n_items = 0;
   int max_rows = 1;
   size_t lds_bytes = 0;
-  int fused_waves = 16;
-  int max_px = 1;
-  int max_pitch = 1;
+  int col_stride = 0;  // LDS column stride of the fused windows (0: none fits)
+  int max_ww = 1;
   int* d_flag = nullptr;  // device overflow flag of the fixed-point image
   bool fused_ok = false;
   void* d_acc = nullptr;  // W*H accumulator, 8 bytes per pixel
@@ -186,7 +185,10 @@ int reserve_events(petstrip_handle* h, int64_t n) {
   return PETSTRIP_OK;
 }
 
-constexpr int kBin = 16;          // spatial bin edge in pixels
+// Spatial bins of the box centres: 16 columns by 8 rows keeps the windows
+// of the tall (sigma_dl > sigma_z) ellipses within a 128-row column stride.
+constexpr int kBinW = 16;
+constexpr int kBinH = 8;
 constexpr int kChunk = 1024;      // events per work item
 // MLEM backprojection values K rho / lambda <= 1 are summed in 2^-31 units
 constexpr int kMlemFixedBits = 31;
@@ -205,20 +207,20 @@ bool rows_are_intervals(const StripConst& s) {
   return C * pw * pw > 50.0 * err;
 }
 
-// Sorts events into kBin x kBin bins of their box centre, cuts each bin into
+// Sorts events into kBinW x kBinH bins of their box centre, cuts each bin into
 // chunks of kChunk events and sizes each chunk's LDS window.
 int prepare_fused(petstrip_handle* h) {
   h->fused_ok = false;
   h->n_items = 0;
   const int64_t n = h->n_events;
   const int W = h->sg.W, H = h->sg.H;
-  const int nbx = (W + kBin - 1) / kBin, nby = (H + kBin - 1) / kBin;
+  const int nbx = (W + kBinW - 1) / kBinW, nby = (H + kBinH - 1) / kBinH;
   const int nb = nbx * nby;
   dfree(h->d_count);
   int rc;
   if ((rc = dalloc(&h->d_count, (size_t)nb * 2))) return rc;
   HIP_TRY(hipMemsetAsync(h->d_count, 0, (size_t)nb * 2 * sizeof(int32_t), h->stream));
-  HIP_TRY(petstrip::launch_bin_count(h->stream, h->d_geom, n, kBin, kBin, nbx,
+  HIP_TRY(petstrip::launch_bin_count(h->stream, h->d_geom, n, kBinW, kBinH, nbx,
                                      h->d_bin, h->d_count));
   std::vector<int32_t> cnt((size_t)nb), off((size_t)nb);
   HIP_TRY(hipMemcpyAsync(cnt.data(), h->d_count, nb * sizeof(int32_t),
@@ -258,21 +260,22 @@ int prepare_fused(petstrip_handle* h) {
                            hipMemcpyDeviceToHost, h->stream));
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
-  int max_px = 1, max_rows = 1, max_pitch = 1;
+  int max_ww = 1, max_wh = 1, max_rows = 1;
   for (auto& it : items) {
-    max_px = std::max(max_px, (it.ww | 1) * it.wh);  // odd LDS pitch
-    max_pitch = std::max(max_pitch, it.ww | 1);
+    max_ww = std::max(max_ww, it.ww);
+    max_wh = std::max(max_wh, it.wh);
     max_rows = std::max(max_rows, it.max_rows);
   }
   h->max_rows = max_rows;
-  h->max_px = max_px;
-  h->max_pitch = max_pitch;
-  // Every mode accumulates its LDS window in exact fixed point: ds_add_u64
-  // runs at several times the rate of ds_add_f32 on gfx950 and the window
-  // sums become exact; the accumulation mode only picks the global format.
-  // A workgroup of 16 waves holds one CU's LDS for the larger windows.
-  h->fused_waves = 16;
-  h->lds_bytes = petstrip::fused_lds_bytes(max_px, max_pitch, petstrip::MODE_MLEM);
+  h->max_ww = max_ww;
+  h->col_stride = petstrip::fused_col_stride(max_wh);
+  // Every mode accumulates its LDS window in exact fixed point (ds_add_u64
+  // runs at several times the rate of ds_add_f32 on gfx950); the
+  // accumulation mode only picks the global format.  A workgroup of 16
+  // waves holds one CU's LDS for the larger windows.
+  h->lds_bytes = h->col_stride
+                     ? petstrip::fused_lds_bytes(max_ww, h->col_stride, petstrip::MODE_MLEM)
+                     : (size_t)-1;
   h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
                 h->opts.update == PETSTRIP_UPDATE_CPU &&
                 h->opts.path == PETSTRIP_PATH_AUTO;
@@ -343,7 +346,7 @@ int one_iteration(petstrip_handle* h) {
                                                        : petstrip::GLOBAL_F32;
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
-    HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_MLEM, h->fused_waves,
+    HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_MLEM, h->col_stride,
                                          accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
                                          h->n_items, h->lds_bytes));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
@@ -689,8 +692,8 @@ int petstrip_forward(petstrip_handle* h, float* lambda) {
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
     HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
     HIP_TRY(petstrip::launch_strip_fused(
-        h->stream, petstrip::MODE_FORWARD, h->fused_waves, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_px, h->max_pitch, petstrip::MODE_FORWARD)));
+        h->stream, petstrip::MODE_FORWARD, h->col_stride, 0, a, h->n_items,
+        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_FORWARD)));
   } else {
     HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
                                          h->d_rho, nullptr, h->d_lambda, h->d_inv));
@@ -768,8 +771,8 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
   HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
   HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
   HIP_TRY(petstrip::launch_strip_fused(
-      h->stream, petstrip::MODE_FORWARD, h->fused_waves, 0, a, h->n_items,
-      petstrip::fused_lds_bytes(h->max_px, h->max_pitch, petstrip::MODE_FORWARD)));
+      h->stream, petstrip::MODE_FORWARD, h->col_stride, 0, a, h->n_items,
+      petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_FORWARD)));
   double scal[8];
   HIP_TRY(hipMemcpyAsync(scal, h->d_scalars, sizeof(scal), hipMemcpyDeviceToHost,
                          h->stream));
@@ -796,8 +799,8 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
     a.acc_kind = petstrip::GLOBAL_U64;
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
     HIP_TRY(petstrip::launch_strip_fused(
-        h->stream, petstrip::MODE_BACKWARD, h->fused_waves, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_px, h->max_pitch, petstrip::MODE_BACKWARD)));
+        h->stream, petstrip::MODE_BACKWARD, h->col_stride, 0, a, h->n_items,
+        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_BACKWARD)));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_U64,
                                       a.inv_fixed_scale, bias, 0, nullptr, h->d_out,

@@ -49,10 +49,16 @@ hipError_t launch_bin_scatter(hipStream_t st, const EventGeom* geom, int64_t n,
 hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
                                WorkItem* items, int n_items);
 
-// LDS bytes per workgroup for windows of at most max_window_px pixels
-// (pitch included) and pitch max_pitch.
-size_t fused_lds_bytes(int max_window_px, int max_pitch, int mode);
-hipError_t launch_strip_fused(hipStream_t st, int mode, int waves, int compensated,
+constexpr int kWaves = 16;           // waves per workgroup of the fused kernel
+constexpr int kMaxColStride = 256;   // largest LDS column stride (window height)
+
+// LDS bytes per workgroup for windows of at most max_ww columns with column
+// stride col_stride.
+size_t fused_lds_bytes(int max_ww, int col_stride, int mode);
+// Column stride for windows of at most max_wh rows: the next multiple of
+// 64, or 0 when above kMaxColStride.
+int fused_col_stride(int max_wh);
+hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int compensated,
                               const FusedArgs& args, int n_items, size_t lds_bytes);
 
 // out = acc * unit + bias (acc read as kind; 64-bit values as signed), the

@@ -32,8 +32,9 @@
 // exponent of K (below), so the pixel step of pass 2 is four VALU
 // operations plus the LDS read and the LDS atomic.  The inner walks are
 // unrolled by four so that each lane keeps several LDS reads and
-// exponentials in flight.  The window pitch is odd so that lanes on
-// neighbouring rows fall in different LDS banks.  At the end the window is
+// exponentials in flight.  The LDS windows are column-major with a column
+// stride that is a multiple of 64, so the lanes (rows) of a wave always hit
+// 64 different LDS banks.  At the end the window is
 // added to the global accumulator with one atomic per touched pixel.
 //
 // Kernel value (GaussianKernel::operator(), gaussian_kernel.h:51-79): the
@@ -147,7 +148,7 @@ struct RowSeg {
 template <bool SIGMA>
 __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
                                             const EventGeom& g, const EventRows& ev,
-                                            int iy, int row_w, float s_pix) {
+                                            int iy, int row_w, int cstride, float s_pix) {
   RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
   const float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
   const float Ady2 = (dy * g.A) * dy;
@@ -193,7 +194,7 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   b = min(b, g.x1 - 1);
   if (a > b) return s;
   s.len = b - a + 1;
-  s.w0 = row_w + a;
+  s.w0 = row_w + a * cstride;
   // pq at the first pixel, with the reference's rounding (gaussian_kernel.h:64)
   const float dz = fmaf((float)a, sg.pw, sg.tl_z_half_w) - g.z;
   s.u0 = fmaf(-dy, g.t, dz) * s_pix;
@@ -221,6 +222,7 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
 }
 
 constexpr int kUnroll = 4;
+constexpr int kSpareCols = kUnroll;  // rho window columns past the window's width
 constexpr int kCachedGroups = 4;  // row groups kept in registers between passes
 
 // k + sgn * log2(x) for x > 0 and integer k, formed as (k + sgn e) + sgn
@@ -232,26 +234,29 @@ __device__ __forceinline__ float log2_shift(float k, float x, float sgn) {
   return fmaf(sgn, (float)e, k) + sgn * __builtin_amdgcn_logf(m);
 }
 
-// Pass-1 sums over the pixels rp[0..3] (u = uo[i] + kf du): t0 = sum K rho,
+// Pass-1 sums over four pixels of a row, rp[0], rp[HP], rp[2 HP], rp[3 HP]
+// (u = uo[i] + kf du): t0 = sum K rho,
 // t1 = sum K rho u^2 (SIGMA); TAIL keeps only the first rem pixels.
-template <bool SIGMA, bool TAIL>
+template <int HP, bool SIGMA, bool TAIL>
 __device__ __forceinline__ void forward_values(const float* rp, float kf, float du,
                                                const float (&uo)[kUnroll], float lsc,
                                                int rem, float& t0, float& t1) {
   float rv[kUnroll];
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u];
+  for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u * HP];
   t0 = 0.0f;
   t1 = 0.0f;
 #pragma unroll
   for (int u = 0; u < kUnroll; ++u) {
     const float uu = fmaf(kf, du, uo[u]);
-    float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, lsc));
-    if constexpr (TAIL) kv = (u < rem) ? kv : 0.0f;
-    if constexpr (SIGMA) {
-      const float kr = kv * rv[u];
+    const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, lsc));
+    if constexpr (TAIL || SIGMA) {
+      // tail pixels past the row are dropped by selecting the product, so
+      // whatever the window holds there cannot reach the sums
+      float kr = kv * rv[u];
+      if constexpr (TAIL) kr = (u < rem) ? kr : 0.0f;
       t0 += kr;
-      t1 = fmaf(kr, uu * uu, t1);
+      if constexpr (SIGMA) t1 = fmaf(kr, uu * uu, t1);
     } else {
       t0 = fmaf(kv, rv[u], t0);
     }
@@ -261,12 +266,15 @@ __device__ __forceinline__ void forward_values(const float* rp, float kf, float 
 // Pass 1 over one row group: the lane's full groups of four pixels, each
 // group's sum masked once, then its last len % 4 pixels at its own offset.
 // s0 += sum K rho (two_sum error into comp when COMP), s1 += sum K rho u^2.
-template <bool COMP, bool SIGMA>
+// A lane past its full groups re-reads its last one (masked), so no read
+// leaves the window.
+template <int HP, bool COMP, bool SIGMA>
 __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, const float* rho_w,
                                              float& s0, float& comp, float& s1) {
   const float* rp = rho_w + s.w0;
   const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
   const int nfull = s.len >> 2;
+  const int lastg = max(nfull - 1, 0);
   const int maxfull = uniform(wave_max_i(nfull));
   auto add = [&](float t0, float t1) {
     if constexpr (COMP) {
@@ -282,7 +290,8 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, const fl
   float kf = 0.0f;
   for (int gk = 0; gk < maxfull; ++gk, kf += (float)kUnroll) {
     float t0, t1;
-    forward_values<SIGMA, false>(rp + gk * kUnroll, kf, du, uo, s.lsc, 0, t0, t1);
+    forward_values<HP, SIGMA, false>(rp + min(gk, lastg) * (kUnroll * HP), kf, du, uo, s.lsc,
+                                     0, t0, t1);
     const bool on = gk < nfull;
     add(on ? t0 : 0.0f, on ? t1 : 0.0f);
   }
@@ -290,19 +299,20 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, const fl
   if (__any(rem > 0)) {
     const int t = nfull * kUnroll;
     float t0, t1;
-    forward_values<SIGMA, true>(rp + t, (float)t, du, uo, s.lsc, rem, t0, t1);
+    forward_values<HP, SIGMA, true>(rp + t * HP, (float)t, du, uo, s.lsc, rem, t0, t1);
     add(t0, t1);
   }
 }
 
-// The four pass-2 values of the pixels rp[0..3] (u = uo[i] + kf du): K * the
+// The four pass-2 values of the pixels rp[0], rp[HP], rp[2 HP], rp[3 HP]
+// (u = uo[i] + kf du): K * the
 // event multiplier (* rho), rounded to integers and split into 32-bit
 // halves.  FOLD: the exponent already carries log2 of the event multiplier
 // (lscw = lsc + lw).  Otherwise (MLEM events whose folded exponent could
 // overflow) x = (K rho / lambda) 2^bits is formed without the large
 // multiplier.  SPLIT: values reach 2^62; the exponent carries a further
 // 2^-32 and the halves are converted apart.
-template <bool RHO, bool FOLD, bool SPLIT>
+template <int HP, bool RHO, bool FOLD, bool SPLIT>
 __device__ __forceinline__ void backproject_values(const float* rp, float kf, float du,
                                                    const float (&uo)[kUnroll], float lscw,
                                                    float lam, float unit_scale,
@@ -312,7 +322,7 @@ __device__ __forceinline__ void backproject_values(const float* rp, float kf, fl
   if constexpr (RHO) {
     // issue the four LDS reads together
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u];
+    for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u * HP];
     asm volatile("" : "+v"(rv[0]), "+v"(rv[1]), "+v"(rv[2]), "+v"(rv[3]));
   }
 #pragma unroll
@@ -363,7 +373,7 @@ __device__ __forceinline__ void set_pair(unsigned long long& v, unsigned lo, uns
 
 // Pass 2 over one row group: the lane's full groups of four pixels under
 // one mask per group, then its last len % 4 pixels at its own offset.
-template <bool RHO, bool FOLD, bool SPLIT, bool SUB>
+template <int HP, bool RHO, bool FOLD, bool SPLIT, bool SUB>
 __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, float lscw,
                                                  float lam, float unit_scale,
                                                  const float* rho_w,
@@ -372,6 +382,7 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
   unsigned long long* ap = acc_w + s.w0;
   const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
   const int nfull = s.len >> 2;
+  const int lastg = max(nfull - 1, 0);
   const int maxfull = uniform(wave_max_i(nfull));
   unsigned lo[kUnroll], hi[kUnroll];
   unsigned long long pv[kUnroll];
@@ -382,49 +393,55 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
   }
   float kf = 0.0f;
   for (int gk = 0; gk < maxfull; ++gk, kf += (float)kUnroll) {
-    backproject_values<RHO, FOLD, SPLIT>(rp + gk * kUnroll, kf, du, uo, lscw, lam,
-                                         unit_scale, lo, hi);
+    backproject_values<HP, RHO, FOLD, SPLIT>(rp + min(gk, lastg) * (kUnroll * HP), kf, du,
+                                             uo, lscw, lam, unit_scale, lo, hi);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) set_pair(pv[u], lo[u], hi[u], !SPLIT);
     if (gk < nfull) {
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) lds_accumulate<SUB>(ap + gk * kUnroll + u, pv[u]);
+      for (int u = 0; u < kUnroll; ++u)
+        lds_accumulate<SUB>(ap + (gk * kUnroll + u) * HP, pv[u]);
     }
   }
   const int rem = s.len & (kUnroll - 1);
   if (__any(rem > 0)) {
     const int t0 = nfull * kUnroll;
-    backproject_values<RHO, FOLD, SPLIT>(rp + t0, (float)t0, du, uo, lscw, lam,
-                                         unit_scale, lo, hi);
+    backproject_values<HP, RHO, FOLD, SPLIT>(rp + t0 * HP, (float)t0, du, uo, lscw, lam,
+                                             unit_scale, lo, hi);
 #pragma unroll
     for (int u = 0; u < kUnroll - 1; ++u) {
       set_pair(pv[u], lo[u], hi[u], !SPLIT);
-      if (u < rem) lds_accumulate<SUB>(ap + t0 + u, pv[u]);
+      if (u < rem) lds_accumulate<SUB>(ap + (t0 + u) * HP, pv[u]);
     }
   }
 }
 
 }  // namespace
 
-// One workgroup per work item (a chunk of events sharing a spatial bin).
-template <int MODE, int WAVES, bool COMP>
-__global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
+// One workgroup of kWaves waves per work item (a chunk of events sharing a
+// spatial bin).  The LDS windows are column-major with column stride HP, a
+// multiple of 64 at least the window height: the 64 lanes of a wave walk 64
+// different rows, so pixel (row, col) sits in bank (row + const) mod 64
+// whatever column each lane has reached, and every LDS access of the walks
+// is free of bank conflicts.
+template <int MODE, bool COMP, int HP>
+__global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   constexpr bool kPass1 = MODE != MODE_BACKWARD;
   constexpr bool kPass2 = MODE != MODE_FORWARD;
   constexpr bool kRho = MODE != MODE_BACKWARD;  // rho window in LDS
   constexpr bool kSigma = MODE == MODE_FORWARD;
-  constexpr int kThreads = 64 * WAVES;
+  constexpr int kThreads = 64 * kWaves;
   using U = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem[];
   const StripConst& sg = a.sg;
   const StripConst& sk = a.sk;
   const WorkItem it = a.items[blockIdx.x];
   const int ww = it.ww, wh = it.wh;
-  const int pitch = ww | 1;  // odd pitch: rows of a tilted ellipse spread over banks
-  const int npx = pitch * wh;
-  // the rho window has spare space: unrolled lanes past their row's end read
-  // up to maxlen + 3 elements beyond it (values discarded)
-  const int nrho = npx + 2 * pitch + 8;
+  const int npx = ww * HP;
+  // the rho window has four spare columns: the tail step reads four pixels
+  // from the end of a lane's full groups, up to three past its row's end
+  // even when the row has no tail (values discarded)
+  const int nrho = (ww + kSpareCols) * HP;
   float* rho_w = reinterpret_cast<float*>(smem);
   U* acc_w = reinterpret_cast<U*>(
       smem + (kRho ? (((size_t)nrho * 4 + 15) & ~(size_t)15) : 0));
@@ -433,7 +450,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
 
   if constexpr (kRho) {
     for (int i = threadIdx.x; i < nrho; i += kThreads) {
-      int r = i / pitch, c = i - r * pitch;
+      int c = i / HP, r = i - c * HP;
       float v = 0.0f;
       if (r < wh && c < ww) v = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
       rho_w[i] = v;
@@ -452,11 +469,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
   const float unit_scale = __builtin_amdgcn_exp2f(a.log2_scale);
   double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0, msum = 0.0;
 
-  for (int e = it.begin + wave; e < it.end; e += WAVES) {
+  for (int e = it.begin + wave; e < it.end; e += kWaves) {
     const EventGeom g = a.sgeom[e];
     const int nrows = g.y1 - g.y0;
     const int ngroups = (nrows + 63) >> 6;
-    const int row0 = (g.y0 - it.wy0) * pitch - it.wx0;  // window index of (g.y0, 0)
+    const int row0 = (g.y0 - it.wy0) - it.wx0 * HP;  // window index of (g.y0, 0)
     const int orig = a.sidx[e];
     EventRows ev;
     ev.c4 = 4.0f * g.C;
@@ -482,10 +499,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
           const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
           if (gg * 64 + lane < nrows)
-            s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r * pitch, s_pix);
+            s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix);
           if (gi < kCachedGroups) cache[gi] = s;
           float s0 = 0.0f, s1 = 0.0f;
-          forward_rows<COMP, kSigma>(s, du, rho_w, s0, comp, s1);
+          forward_rows<HP, COMP, kSigma>(s, du, rho_w, s0, comp, s1);
           if constexpr (COMP) {
             float sum, err;
             two_sum(acc, s0, sum, err);
@@ -549,34 +566,34 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
             const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
             s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
             if (gg * 64 + lane < nrows)
-              s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r * pitch, s_pix);
+              s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix);
           }
           const float lscw = s.lsc + lw;
           if constexpr (MODE == MODE_MLEM) {
             // K rho 2^bits / lambda <= 2^bits, but K 2^bits / lambda alone
             // overflows a float when lambda is tiny
             if (__any(lscw > 120.0f))
-              backproject_rows<true, false, false, false>(s, du, s.lsc, lam,
+              backproject_rows<HP, true, false, false, false>(s, du, s.lsc, lam,
                                                           unit_scale, rho_w, acc_w);
             else
-              backproject_rows<true, true, false, false>(s, du, lscw, lam,
+              backproject_rows<HP, true, true, false, false>(s, du, lscw, lam,
                                                          unit_scale, rho_w, acc_w);
           } else if (a.split) {
             // |g| K 2^k <= 2^62 / n_events by the choice of k
             const float l32 = lscw - 32.0f;
             if (neg)
-              backproject_rows<false, true, true, true>(s, du, l32, 0.0f,
+              backproject_rows<HP, false, true, true, true>(s, du, l32, 0.0f,
                                                         unit_scale, rho_w, acc_w);
             else
-              backproject_rows<false, true, true, false>(s, du, l32, 0.0f,
+              backproject_rows<HP, false, true, true, false>(s, du, l32, 0.0f,
                                                          unit_scale, rho_w, acc_w);
           } else {
             // |g| K 2^k <= 2^31 by the choice of k
             if (neg)
-              backproject_rows<false, true, false, true>(s, du, lscw, 0.0f,
+              backproject_rows<HP, false, true, false, true>(s, du, lscw, 0.0f,
                                                          unit_scale, rho_w, acc_w);
             else
-              backproject_rows<false, true, false, false>(s, du, lscw, 0.0f,
+              backproject_rows<HP, false, true, false, false>(s, du, lscw, 0.0f,
                                                           unit_scale, rho_w, acc_w);
           }
         }
@@ -598,9 +615,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
 
   if constexpr (kPass2) {
     __syncthreads();
-    for (int i = threadIdx.x; i < ww * wh; i += kThreads) {
-      int r = i / ww, c = i - r * ww;
-      U v = acc_w[r * pitch + c];
+    for (int i = threadIdx.x; i < npx; i += kThreads) {
+      int c = i / HP, r = i - c * HP;
+      if (r >= wh) continue;
+      U v = acc_w[i];
       if (v == 0ull) continue;
       size_t p = (size_t)(it.wy0 + r) * sg.W + (it.wx0 + c);
       if (a.acc_kind == GLOBAL_U64)
@@ -617,50 +635,60 @@ __global__ __launch_bounds__(64 * WAVES) void k_strip_fused(FusedArgs a) {
 
 // ---------------------------------------------------------------------------
 
-size_t fused_lds_bytes(int max_window_px, int max_pitch, int mode) {
-  size_t npx = (size_t)max_window_px;
+size_t fused_lds_bytes(int max_ww, int col_stride, int mode) {
+  const size_t hp = (size_t)col_stride;
   size_t rho = mode == MODE_BACKWARD
                    ? 0
-                   : (((npx + 2 * (size_t)max_pitch + 8) * 4 + 15) & ~(size_t)15);
-  size_t acc = mode == MODE_FORWARD ? 0 : npx * 8;
+                   : ((((size_t)max_ww + kSpareCols) * hp * 4 + 15) & ~(size_t)15);
+  size_t acc = mode == MODE_FORWARD ? 0 : (size_t)max_ww * hp * 8;
   return rho + acc + 16;
+}
+
+int fused_col_stride(int max_wh) {
+  if (max_wh < 1) max_wh = 1;
+  const int hp = (max_wh + 63) / 64 * 64;
+  return hp <= kMaxColStride ? hp : 0;
 }
 
 namespace {
 
-template <int MODE, int WAVES, bool COMP>
+template <int MODE, bool COMP, int HP>
 hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds) {
-  auto kern = k_strip_fused<MODE, WAVES, COMP>;
+  auto kern = k_strip_fused<MODE, COMP, HP>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(n_items), dim3(64 * WAVES), lds, st, a);
+  hipLaunchKernelGGL(kern, dim3(n_items), dim3(64 * kWaves), lds, st, a);
   return hipGetLastError();
 }
 
 template <int MODE, bool COMP>
-hipError_t launch_w(int waves, hipStream_t st, const FusedArgs& a, int n_items, size_t lds) {
-  switch (waves) {
-    case 16: return launch_t<MODE, 16, COMP>(st, a, n_items, lds);
-    case 8: return launch_t<MODE, 8, COMP>(st, a, n_items, lds);
-    default: return launch_t<MODE, 4, COMP>(st, a, n_items, lds);
+hipError_t launch_h(int col_stride, hipStream_t st, const FusedArgs& a, int n_items,
+                    size_t lds) {
+  switch (col_stride) {
+    case 64: return launch_t<MODE, COMP, 64>(st, a, n_items, lds);
+    case 128: return launch_t<MODE, COMP, 128>(st, a, n_items, lds);
+    case 192: return launch_t<MODE, COMP, 192>(st, a, n_items, lds);
+    case 256: return launch_t<MODE, COMP, 256>(st, a, n_items, lds);
+    default: return hipErrorInvalidValue;
   }
 }
 
 }  // namespace
 
-hipError_t launch_strip_fused(hipStream_t st, int mode, int waves, int compensated,
+hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int compensated,
                               const FusedArgs& a, int n_items, size_t lds_bytes) {
   if (n_items <= 0) return hipSuccess;
   switch (mode) {
     case MODE_MLEM:
-      return compensated ? launch_w<MODE_MLEM, true>(waves, st, a, n_items, lds_bytes)
-                         : launch_w<MODE_MLEM, false>(waves, st, a, n_items, lds_bytes);
+      return compensated ? launch_h<MODE_MLEM, true>(col_stride, st, a, n_items, lds_bytes)
+                         : launch_h<MODE_MLEM, false>(col_stride, st, a, n_items, lds_bytes);
     case MODE_FORWARD:
-      return compensated ? launch_w<MODE_FORWARD, true>(waves, st, a, n_items, lds_bytes)
-                         : launch_w<MODE_FORWARD, false>(waves, st, a, n_items, lds_bytes);
+      return compensated
+                 ? launch_h<MODE_FORWARD, true>(col_stride, st, a, n_items, lds_bytes)
+                 : launch_h<MODE_FORWARD, false>(col_stride, st, a, n_items, lds_bytes);
     default:
-      return launch_w<MODE_BACKWARD, false>(waves, st, a, n_items, lds_bytes);
+      return launch_h<MODE_BACKWARD, false>(col_stride, st, a, n_items, lds_bytes);
   }
 }
 
