@@ -39,7 +39,7 @@ def parse():
     p.add_argument("--workload", default="strip-512px-1mm-2e7ev")
     p.add_argument("--accumulation", default="fp32",
                    choices=["fp32", "compensated", "fixed64"])
-    p.add_argument("--cpu-sample", type=float, default=2e6,
+    p.add_argument("--cpu-sample", type=float, default=6e6,
                    help="events in the CPU baseline sample")
     p.add_argument("--cpu-threads", type=int, default=16)
     p.add_argument("--no-cpu", action="store_true")

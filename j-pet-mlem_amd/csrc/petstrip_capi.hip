@@ -146,6 +146,10 @@ struct petstrip_handle {
   int col_stride = 0;  // LDS column stride of the fused windows (0: none fits)
   int max_ww = 1;
   int* d_flag = nullptr;  // device overflow flag of the fixed-point image
+  // G(k) = 2^(-(k du)^2) of the fused walks, for the current sk and pw
+  float* d_gtab = nullptr;
+  size_t gtab_n = 0;
+  float s_pix = 0.0f, du = -1.0f;
   bool fused_ok = false;
   void* d_acc = nullptr;  // W*H accumulator, 8 bytes per pixel
 };
@@ -207,11 +211,40 @@ bool rows_are_intervals(const StripConst& s) {
   return C * pw * pw > 50.0 * err;
 }
 
+// The per-column factor table of the fused walks (see strip_fused.hip):
+// G(k) = 2^(-(k du)^2), du = pw sqrt(isz log2 e) with the kernel's sigma_z,
+// rounded once from double; rebuilt when sigma_z or the width change.
+int update_kernel_table(petstrip_handle* h) {
+  const float s_pix = sqrtf(h->sk.isz * 1.44269504088896341f);
+  const float du = h->sg.pw * s_pix;
+  const size_t n = (size_t)h->sg.W + 8;
+  if (h->d_gtab && du == h->du && n == h->gtab_n) return PETSTRIP_OK;
+  std::vector<float> t(n);
+  for (size_t k = 0; k < n; ++k) {
+    const double x = (double)k * (double)du;
+    t[k] = (float)std::exp2(-x * x);
+  }
+  if (n != h->gtab_n) {
+    dfree(h->d_gtab);
+    h->gtab_n = 0;
+    int rc;
+    if ((rc = dalloc(&h->d_gtab, n))) return rc;
+    h->gtab_n = n;
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(h->d_gtab, t.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  h->s_pix = s_pix;
+  h->du = du;
+  return PETSTRIP_OK;
+}
+
 // Sorts events into kBinW x kBinH bins of their box centre, cuts each bin into
 // chunks of kChunk events and sizes each chunk's LDS window.
 int prepare_fused(petstrip_handle* h) {
   h->fused_ok = false;
   h->n_items = 0;
+  int rc0;
+  if ((rc0 = update_kernel_table(h))) return rc0;
   const int64_t n = h->n_events;
   const int W = h->sg.W, H = h->sg.H;
   const int nbx = (W + kBinW - 1) / kBinW, nby = (H + kBinH - 1) / kBinH;
@@ -333,6 +366,9 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.split = 0;
   a.inv_fixed_scale = std::ldexp(1.0, -kMlemFixedBits);
   a.scalars = h->d_scalars;
+  a.s_pix = h->s_pix;
+  a.du = h->du;
+  a.gtab = h->d_gtab;
   return a;
 }
 
@@ -473,6 +509,7 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_sidx);
   dfree(h->d_items);
   dfree(h->d_flag);
+  dfree(h->d_gtab);
   if (h->d_acc) (void)hipFree(h->d_acc);
   h->d_acc = nullptr;
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -841,7 +878,11 @@ int petstrip_set_sigma(petstrip_handle* h, double sigma_z, double sigma_dl,
     return fail(PETSTRIP_EINVAL, "sigmas must be positive");
   h->sk = make_const(h->scanner, sigma_z, sigma_dl);
   h->lambda_valid = false;
-  if (!freeze_support) {
+  if (freeze_support) {
+    int rc;
+    if ((rc = ensure_device(h))) return rc;
+    if (h->fused_ok && (rc = update_kernel_table(h))) return rc;
+  } else {
     h->sg = h->sk;
     h->scanner.sigma_z = sigma_z;
     h->scanner.sigma_dl = sigma_dl;

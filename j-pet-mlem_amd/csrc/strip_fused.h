@@ -35,6 +35,9 @@ struct FusedArgs {
   float log2_scale;          // k: pass-2 values are added in units 2^-k
   int split;                 // BACKWARD: values up to 2^62 (else below 2^31)
   double inv_fixed_scale;    // 2^-k, the unit when flushing to floats
+  float s_pix;               // sqrt(sk.isz log2 e): u = pq s_pix
+  float du;                  // sg.pw s_pix, the u step per column
+  const float* gtab;         // G(k) = 2^(-(k du)^2) for k < the image width + 8
   double* scalars;           // FORWARD: {sum log lambda, dl/dsz, dl/dsdl,
                              //           max_e |g_e| K_e, sum_e |g_e| K_e}
                              // with K_e the event's largest kernel value

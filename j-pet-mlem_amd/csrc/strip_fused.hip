@@ -234,43 +234,50 @@ __device__ __forceinline__ float log2_shift(float k, float x, float sgn) {
   return fmaf(sgn, (float)e, k) + sgn * __builtin_amdgcn_logf(m);
 }
 
-// Pass-1 sums over four pixels of a row, rp[0], rp[HP], rp[2 HP], rp[3 HP]
-// (u = uo[i] + kf du): t0 = sum K rho,
-// t1 = sum K rho u^2 (SIGMA); TAIL keeps only the first rem pixels.
-template <int HP, bool SIGMA, bool TAIL>
-__device__ __forceinline__ void forward_values(const float* rp, float kf, float du,
-                                               const float (&uo)[kUnroll], float lsc,
-                                               int rem, float& t0, float& t1) {
-  float rv[kUnroll];
+// Kernel values along a row.  With u = u0 + k du at the row's k-th pixel,
+//   K(k) = 2^(lsc - u^2) = 2^(lsc - u0^2 + e1 k) G(k),
+//   e1 = -2 u0 du,  G(k) = 2^(-(k du)^2),
+// the first factor is a geometric sequence in k for each lane and G(k) is
+// the same for every lane, read from a table through the scalar cache: the
+// walks take two multiplications per pixel instead of an exponential.  Each
+// of the four unrolled pixels keeps its own chain, advanced by b4 = 2^(4 e1)
+// per group; over a row's dozen groups the chain's relative drift stays
+// below 1e-6.  The event multiplier of pass 2 (and the 2^-32 of the split
+// conversion) enters as e_add.
+struct Chain {
+  float p[kUnroll];
+  float b4;
+};
+
+__device__ __forceinline__ Chain chain_start(const RowSeg& s, float du, float e_add) {
+  Chain c;
+  const float e1 = -2.0f * s.u0 * du;
+  const float e0 = fmaf(-s.u0, s.u0, s.lsc) + e_add;
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u * HP];
-  t0 = 0.0f;
-  t1 = 0.0f;
-#pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
-    const float uu = fmaf(kf, du, uo[u]);
-    const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, lsc));
-    if constexpr (TAIL || SIGMA) {
-      // tail pixels past the row are dropped by selecting the product, so
-      // whatever the window holds there cannot reach the sums
-      float kr = kv * rv[u];
-      if constexpr (TAIL) kr = (u < rem) ? kr : 0.0f;
-      t0 += kr;
-      if constexpr (SIGMA) t1 = fmaf(kr, uu * uu, t1);
-    } else {
-      t0 = fmaf(kv, rv[u], t0);
-    }
-  }
+  for (int u = 0; u < kUnroll; ++u) c.p[u] = __builtin_amdgcn_exp2f(fmaf((float)u, e1, e0));
+  c.b4 = __builtin_amdgcn_exp2f(4.0f * e1);
+  return c;
 }
 
-// Pass 1 over one row group: the lane's full groups of four pixels, each
-// group's sum masked once, then its last len % 4 pixels at its own offset.
+using ConstF = const __attribute__((address_space(4))) float;
+
+// The lane's four LDS reads of a group (two ds_read2st64).
+template <int HP>
+__device__ __forceinline__ void read4(const float* rp, float (&rv)[kUnroll]) {
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u * HP];
+}
+
+// Pass 1 over one row group: the lane's full groups of four pixels (kernel
+// values from the chain), each group's sum masked once, then its last
+// len % 4 pixels at its own offset (kernel values evaluated directly).
 // s0 += sum K rho (two_sum error into comp when COMP), s1 += sum K rho u^2.
 // A lane past its full groups re-reads its last one (masked), so no read
 // leaves the window.
 template <int HP, bool COMP, bool SIGMA>
-__device__ __forceinline__ void forward_rows(const RowSeg& s, float du, const float* rho_w,
-                                             float& s0, float& comp, float& s1) {
+__device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* gtab,
+                                             const float* rho_w, float& s0, float& comp,
+                                             float& s1) {
   const float* rp = rho_w + s.w0;
   const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
   const int nfull = s.len >> 2;
@@ -287,69 +294,71 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, const fl
     }
     if constexpr (SIGMA) s1 += t1;
   };
+  Chain c = chain_start(s, du, 0.0f);
   float kf = 0.0f;
   for (int gk = 0; gk < maxfull; ++gk, kf += (float)kUnroll) {
-    float t0, t1;
-    forward_values<HP, SIGMA, false>(rp + min(gk, lastg) * (kUnroll * HP), kf, du, uo, s.lsc,
-                                     0, t0, t1);
+    float rv[kUnroll];
+    read4<HP>(rp + min(gk, lastg) * (kUnroll * HP), rv);
+    float t0 = 0.0f, t1 = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const float kv = c.p[u] * gtab[gk * kUnroll + u];
+      c.p[u] *= c.b4;
+      if constexpr (SIGMA) {
+        const float kr = kv * rv[u];
+        const float uu = fmaf(kf, du, uo[u]);
+        t0 += kr;
+        t1 = fmaf(kr, uu * uu, t1);
+      } else {
+        t0 = fmaf(kv, rv[u], t0);
+      }
+    }
     const bool on = gk < nfull;
     add(on ? t0 : 0.0f, on ? t1 : 0.0f);
   }
   const int rem = s.len & (kUnroll - 1);
   if (__any(rem > 0)) {
     const int t = nfull * kUnroll;
-    float t0, t1;
-    forward_values<HP, SIGMA, true>(rp + t * HP, (float)t, du, uo, s.lsc, rem, t0, t1);
+    float rv[kUnroll];
+    read4<HP>(rp + t * HP, rv);
+    float t0 = 0.0f, t1 = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kUnroll - 1; ++u) {
+      const float uu = fmaf((float)t, du, uo[u]);
+      // pixels past the row are dropped by selecting the product, so
+      // whatever the window holds there cannot reach the sums
+      float kr = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * rv[u];
+      kr = (u < rem) ? kr : 0.0f;
+      t0 += kr;
+      if constexpr (SIGMA) t1 = fmaf(kr, uu * uu, t1);
+    }
     add(t0, t1);
   }
 }
 
-// The four pass-2 values of the pixels rp[0], rp[HP], rp[2 HP], rp[3 HP]
-// (u = uo[i] + kf du): K * the
-// event multiplier (* rho), rounded to integers and split into 32-bit
-// halves.  FOLD: the exponent already carries log2 of the event multiplier
-// (lscw = lsc + lw).  Otherwise (MLEM events whose folded exponent could
-// overflow) x = (K rho / lambda) 2^bits is formed without the large
-// multiplier.  SPLIT: values reach 2^62; the exponent carries a further
-// 2^-32 and the halves are converted apart.
-template <int HP, bool RHO, bool FOLD, bool SPLIT>
-__device__ __forceinline__ void backproject_values(const float* rp, float kf, float du,
-                                                   const float (&uo)[kUnroll], float lscw,
-                                                   float lam, float unit_scale,
-                                                   unsigned (&lo)[kUnroll],
-                                                   unsigned (&hi)[kUnroll]) {
-  float rv[kUnroll];
-  if constexpr (RHO) {
-    // issue the four LDS reads together
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u * HP];
-    asm volatile("" : "+v"(rv[0]), "+v"(rv[1]), "+v"(rv[2]), "+v"(rv[3]));
+// x -> the integer round(x) as (lo, hi); SPLIT: kv = x 2^-32 < 2^30 and the
+// halves are converted apart, else x < 2^32.
+template <bool SPLIT>
+__device__ __forceinline__ void to_fixed(float x_or_kv, unsigned& lo, unsigned& hi) {
+  if constexpr (SPLIT) {
+    hi = (unsigned)x_or_kv;
+    lo = (unsigned)fmaf(__builtin_amdgcn_fractf(x_or_kv), 4294967296.0f, 0.5f);
+  } else {
+    lo = (unsigned)x_or_kv;  // x in [0.5, 2^32)
+    hi = 0u;
   }
-#pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
-    const float uu = fmaf(kf, du, uo[u]);
-    const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, lscw));
-    if constexpr (SPLIT) {
-      // kv = x 2^-32 < 2^30
-      hi[u] = (unsigned)kv;
-      lo[u] = (unsigned)fmaf(__builtin_amdgcn_fractf(kv), 4294967296.0f, 0.5f);
-    } else {
-      float x;
-      if constexpr (FOLD) {
-        if constexpr (RHO)
-          x = fmaf(kv, rv[u], 0.5f);
-        else
-          x = kv + 0.5f;
-      } else {
-        x = fmaf((kv * rv[u]) / lam, unit_scale, 0.5f);
-      }
-      lo[u] = (unsigned)x;  // round(x), x in [0.5, 2^32)
-      hi[u] = 0u;
-    }
-    // keep the values (and their LDS reads) ahead of the masked atomics
-    // instead of letting each sink into its own branch
-    asm volatile("" : "+v"(lo[u]));
-  }
+  // keep the values (and their LDS reads) ahead of the masked atomics
+  // instead of letting each sink into its own branch
+  asm volatile("" : "+v"(lo));
+}
+
+// One pass-2 value from the kernel value kv (which carries the event
+// multiplier): K w rho or K w (+ 0.5 for rounding); SPLIT converts kv itself.
+template <bool RHO, bool SPLIT>
+__device__ __forceinline__ float fixed_arg(float kv, float rho) {
+  if constexpr (SPLIT) return kv;
+  else if constexpr (RHO) return fmaf(kv, rho, 0.5f);
+  else return kv + 0.5f;
 }
 
 template <bool SUB>
@@ -373,9 +382,14 @@ __device__ __forceinline__ void set_pair(unsigned long long& v, unsigned lo, uns
 
 // Pass 2 over one row group: the lane's full groups of four pixels under
 // one mask per group, then its last len % 4 pixels at its own offset.
+// lw = log2 of the event multiplier in fixed-point units.  FOLD: the
+// multiplier rides in the kernel values (chain and direct); otherwise (MLEM
+// events whose folded values could overflow a float) x = (K rho / lambda)
+// 2^bits is formed per pixel without the large multiplier.  SPLIT: values
+// reach 2^62 and carry a further 2^-32 until converted.
 template <int HP, bool RHO, bool FOLD, bool SPLIT, bool SUB>
-__device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, float lscw,
-                                                 float lam, float unit_scale,
+__device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, float lw,
+                                                 float lam, float unit_scale, ConstF* gtab,
                                                  const float* rho_w,
                                                  unsigned long long* acc_w) {
   const float* rp = rho_w + s.w0;
@@ -384,6 +398,18 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
   const int nfull = s.len >> 2;
   const int lastg = max(nfull - 1, 0);
   const int maxfull = uniform(wave_max_i(nfull));
+  const float e_add = SPLIT ? lw - 32.0f : lw;
+  const float lscw = s.lsc + e_add;
+  // direct value of pixel k of the row (tail, and MLEM without folding)
+  auto direct = [&](float kf, int u, float rho) {
+    const float uu = fmaf(kf, du, uo[u]);
+    if constexpr (FOLD) {
+      return fixed_arg<RHO, SPLIT>(__builtin_amdgcn_exp2f(fmaf(-uu, uu, lscw)), rho);
+    } else {
+      const float kr = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * rho;
+      return fmaf(kr / lam, unit_scale, 0.5f);
+    }
+  };
   unsigned lo[kUnroll], hi[kUnroll];
   unsigned long long pv[kUnroll];
 #pragma unroll
@@ -391,12 +417,23 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
     pv[u] = 0ull;
     asm volatile("" : "+v"(pv[u]));
   }
+  Chain c = chain_start(s, du, e_add);
   float kf = 0.0f;
   for (int gk = 0; gk < maxfull; ++gk, kf += (float)kUnroll) {
-    backproject_values<HP, RHO, FOLD, SPLIT>(rp + min(gk, lastg) * (kUnroll * HP), kf, du,
-                                             uo, lscw, lam, unit_scale, lo, hi);
+    float rv[kUnroll] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (RHO) read4<HP>(rp + min(gk, lastg) * (kUnroll * HP), rv);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) set_pair(pv[u], lo[u], hi[u], !SPLIT);
+    for (int u = 0; u < kUnroll; ++u) {
+      float x;
+      if constexpr (FOLD) {
+        x = fixed_arg<RHO, SPLIT>(c.p[u] * gtab[gk * kUnroll + u], rv[u]);
+        c.p[u] *= c.b4;
+      } else {
+        x = direct(kf, u, rv[u]);
+      }
+      to_fixed<SPLIT>(x, lo[u], hi[u]);
+      set_pair(pv[u], lo[u], hi[u], !SPLIT);
+    }
     if (gk < nfull) {
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
@@ -406,10 +443,11 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
   const int rem = s.len & (kUnroll - 1);
   if (__any(rem > 0)) {
     const int t0 = nfull * kUnroll;
-    backproject_values<HP, RHO, FOLD, SPLIT>(rp + t0 * HP, (float)t0, du, uo, lscw, lam,
-                                             unit_scale, lo, hi);
+    float rv[kUnroll] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (RHO) read4<HP>(rp + t0 * HP, rv);
 #pragma unroll
     for (int u = 0; u < kUnroll - 1; ++u) {
+      to_fixed<SPLIT>(direct((float)t0, u, rv[u]), lo[u], hi[u]);
       set_pair(pv[u], lo[u], hi[u], !SPLIT);
       if (u < rem) lds_accumulate<SUB>(ap + (t0 + u) * HP, pv[u]);
     }
@@ -461,8 +499,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   }
   __syncthreads();
 
-  const float s_pix = sqrtf(sk.isz * kLog2e);  // sqrt(isz log2 e)
-  const float du = sg.pw * s_pix;               // u step per column
+  const float s_pix = a.s_pix;  // sqrt(isz log2 e)
+  const float du = a.du;        // u step per column, pw s_pix
+  ConstF* gtab = (ConstF*)a.gtab;  // uniform reads through the scalar cache
   const float inv_pw = 1.0f / sg.pw;
   const float inv_ph = 1.0f / sg.ph;
   const float cz_pix = 2.0f / (kLog2e * sk.sigma_z);  // 2 isz pq^2 / sz = cz u^2
@@ -502,7 +541,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix);
           if (gi < kCachedGroups) cache[gi] = s;
           float s0 = 0.0f, s1 = 0.0f;
-          forward_rows<HP, COMP, kSigma>(s, du, rho_w, s0, comp, s1);
+          forward_rows<HP, COMP, kSigma>(s, du, gtab, rho_w, s0, comp, s1);
           if constexpr (COMP) {
             float sum, err;
             two_sum(acc, s0, sum, err);
@@ -568,33 +607,32 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             if (gg * 64 + lane < nrows)
               s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix);
           }
-          const float lscw = s.lsc + lw;
           if constexpr (MODE == MODE_MLEM) {
-            // K rho 2^bits / lambda <= 2^bits, but K 2^bits / lambda alone
-            // overflows a float when lambda is tiny
-            if (__any(lscw > 120.0f))
-              backproject_rows<HP, true, false, false, false>(s, du, s.lsc, lam,
-                                                          unit_scale, rho_w, acc_w);
+            // K rho 2^bits / lambda <= 2^bits, but the chain's K 2^bits /
+            // lambda alone (up to 2^(lsc + lw + 20) along a row) overflows a
+            // float when lambda is tiny
+            if (__any(s.lsc + lw > 100.0f))
+              backproject_rows<HP, true, false, false, false>(s, du, lw, lam, unit_scale,
+                                                              gtab, rho_w, acc_w);
             else
-              backproject_rows<HP, true, true, false, false>(s, du, lscw, lam,
-                                                         unit_scale, rho_w, acc_w);
+              backproject_rows<HP, true, true, false, false>(s, du, lw, lam, unit_scale,
+                                                             gtab, rho_w, acc_w);
           } else if (a.split) {
             // |g| K 2^k <= 2^62 / n_events by the choice of k
-            const float l32 = lscw - 32.0f;
             if (neg)
-              backproject_rows<HP, false, true, true, true>(s, du, l32, 0.0f,
-                                                        unit_scale, rho_w, acc_w);
+              backproject_rows<HP, false, true, true, true>(s, du, lw, 0.0f, unit_scale,
+                                                            gtab, rho_w, acc_w);
             else
-              backproject_rows<HP, false, true, true, false>(s, du, l32, 0.0f,
-                                                         unit_scale, rho_w, acc_w);
+              backproject_rows<HP, false, true, true, false>(s, du, lw, 0.0f, unit_scale,
+                                                             gtab, rho_w, acc_w);
           } else {
             // |g| K 2^k <= 2^31 by the choice of k
             if (neg)
-              backproject_rows<HP, false, true, false, true>(s, du, lscw, 0.0f,
-                                                         unit_scale, rho_w, acc_w);
+              backproject_rows<HP, false, true, false, true>(s, du, lw, 0.0f, unit_scale,
+                                                             gtab, rho_w, acc_w);
             else
-              backproject_rows<HP, false, true, false, false>(s, du, lscw, 0.0f,
-                                                          unit_scale, rho_w, acc_w);
+              backproject_rows<HP, false, true, false, false>(s, du, lw, 0.0f, unit_scale,
+                                                              gtab, rho_w, acc_w);
           }
         }
       }
