@@ -195,7 +195,10 @@ constexpr int kBinW = 16;
 constexpr int kBinH = 8;
 constexpr int kChunk = 1024;      // events per work item
 // MLEM backprojection values K rho / lambda <= 1 are summed in 2^-31 units
+// (one conversion per value), or in 2^-40 units in the deterministic
+// accumulation mode (two conversions)
 constexpr int kMlemFixedBits = 31;
+constexpr int kFixed64Bits = 40;
 
 // Along a box row the reference's float ellipse test has no holes when the
 // quadratic's column-to-column curvature C*pw^2 dominates the rounding of q
@@ -380,13 +383,19 @@ int one_iteration(petstrip_handle* h) {
     a.acc_kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
                  : accm == PETSTRIP_ACCUM_COMPENSATED ? petstrip::GLOBAL_F64
                                                        : petstrip::GLOBAL_F32;
+    if (accm == PETSTRIP_ACCUM_FIXED64) {
+      // the deterministic image sums every value in 2^-40 units
+      a.log2_scale = (float)kFixed64Bits;
+      a.split = 1;
+      a.inv_fixed_scale = std::ldexp(1.0, -kFixed64Bits);
+    }
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
     HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_MLEM, h->col_stride,
                                          accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
                                          h->n_items, h->lds_bytes));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
-    // float accumulators already hold image units; the u64 one holds 2^-31
+    // float accumulators already hold image units; the u64 one holds 2^-40
     double unit = a.acc_kind == petstrip::GLOBAL_U64 ? a.inv_fixed_scale : 1.0;
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, a.acc_kind, unit, 0.0f, 1,
                                       h->d_flag, h->d_out, (int64_t)h->n_pix));
@@ -635,7 +644,7 @@ int petstrip_iterate(petstrip_handle* h, int n_iterations) {
     if (flag) {
       HIP_TRY(hipMemset(h->d_flag, 0, sizeof(int)));
       return fail(PETSTRIP_ESTATE,
-                  "fixed-point image overflow: a pixel sum exceeded 2^32 in 2^-31 "
+                  "fixed-point image overflow: a pixel sum exceeded 2^23 in 2^-40 "
                   "units");
     }
   }

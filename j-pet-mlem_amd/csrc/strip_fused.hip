@@ -19,7 +19,8 @@
 //      accumulation window as 64-bit fixed point.
 // Modes:
 //   MLEM      pass 1 + pass 2 with w_e = 1/lambda_e; each value K rho /
-//             lambda_e <= 1 is added in 2^-31 units
+//             lambda_e <= 1 is added in 2^-31 units, or in 2^-40 units
+//             (split conversion) for the deterministic accumulation
 //   FORWARD   pass 1 only, plus the sigma derivative of lambda_e and the
 //             log-likelihood terms, reduced per workgroup into doubles
 //   BACKWARD  pass 2 only with upstream weights g_e, values |K g_e| in
@@ -353,10 +354,11 @@ __device__ __forceinline__ void to_fixed(float x_or_kv, unsigned& lo, unsigned& 
 }
 
 // One pass-2 value from the kernel value kv (which carries the event
-// multiplier): K w rho or K w (+ 0.5 for rounding); SPLIT converts kv itself.
+// multiplier): K w rho or K w, + 0.5 for rounding unless SPLIT (whose
+// conversion rounds the low half itself).
 template <bool RHO, bool SPLIT>
 __device__ __forceinline__ float fixed_arg(float kv, float rho) {
-  if constexpr (SPLIT) return kv;
+  if constexpr (SPLIT) return RHO ? kv * rho : kv;
   else if constexpr (RHO) return fmaf(kv, rho, 0.5f);
   else return kv + 0.5f;
 }
@@ -406,8 +408,9 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
     if constexpr (FOLD) {
       return fixed_arg<RHO, SPLIT>(__builtin_amdgcn_exp2f(fmaf(-uu, uu, lscw)), rho);
     } else {
+      // unit_scale = 2^bits (2^(bits - 32) when SPLIT)
       const float kr = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * rho;
-      return fmaf(kr / lam, unit_scale, 0.5f);
+      return SPLIT ? (kr / lam) * unit_scale : fmaf(kr / lam, unit_scale, 0.5f);
     }
   };
   unsigned lo[kUnroll], hi[kUnroll];
@@ -505,7 +508,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   const float inv_pw = 1.0f / sg.pw;
   const float inv_ph = 1.0f / sg.ph;
   const float cz_pix = 2.0f / (kLog2e * sk.sigma_z);  // 2 isz pq^2 / sz = cz u^2
-  const float unit_scale = __builtin_amdgcn_exp2f(a.log2_scale);
+  // 2^k, or 2^(k - 32) for the split conversion (MLEM's unfolded path)
+  const float unit_scale = __builtin_amdgcn_exp2f(a.log2_scale - (a.split ? 32.0f : 0.0f));
   double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0, msum = 0.0;
 
   for (int e = it.begin + wave; e < it.end; e += kWaves) {
@@ -611,12 +615,22 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             // K rho 2^bits / lambda <= 2^bits, but the chain's K 2^bits /
             // lambda alone (up to 2^(lsc + lw + 20) along a row) overflows a
             // float when lambda is tiny
-            if (__any(s.lsc + lw > 100.0f))
-              backproject_rows<HP, true, false, false, false>(s, du, lw, lam, unit_scale,
+            const bool fold = !__any(s.lsc + lw > 100.0f);
+            if (a.split) {
+              // 2^-40 units (the deterministic accumulation): values up to 2^40
+              if (fold)
+                backproject_rows<HP, true, true, true, false>(s, du, lw, lam, unit_scale,
                                                               gtab, rho_w, acc_w);
-            else
+              else
+                backproject_rows<HP, true, false, true, false>(s, du, lw, lam, unit_scale,
+                                                               gtab, rho_w, acc_w);
+            } else if (fold) {
               backproject_rows<HP, true, true, false, false>(s, du, lw, lam, unit_scale,
                                                              gtab, rho_w, acc_w);
+            } else {
+              backproject_rows<HP, true, false, false, false>(s, du, lw, lam, unit_scale,
+                                                              gtab, rho_w, acc_w);
+            }
           } else if (a.split) {
             // |g| K 2^k <= 2^62 / n_events by the choice of k
             if (neg)
