@@ -91,7 +91,10 @@ typedef struct petstrip_options {
   int32_t accumulation; /* PETSTRIP_ACCUM_*                                */
   int32_t update;       /* PETSTRIP_UPDATE_*                               */
   int32_t path;         /* PETSTRIP_PATH_*                                 */
-  int32_t reserved[5];  /* zero                                            */
+  int32_t row_scan;     /* verification switch: 1 = find every row's ellipse
+                           interval by scanning instead of from the roots of
+                           its quadratic; results are identical, slower     */
+  int32_t reserved[4];  /* zero                                            */
 } petstrip_options;
 
 typedef struct petstrip_handle petstrip_handle;

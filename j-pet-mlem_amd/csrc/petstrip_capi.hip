@@ -323,8 +323,14 @@ int prepare_events(petstrip_handle* h) {
   const int64_t n = h->n_events;
   HIP_TRY(petstrip::launch_event_setup(h->stream, h->sg, h->d_resp, n,
                                        h->d_geom));
-  HIP_TRY(petstrip::launch_footprint(h->stream, h->sg, h->d_geom, n,
-                                     h->d_nbox, h->d_nell));
+  // footprints from the row intervals the fused walks use when rows are
+  // hole-free, else by testing every box pixel
+  if (rows_are_intervals(h->sg))
+    HIP_TRY(petstrip::launch_row_counts(h->stream, h->sg, h->d_geom, n, h->opts.row_scan,
+                                        h->d_nbox, h->d_nell));
+  else
+    HIP_TRY(petstrip::launch_footprint(h->stream, h->sg, h->d_geom, n, h->d_nbox,
+                                       h->d_nell));
   std::vector<int32_t> nb((size_t)n), ne((size_t)n);
   if (n > 0) {
     HIP_TRY(hipMemcpyAsync(nb.data(), h->d_nbox, n * sizeof(int32_t),
@@ -369,6 +375,7 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.split = 0;
   a.inv_fixed_scale = std::ldexp(1.0, -kMlemFixedBits);
   a.scalars = h->d_scalars;
+  a.row_scan = h->opts.row_scan;
   a.s_pix = h->s_pix;
   a.du = h->du;
   a.gtab = h->d_gtab;
@@ -440,7 +447,7 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   if (rc) return rc;
   if (opts && (opts->accumulation < 0 || opts->accumulation > 2 ||
                opts->update < 0 || opts->update > 1 || opts->path < 0 ||
-      opts->path > 1))
+               opts->path > 1 || opts->row_scan < 0 || opts->row_scan > 1))
     return fail(PETSTRIP_EINVAL, "invalid options");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -545,13 +552,15 @@ int petstrip_stream(petstrip_handle* h, void** stream) {
 int petstrip_set_options(petstrip_handle* h, const petstrip_options* opts) {
   if (!h || !opts) return fail(PETSTRIP_EINVAL, "bad argument");
   if (opts->accumulation < 0 || opts->accumulation > 2 || opts->update < 0 ||
-      opts->update > 1)
+      opts->update > 1 || opts->path < 0 || opts->path > 1 || opts->row_scan < 0 ||
+      opts->row_scan > 1)
     return fail(PETSTRIP_EINVAL, "invalid options");
   h->opts = *opts;
   h->lambda_valid = false;
   int rc;
   if ((rc = ensure_device(h))) return rc;
-  return prepare_fused(h);
+  // the footprint counts follow the row-interval method
+  return h->n_events > 0 ? prepare_events(h) : prepare_fused(h);
 }
 
 int petstrip_set_events(petstrip_handle* h, const float* responses, int64_t n) {

@@ -35,6 +35,7 @@ struct FusedArgs {
   float log2_scale;          // k: pass-2 values are added in units 2^-k
   int split;                 // BACKWARD: values up to 2^62 (else below 2^31)
   double inv_fixed_scale;    // 2^-k, the unit when flushing to floats
+  int row_scan;              // 1: every row interval by the scanning path
   float s_pix;               // sqrt(sk.isz log2 e): u = pq s_pix
   float du;                  // sg.pw s_pix, the u step per column
   const float* gtab;         // G(k) = 2^(-(k du)^2) for k < the image width + 8
@@ -54,6 +55,12 @@ hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
 
 constexpr int kWaves = 16;           // waves per workgroup of the fused kernel
 constexpr int kMaxColStride = 256;   // largest LDS column stride (window height)
+
+// n_box and n_ell per event from the row intervals of the fused walks
+// (valid when rows are hole-free intervals); scan = 1 forces the scanning
+// path for every row.
+hipError_t launch_row_counts(hipStream_t st, const StripConst& sg, const EventGeom* geom,
+                             int64_t n, int scan, int32_t* nbox, int32_t* nell);
 
 // LDS bytes per workgroup for windows of at most max_ww columns with column
 // stride col_stride.

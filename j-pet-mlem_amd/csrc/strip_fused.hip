@@ -136,6 +136,16 @@ struct EventRows {
   float c005;  // 4 C * 0.05
 };
 
+__device__ __forceinline__ EventRows event_rows(const StripConst& sg, const EventGeom& g,
+                                               float inv_pw) {
+  EventRows ev;
+  ev.c4 = 4.0f * g.C;
+  ev.c005 = ev.c4 * 0.05f;
+  ev.kcol = 0.5f * inv_pw * __builtin_amdgcn_rcpf(g.C);
+  ev.zc = (g.z - sg.tl_z_half_w) * inv_pw;
+  return ev;
+}
+
 // One box row of an event as a lane walks it.
 struct RowSeg {
   int len;        // ellipse pixels in the row (0: nothing to do)
@@ -149,7 +159,8 @@ struct RowSeg {
 template <bool SIGMA>
 __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
                                             const EventGeom& g, const EventRows& ev,
-                                            int iy, int row_w, int cstride, float s_pix) {
+                                            int iy, int row_w, int cstride, float s_pix,
+                                            bool scan) {
   RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
   const float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
   const float Ady2 = (dy * g.A) * dy;
@@ -176,9 +187,10 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   // no column of [r-1, q+1] passes: the row is empty when the estimated
   // interval is shorter than two columns (every column in it was tested)
   const bool none = !(in_r || in_r2 || in_q || in_q2) && (chi - clo < 2.0f);
-  if (!((left && right) || none)) {
-    // the estimate did not settle the row: scan a superset (roots at 9.05
-    // widened by a column) from both ends
+  if (scan || !((left && right) || none)) {
+    // the estimate did not settle the row (or the caller asks for the
+    // scan): scan a superset (roots at 9.05 widened by a column) from both
+    // ends
     const float sq2 = __builtin_amdgcn_sqrtf(d9 + ev.c005);
     const float flo = floorf(fmaf(-(Bdy + sq2), ev.kcol, ev.zc)) - 1.0f;
     const float fhi = ceilf(fmaf(sq2 - Bdy, ev.kcol, ev.zc)) + 1.0f;
@@ -489,12 +501,18 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = __lane_id();
 
+  // Window load and flush visit tiles of 16 columns x 4 rows per wave, so
+  // that the global reads and atomics of a wave cover 64-byte row segments.
+  const int tc_lane = lane & 15, tr_lane = lane >> 4;
   if constexpr (kRho) {
-    for (int i = threadIdx.x; i < nrho; i += kThreads) {
-      int c = i / HP, r = i - c * HP;
+    const int ntc = (ww + kSpareCols + 15) >> 4, ntr = HP / 4;
+    for (int t = wave; t < ntc * ntr; t += kWaves) {
+      const int tr = t / ntc;
+      const int c = (t - tr * ntc) * 16 + tc_lane, r = tr * 4 + tr_lane;
+      if (c >= ww + kSpareCols) continue;
       float v = 0.0f;
       if (r < wh && c < ww) v = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
-      rho_w[i] = v;
+      rho_w[c * HP + r] = v;
     }
   }
   if constexpr (kPass2) {
@@ -503,6 +521,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   __syncthreads();
 
   const float s_pix = a.s_pix;  // sqrt(isz log2 e)
+  const bool scan = a.row_scan != 0;
   const float du = a.du;        // u step per column, pw s_pix
   ConstF* gtab = (ConstF*)a.gtab;  // uniform reads through the scalar cache
   const float inv_pw = 1.0f / sg.pw;
@@ -518,11 +537,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
     const int ngroups = (nrows + 63) >> 6;
     const int row0 = (g.y0 - it.wy0) - it.wx0 * HP;  // window index of (g.y0, 0)
     const int orig = a.sidx[e];
-    EventRows ev;
-    ev.c4 = 4.0f * g.C;
-    ev.c005 = ev.c4 * 0.05f;
-    ev.kcol = 0.5f * inv_pw * __builtin_amdgcn_rcpf(g.C);
-    ev.zc = (g.z - sg.tl_z_half_w) * inv_pw;
+    const EventRows ev = event_rows(sg, g, inv_pw);
     // group 0 takes the 64 rows around the ellipse centre, later groups the
     // rows above and below it in order
     const int mid = (int)((sg.tl_y - g.y) * inv_ph) - g.y0;
@@ -542,7 +557,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
           if (gg * 64 + lane < nrows)
-            s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix);
+            s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix, scan);
           if (gi < kCachedGroups) cache[gi] = s;
           float s0 = 0.0f, s1 = 0.0f;
           forward_rows<HP, COMP, kSigma>(s, du, gtab, rho_w, s0, comp, s1);
@@ -609,7 +624,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
             s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
             if (gg * 64 + lane < nrows)
-              s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix);
+              s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix, scan);
           }
           if constexpr (MODE == MODE_MLEM) {
             // K rho 2^bits / lambda <= 2^bits, but the chain's K 2^bits /
@@ -667,10 +682,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
 
   if constexpr (kPass2) {
     __syncthreads();
-    for (int i = threadIdx.x; i < npx; i += kThreads) {
-      int c = i / HP, r = i - c * HP;
-      if (r >= wh) continue;
-      U v = acc_w[i];
+    const int ntc = (ww + 15) >> 4, ntr = (wh + 3) >> 2;
+    for (int t = wave; t < ntc * ntr; t += kWaves) {
+      const int tr = t / ntc;
+      const int c = (t - tr * ntc) * 16 + tc_lane, r = tr * 4 + tr_lane;
+      if (c >= ww || r >= wh) continue;
+      U v = acc_w[c * HP + r];
       if (v == 0ull) continue;
       size_t p = (size_t)(it.wy0 + r) * sg.W + (it.wx0 + c);
       if (a.acc_kind == GLOBAL_U64)
@@ -683,6 +700,38 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
                         (float)((double)(long long)v * a.inv_fixed_scale));
     }
   }
+}
+
+// Per-event pixel counts of the fused walks, one wave per event: n_box of
+// the clamped box and n_ell, the sum of the row intervals row_setup finds
+// (the pixels the reference's box scan accepts, reconstruction.h:236-247).
+__global__ __launch_bounds__(256) void k_row_counts(StripConst sg, const EventGeom* geom,
+                                                    int64_t n, int scan, int32_t* nbox,
+                                                    int32_t* nell) {
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= n) return;  // uniform per wave
+  const int lane = __lane_id();
+  const EventGeom g = geom[e];
+  const int nrows = max(g.y1 - g.y0, 0), ncols = max(g.x1 - g.x0, 0);
+  const EventRows ev = event_rows(sg, g, 1.0f / sg.pw);
+  int cnt = 0;
+  if (ncols > 0) {
+    for (int r = lane; r < nrows; r += 64)
+      cnt += row_setup<false>(sg, sg, g, ev, g.y0 + r, 0, 0, 1.0f, scan != 0).len;
+  }
+  for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  if (lane == 0) {
+    nbox[e] = nrows * ncols;
+    nell[e] = cnt;
+  }
+}
+
+hipError_t launch_row_counts(hipStream_t st, const StripConst& sg, const EventGeom* geom,
+                             int64_t n, int scan, int32_t* nbox, int32_t* nell) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, sg,
+                     geom, n, scan, nbox, nell);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -26,18 +26,54 @@ def small(gpu_pkg, n_pix=64, size=0.008, n=20000, seed=3, phantom=None):
     return sc, ev
 
 
-def test_footprints_exact(gpu_pkg):
+@pytest.mark.parametrize("row_scan", [False, True])
+def test_footprints_exact(gpu_pkg, row_scan):
+    """Per-event box and ellipse pixel counts of the row intervals the fused
+    walks use (from the roots of each row's quadratic, or by scanning)
+    against the oracle's test of every box pixel."""
     sc, ev = small(gpu_pkg, n=30000)
-    with gpu_pkg.Reconstruction(sc) as rec:
+    with gpu_pkg.Reconstruction(sc, row_scan=row_scan) as rec:
         rec.set_events(ev)
         nb, ne = rec.footprints()
         st = rec.stats()
     onb, one = ol.footprint(sc, ev)
-    print(f"case: 64x64, 3e4 events, box {onb.sum()} ellipse {one.sum()}", flush=True)
+    print(f"case: 64x64, 3e4 events, row_scan {row_scan}, box {onb.sum()} "
+          f"ellipse {one.sum()}", flush=True)
     assert np.array_equal(nb, onb)
     assert np.array_equal(ne, one)
     assert st["n_box_pixels"] == int(onb.sum())
     assert st["n_ellipse_pixels"] == int(one.sum())
+
+
+def test_footprints_exact_512px_workload(gpu_pkg):
+    """Row intervals at 512 x 512 pixels of 1 mm (2e5 events of the
+    six-ellipse phantom, ~3900 ellipse pixels each) against the oracle."""
+    name = "strip-512px-1mm-2e7ev"
+    sc = gpu_pkg.workload_scanner(name)
+    ev = gpu_pkg.workload_events(name, n=200000)
+    with gpu_pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        nb, ne = rec.footprints()
+    onb, one = ol.footprint(sc, ev, n_threads=16)
+    bad = np.flatnonzero(ne != one)
+    print(f"case: {name} 2e5 events, ellipse pixels {one.sum()}, mismatches "
+          f"{bad.size}", flush=True)
+    assert np.array_equal(nb, onb)
+    assert bad.size == 0
+
+
+def test_row_scan_identical_image(gpu_pkg):
+    """The deterministic MLEM image is bit-identical whether each row's
+    interval comes from the root estimate or from the scan."""
+    sc, ev = small(gpu_pkg, n=20000)
+    outs = []
+    for row_scan in (False, True):
+        with gpu_pkg.Reconstruction(sc, accumulation=gpu_pkg.ACCUM_FIXED64,
+                                    row_scan=row_scan) as rec:
+            rec.set_events(ev)
+            rec(2)
+            outs.append(rec.rho)
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
 def test_denominators(gpu_pkg):
