@@ -148,6 +148,7 @@ struct petstrip_handle {
   int* d_flag = nullptr;  // device overflow flag of the fixed-point image
   // G(k) = 2^(-(k du)^2) of the fused walks, for the current sk and pw
   float* d_gtab = nullptr;
+  float* d_mtab = nullptr;  // (k, k^2) pairs
   size_t gtab_n = 0;
   float s_pix = 0.0f, du = -1.0f;
   bool fused_ok = false;
@@ -199,6 +200,9 @@ constexpr int kChunk = 1024;      // events per work item
 // accumulation mode (two conversions)
 constexpr int kMlemFixedBits = 31;
 constexpr int kFixed64Bits = 40;
+// log-likelihood gradient (K / lambda, unbounded): 2^-26 units, values up to
+// 2^31.5 units take one conversion, up to 2^52 the split one
+constexpr int kGradBits = 26;
 
 // Along a box row the reference's float ellipse test has no holes when the
 // quadratic's column-to-column curvature C*pw^2 dominates the rounding of q
@@ -214,9 +218,10 @@ bool rows_are_intervals(const StripConst& s) {
   return C * pw * pw > 50.0 * err;
 }
 
-// The per-column factor table of the fused walks (see strip_fused.hip):
-// G(k) = 2^(-(k du)^2), du = pw sqrt(isz log2 e) with the kernel's sigma_z,
-// rounded once from double; rebuilt when sigma_z or the width change.
+// The per-column tables of the fused walks (see strip_fused.hip): G(k) =
+// 2^(-(k du)^2), du = pw sqrt(isz log2 e) with the kernel's sigma_z, rounded
+// once from double, and the (k, k^2) pairs of the sigma moments; rebuilt
+// when sigma_z or the width change.
 int update_kernel_table(petstrip_handle* h) {
   const float s_pix = sqrtf(h->sk.isz * 1.44269504088896341f);
   const float du = h->sg.pw * s_pix;
@@ -229,9 +234,17 @@ int update_kernel_table(petstrip_handle* h) {
   }
   if (n != h->gtab_n) {
     dfree(h->d_gtab);
+    dfree(h->d_mtab);
     h->gtab_n = 0;
     int rc;
     if ((rc = dalloc(&h->d_gtab, n))) return rc;
+    if ((rc = dalloc(&h->d_mtab, 2 * n))) return rc;
+    std::vector<float> m(2 * n);
+    for (size_t k = 0; k < n; ++k) {
+      m[2 * k] = (float)k;
+      m[2 * k + 1] = (float)((double)k * (double)k);
+    }
+    HIP_TRY(hipMemcpy(h->d_mtab, m.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice));
     h->gtab_n = n;
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -379,6 +392,7 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.s_pix = h->s_pix;
   a.du = h->du;
   a.gtab = h->d_gtab;
+  a.mtab = h->d_mtab;
   return a;
 }
 
@@ -526,6 +540,7 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_items);
   dfree(h->d_flag);
   dfree(h->d_gtab);
+  dfree(h->d_mtab);
   if (h->d_acc) (void)hipFree(h->d_acc);
   h->d_acc = nullptr;
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -803,6 +818,41 @@ int backward_per_event(petstrip_handle* h, const float* g, float* grad_rho,
   return PETSTRIP_OK;
 }
 
+// The log-likelihood gradient (g = 1/lambda) in one GRAD launch: lambda,
+// sigma derivative and log-likelihood terms in pass 1, sum_e K / lambda_e in
+// pass 2 with the row setups kept from pass 1; grad = that - 1.
+int gradient_fused(petstrip_handle* h, float* grad_rho, double grad_sigma[2],
+                   double* ll_terms) {
+  const int64_t n = h->n_events;
+  petstrip::FusedArgs a = fused_args(h);
+  a.acc_kind = petstrip::GLOBAL_F64;
+  a.log2_scale = (float)kGradBits;
+  a.split = 0;
+  a.inv_fixed_scale = std::ldexp(1.0, -kGradBits);
+  HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
+  HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
+  HIP_TRY(hipEventRecord(h->evk0, h->stream));
+  HIP_TRY(petstrip::launch_strip_fused(
+      h->stream, petstrip::MODE_GRAD, h->col_stride, 0, a, h->n_items,
+      petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD)));
+  HIP_TRY(hipEventRecord(h->evk1, h->stream));
+  HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0, -1.0f,
+                                    0, nullptr, h->d_out, (int64_t)h->n_pix));
+  double scal[8];
+  HIP_TRY(hipMemcpyAsync(scal, h->d_scalars, sizeof(scal), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipMemcpyAsync(grad_rho, h->d_out, h->n_pix * sizeof(float),
+                         hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->lambda_valid = true;
+  if (grad_sigma) {
+    grad_sigma[0] = scal[1];
+    grad_sigma[1] = scal[2];
+  }
+  *ll_terms = scal[0];
+  return PETSTRIP_OK;
+}
+
 // Fused form: one FORWARD pass (lambda, sigma derivative, log-likelihood
 // terms, largest and mean |g| K) and one BACKWARD pass (K g_e into the image
 // in signed fixed point).  For non-negative weights whose largest |g| K is
@@ -815,6 +865,7 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
                    double grad_sigma[2], double* ll_terms) {
   const int64_t n = h->n_events;
   petstrip::FusedArgs a = fused_args(h);
+  if (!g && grad_rho) return gradient_fused(h, grad_rho, grad_sigma, ll_terms);
   bool mixed_signs = false;
   if (g) {
     if (n > 0)

@@ -19,7 +19,7 @@ struct WorkItem {
 };
 
 enum { GLOBAL_F32 = 0, GLOBAL_F64 = 1, GLOBAL_U64 = 2 };
-enum { MODE_MLEM = 0, MODE_FORWARD = 1, MODE_BACKWARD = 2 };
+enum { MODE_MLEM = 0, MODE_FORWARD = 1, MODE_BACKWARD = 2, MODE_GRAD = 3 };
 
 struct FusedArgs {
   StripConst sg;             // constants the ellipse/box support was built with
@@ -39,6 +39,7 @@ struct FusedArgs {
   float s_pix;               // sqrt(sk.isz log2 e): u = pq s_pix
   float du;                  // sg.pw s_pix, the u step per column
   const float* gtab;         // G(k) = 2^(-(k du)^2) for k < the image width + 8
+  const float* mtab;         // (k, k^2) pairs for the same k
   double* scalars;           // FORWARD: {sum log lambda, dl/dsz, dl/dsdl,
                              //           max_e |g_e| K_e, sum_e |g_e| K_e}
                              // with K_e the event's largest kernel value

@@ -23,6 +23,9 @@
 //             (split conversion) for the deterministic accumulation
 //   FORWARD   pass 1 only, plus the sigma derivative of lambda_e and the
 //             log-likelihood terms, reduced per workgroup into doubles
+//   GRAD      FORWARD and a pass 2 with w_e = 1/lambda_e (no rho): the
+//             log-likelihood gradient in one launch, values in 2^-k units
+//             with a per-row-group choice of conversion
 //   BACKWARD  pass 2 only with upstream weights g_e, values |K g_e| in
 //             units 2^-k chosen on the host, added or subtracted by the
 //             sign of g_e: each value below 2^31 when the largest |g_e| K
@@ -284,19 +287,22 @@ __device__ __forceinline__ void read4(const float* rp, float (&rv)[kUnroll]) {
 // Pass 1 over one row group: the lane's full groups of four pixels (kernel
 // values from the chain), each group's sum masked once, then its last
 // len % 4 pixels at its own offset (kernel values evaluated directly).
-// s0 += sum K rho (two_sum error into comp when COMP), s1 += sum K rho u^2.
+// s0 += sum K rho (two_sum error into comp when COMP); SIGMA: s1 += sum K
+// rho u^2, accumulated over the full groups as the moments sum k K rho and
+// sum k^2 K rho of the pixel index k (k and k^2 from a table through the
+// scalar cache) and combined per row as u0^2 S + 2 u0 du M1 + du^2 M2.
 // A lane past its full groups re-reads its last one (masked), so no read
 // leaves the window.
 template <int HP, bool COMP, bool SIGMA>
 __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* gtab,
-                                             const float* rho_w, float& s0, float& comp,
-                                             float& s1) {
+                                             ConstF* mtab, const float* rho_w, float& s0,
+                                             float& comp, float& s1) {
   const float* rp = rho_w + s.w0;
   const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
   const int nfull = s.len >> 2;
   const int lastg = max(nfull - 1, 0);
   const int maxfull = uniform(wave_max_i(nfull));
-  auto add = [&](float t0, float t1) {
+  auto add = [&](float t0) {
     if constexpr (COMP) {
       float sum, err;
       two_sum(s0, t0, sum, err);
@@ -305,29 +311,38 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* 
     } else {
       s0 += t0;
     }
-    if constexpr (SIGMA) s1 += t1;
   };
   Chain c = chain_start(s, du, 0.0f);
-  float kf = 0.0f;
-  for (int gk = 0; gk < maxfull; ++gk, kf += (float)kUnroll) {
+  float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;  // full-group moments (SIGMA)
+  for (int gk = 0; gk < maxfull; ++gk) {
     float rv[kUnroll];
     read4<HP>(rp + min(gk, lastg) * (kUnroll * HP), rv);
-    float t0 = 0.0f, t1 = 0.0f;
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const float kv = c.p[u] * gtab[gk * kUnroll + u];
+      const int k = gk * kUnroll + u;
+      const float kv = c.p[u] * gtab[k];
       c.p[u] *= c.b4;
       if constexpr (SIGMA) {
         const float kr = kv * rv[u];
-        const float uu = fmaf(kf, du, uo[u]);
         t0 += kr;
-        t1 = fmaf(kr, uu * uu, t1);
+        t1 = fmaf(kr, mtab[2 * k], t1);
+        t2 = fmaf(kr, mtab[2 * k + 1], t2);
       } else {
         t0 = fmaf(kv, rv[u], t0);
       }
     }
     const bool on = gk < nfull;
-    add(on ? t0 : 0.0f, on ? t1 : 0.0f);
+    add(on ? t0 : 0.0f);
+    if constexpr (SIGMA) {
+      m0 += on ? t0 : 0.0f;
+      m1 += on ? t1 : 0.0f;
+      m2 += on ? t2 : 0.0f;
+    }
+  }
+  if constexpr (SIGMA) {
+    const float a1 = 2.0f * s.u0 * du, a2 = du * du;
+    s1 += fmaf(a2, m2, fmaf(a1, m1, s.u0 * s.u0 * m0));
   }
   const int rem = s.len & (kUnroll - 1);
   if (__any(rem > 0)) {
@@ -345,7 +360,8 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* 
       t0 += kr;
       if constexpr (SIGMA) t1 = fmaf(kr, uu * uu, t1);
     }
-    add(t0, t1);
+    add(t0);
+    if constexpr (SIGMA) s1 += t1;
   }
 }
 
@@ -482,7 +498,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   constexpr bool kPass1 = MODE != MODE_BACKWARD;
   constexpr bool kPass2 = MODE != MODE_FORWARD;
   constexpr bool kRho = MODE != MODE_BACKWARD;  // rho window in LDS
-  constexpr bool kSigma = MODE == MODE_FORWARD;
+  constexpr bool kSigma = MODE == MODE_FORWARD || MODE == MODE_GRAD;
   constexpr int kThreads = 64 * kWaves;
   using U = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -524,6 +540,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   const bool scan = a.row_scan != 0;
   const float du = a.du;        // u step per column, pw s_pix
   ConstF* gtab = (ConstF*)a.gtab;  // uniform reads through the scalar cache
+  ConstF* mtab = (ConstF*)a.mtab;
   const float inv_pw = 1.0f / sg.pw;
   const float inv_ph = 1.0f / sg.ph;
   const float cz_pix = 2.0f / (kLog2e * sk.sigma_z);  // 2 isz pq^2 / sz = cz u^2
@@ -560,7 +577,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix, scan);
           if (gi < kCachedGroups) cache[gi] = s;
           float s0 = 0.0f, s1 = 0.0f;
-          forward_rows<HP, COMP, kSigma>(s, du, gtab, rho_w, s0, comp, s1);
+          forward_rows<HP, COMP, kSigma>(s, du, gtab, mtab, rho_w, s0, comp, s1);
           if constexpr (COMP) {
             float sum, err;
             two_sum(acc, s0, sum, err);
@@ -599,7 +616,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
       // lw = log2 of the event multiplier in fixed-point units
       float lw;
       bool neg = false;
-      if constexpr (MODE == MODE_MLEM) {
+      if constexpr (MODE == MODE_MLEM || MODE == MODE_GRAD) {
         if (!(lam > 0.0f)) continue;
         lw = log2_shift(a.log2_scale, lam, -1.0f);
       } else if (a.g) {
@@ -617,11 +634,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
           if (gg >= ngroups) break;
           RowSeg s;
+          const int j = gg * 64 + lane - 64;
+          const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           if (kPass1 && gi < kCachedGroups) {
             s = cache[gi];
           } else {
-            const int j = gg * 64 + lane - 64;
-            const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
             s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
             if (gg * 64 + lane < nrows)
               s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix, scan);
@@ -645,6 +662,28 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             } else {
               backproject_rows<HP, true, false, false, false>(s, du, lw, lam, unit_scale,
                                                               gtab, rho_w, acc_w);
+            }
+          } else if constexpr (MODE == MODE_GRAD) {
+            // K / lambda has no bound: rows whose values stay below 2^31.5
+            // (or 2^52) fixed-point units take the single (split)
+            // conversion, the rare rest is added to the global image in
+            // doubles
+            const float vmax = s.lsc + lw;
+            if (!__any(vmax > 31.5f)) {
+              backproject_rows<HP, false, true, false, false>(s, du, lw, 0.0f, unit_scale,
+                                                              gtab, rho_w, acc_w);
+            } else if (!__any(vmax > 52.0f)) {
+              backproject_rows<HP, false, true, true, false>(s, du, lw, 0.0f, unit_scale,
+                                                             gtab, rho_w, acc_w);
+            } else if (s.len > 0) {
+              const int col0 = it.wx0 + (s.w0 - (row0 + r + it.wx0 * HP)) / HP;
+              double* img = reinterpret_cast<double*>(a.acc) +
+                            (size_t)(g.y0 + r) * sg.W + col0;
+              for (int k = 0; k < s.len; ++k) {
+                const float uu = fmaf((float)k, du, s.u0);
+                const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc));
+                unsafeAtomicAdd(img + k, (double)kv / (double)lam);
+              }
             }
           } else if (a.split) {
             // |g| K 2^k <= 2^62 / n_events by the choice of k
@@ -788,6 +827,8 @@ hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int comp
       return compensated
                  ? launch_h<MODE_FORWARD, true>(col_stride, st, a, n_items, lds_bytes)
                  : launch_h<MODE_FORWARD, false>(col_stride, st, a, n_items, lds_bytes);
+    case MODE_GRAD:
+      return launch_h<MODE_GRAD, false>(col_stride, st, a, n_items, lds_bytes);
     default:
       return launch_h<MODE_BACKWARD, false>(col_stride, st, a, n_items, lds_bytes);
   }

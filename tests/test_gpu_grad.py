@@ -49,6 +49,26 @@ def test_gradient_vs_oracle(gpu_pkg, path):
     assert ll == pytest.approx(ll_ref, rel=1e-6)
 
 
+@pytest.mark.parametrize("scale", [1.0, 1e-6, 1e-12])
+def test_gradient_scaled_image(gpu_pkg, scale):
+    """Log-likelihood gradient for images scaled down so that K / lambda
+    grows by 1/scale: the single-conversion, split-conversion and direct
+    global paths of the fused gradient against the FP64 oracle."""
+    pkg = gpu_pkg
+    sc, ev, rho, _ = setup(pkg, n=10000)
+    rho = (rho * scale).astype(np.float32)
+    with pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        rec.rho = rho
+        grad, gsig, _ = rec.backward()
+    _, _, g_ref, gs_ref = ol.loglik_f64(sc, ev, rho)
+    print(f"case scale {scale}: grad {rel(grad, g_ref):.2e} sigma {gsig} vs {gs_ref}",
+          flush=True)
+    assert rel(grad, g_ref) < 1e-5
+    assert gsig[0] == pytest.approx(gs_ref[0], rel=2e-4)
+    assert gsig[1] == pytest.approx(gs_ref[1], rel=2e-4)
+
+
 def test_adjoint_dot_product(gpu_pkg):
     pkg = gpu_pkg
     sc, ev, rho, _ = setup(pkg, n=15000)
