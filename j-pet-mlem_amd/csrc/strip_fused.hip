@@ -231,8 +231,10 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
     //   d lnK/d sz  = (-2 + isz (ap^2 + aq^2)/norm + 2 isz pq^2) / sz
     //   d lnK/d sdl = (-1 + isdl ar^2/norm + isdl br^2) / sdl
     const float apq = fmaf(ap, ap, aq * aq);
-    s.dz_row = (-2.0f + sk.isz * apq / norm) / sk.sigma_z;
-    s.ddl_row = (-1.0f + sk.isdl * (ar * ar) / norm + brb) / sk.sigma_dl;
+    const float rn = __builtin_amdgcn_rcpf(norm);
+    s.dz_row = fmaf(sk.isz * apq, rn, -2.0f) * __builtin_amdgcn_rcpf(sk.sigma_z);
+    s.ddl_row = (fmaf(sk.isdl * (ar * ar), rn, -1.0f) + brb) *
+                __builtin_amdgcn_rcpf(sk.sigma_dl);
   }
   return s;
 }
@@ -285,83 +287,86 @@ __device__ __forceinline__ void read4(const float* rp, float (&rv)[kUnroll]) {
 }
 
 // Pass 1 over one row group: the lane's full groups of four pixels (kernel
-// values from the chain), each group's sum masked once, then its last
-// len % 4 pixels at its own offset (kernel values evaluated directly).
+// values from the chain), then its last len % 4 pixels at its own offset
+// (kernel values evaluated directly).  A lane whose full groups are done
+// re-reads its last one (so no read leaves the window) with its chain set
+// to zero, which drops those pixels from every sum without a mask.
 // s0 += sum K rho (two_sum error into comp when COMP); SIGMA: s1 += sum K
-// rho u^2, accumulated over the full groups as the moments sum k K rho and
-// sum k^2 K rho of the pixel index k (k and k^2 from a table through the
-// scalar cache) and combined per row as u0^2 S + 2 u0 du M1 + du^2 M2.
-// A lane past its full groups re-reads its last one (masked), so no read
-// leaves the window.
+// rho u^2, from the moments M1 = sum k K rho and M2 = sum k^2 K rho of the
+// pixel index k (k and k^2 from a table through the scalar cache for the
+// full groups), combined per row as u0^2 S + 2 u0 du M1 + du^2 M2.
 template <int HP, bool COMP, bool SIGMA>
 __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* gtab,
                                              ConstF* mtab, const float* rho_w, float& s0,
                                              float& comp, float& s1) {
   const float* rp = rho_w + s.w0;
-  const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
   const int nfull = s.len >> 2;
   const int lastg = max(nfull - 1, 0);
   const int maxfull = uniform(wave_max_i(nfull));
+  float sum0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
   auto add = [&](float t0) {
     if constexpr (COMP) {
       float sum, err;
-      two_sum(s0, t0, sum, err);
-      s0 = sum;
+      two_sum(sum0, t0, sum, err);
+      sum0 = sum;
       comp += err;
     } else {
-      s0 += t0;
+      sum0 += t0;
     }
   };
   Chain c = chain_start(s, du, 0.0f);
-  float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;  // full-group moments (SIGMA)
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) c.p[u] = nfull > 0 ? c.p[u] : 0.0f;
   for (int gk = 0; gk < maxfull; ++gk) {
     float rv[kUnroll];
     read4<HP>(rp + min(gk, lastg) * (kUnroll * HP), rv);
-    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+    const float b4 = gk + 1 < nfull ? c.b4 : 0.0f;
+    // COMP sums the group before its two-sum; otherwise every pixel goes
+    // straight into the row sum
+    float t0 = 0.0f;
+    float& acc = COMP ? t0 : sum0;
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int k = gk * kUnroll + u;
       const float kv = c.p[u] * gtab[k];
-      c.p[u] *= c.b4;
+      c.p[u] *= b4;
       if constexpr (SIGMA) {
         const float kr = kv * rv[u];
-        t0 += kr;
-        t1 = fmaf(kr, mtab[2 * k], t1);
-        t2 = fmaf(kr, mtab[2 * k + 1], t2);
+        acc += kr;
+        m1 = fmaf(kr, mtab[2 * k], m1);
+        m2 = fmaf(kr, mtab[2 * k + 1], m2);
       } else {
-        t0 = fmaf(kv, rv[u], t0);
+        acc = fmaf(kv, rv[u], acc);
       }
     }
-    const bool on = gk < nfull;
-    add(on ? t0 : 0.0f);
-    if constexpr (SIGMA) {
-      m0 += on ? t0 : 0.0f;
-      m1 += on ? t1 : 0.0f;
-      m2 += on ? t2 : 0.0f;
-    }
-  }
-  if constexpr (SIGMA) {
-    const float a1 = 2.0f * s.u0 * du, a2 = du * du;
-    s1 += fmaf(a2, m2, fmaf(a1, m1, s.u0 * s.u0 * m0));
+    if constexpr (COMP) add(t0);
   }
   const int rem = s.len & (kUnroll - 1);
   if (__any(rem > 0)) {
     const int t = nfull * kUnroll;
     float rv[kUnroll];
     read4<HP>(rp + t * HP, rv);
-    float t0 = 0.0f, t1 = 0.0f;
+    float t0 = 0.0f;
 #pragma unroll
     for (int u = 0; u < kUnroll - 1; ++u) {
-      const float uu = fmaf((float)t, du, uo[u]);
+      const float kf = (float)(t + u);
+      const float uu = fmaf(kf, du, s.u0);
       // pixels past the row are dropped by selecting the product, so
       // whatever the window holds there cannot reach the sums
       float kr = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * rv[u];
       kr = (u < rem) ? kr : 0.0f;
       t0 += kr;
-      if constexpr (SIGMA) t1 = fmaf(kr, uu * uu, t1);
+      if constexpr (SIGMA) {
+        m1 = fmaf(kr, kf, m1);
+        m2 = fmaf(kr * kf, kf, m2);
+      }
     }
     add(t0);
-    if constexpr (SIGMA) s1 += t1;
+  }
+  s0 += sum0;
+  if constexpr (SIGMA) {
+    const float a1 = 2.0f * s.u0 * du, a2 = du * du;
+    s1 += fmaf(a2, m2, fmaf(a1, m1, s.u0 * s.u0 * sum0));
   }
 }
 
@@ -597,16 +602,21 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
       if (lane == 0) a.lambda[orig] = lam;
       if constexpr (kSigma) {
         // weight of this event: upstream g_e or 1/lambda_e (log-likelihood)
-        float ge = a.g ? a.g[orig] : (lam > 0.0f ? 1.0f / lam : 0.0f);
-        double wsz = wave_sum_d((double)sz), wsdl = wave_sum_d((double)sdl);
-        float kmax = __builtin_amdgcn_exp2f(wave_max_f(lmax));
+        const float ge =
+            a.g ? a.g[orig] : (lam > 0.0f ? __builtin_amdgcn_rcpf(lam) : 0.0f);
+        const float wsz = wave_sum(sz), wsdl = wave_sum(sdl);
+        float kmax = 0.0f;
+        if constexpr (MODE == MODE_FORWARD) kmax = __builtin_amdgcn_exp2f(wave_max_f(lmax));
         if (lane == 0) {
-          if (lam > 0.0f) ll += log((double)lam);
-          gsz += (double)ge * wsz;
-          gsdl += (double)ge * wsdl;
-          const double m = (double)fabsf(ge) * (double)kmax;
-          mmax = fmax(mmax, m);
-          msum += m;
+          // ln lambda from the hardware log2; the sum over events is in doubles
+          if (lam > 0.0f) ll += (double)__builtin_amdgcn_logf(lam) * 0.69314718055994531;
+          gsz += (double)ge * (double)wsz;
+          gsdl += (double)ge * (double)wsdl;
+          if constexpr (MODE == MODE_FORWARD) {
+            const double m = (double)fabsf(ge) * (double)kmax;
+            mmax = fmax(mmax, m);
+            msum += m;
+          }
         }
       }
     }
