@@ -94,7 +94,10 @@ typedef struct petstrip_options {
   int32_t row_scan;     /* verification switch: 1 = find every row's ellipse
                            interval by scanning instead of from the roots of
                            its quadratic; results are identical, slower     */
-  int32_t reserved[4];  /* zero                                            */
+  int32_t force_bands;  /* verification switch: 1 = run the fused passes in
+                           64-row bands (the form used when a window does
+                           not fit LDS) even when windows fit              */
+  int32_t reserved[3];  /* zero                                            */
 } petstrip_options;
 
 typedef struct petstrip_handle petstrip_handle;
@@ -108,7 +111,8 @@ typedef struct petstrip_stats {
   double flop_alg;           /* sum_e 15 N_box + 41 N_ell + 50           */
   double last_iteration_ms;  /* device time of the last iteration        */
   double last_kernel_ms;     /* device time of its event kernel(s)       */
-  int32_t fused;             /* 1 if iterations use the binned kernel    */
+  int32_t fused;             /* 1: iterations use the binned kernel, 2: in
+                                64-row bands, 0: the per-event kernels */
   int32_t n_work_items;      /* workgroups of the binned kernel          */
   int64_t lds_bytes;         /* LDS per workgroup of the binned kernel   */
 } petstrip_stats;

@@ -41,6 +41,12 @@ int fail(int code, const std::string& msg) {
                   std::string(#expr) + ": " + hipGetErrorString(_e));        \
   } while (0)
 
+int hiperr(hipError_t e) {
+  if (e == hipSuccess) return PETSTRIP_OK;
+  return fail(e == hipErrorOutOfMemory ? PETSTRIP_ENOMEM : PETSTRIP_EDEVICE,
+              std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
 // Scanner<float, short> and GaussianKernel<float> constants, computed with the
 // reference's float operation order (scanner.h:50-67, gaussian_kernel.h:27-33).
 StripConst make_const(const petstrip_scanner& sc, double sigma_z,
@@ -145,6 +151,13 @@ struct petstrip_handle {
   size_t lds_bytes = 0;
   int col_stride = 0;  // LDS column stride of the fused windows (0: none fits)
   int max_ww = 1;
+  // banded form (windows taller than LDS allows): per-band partial sums
+  bool banded = false;
+  int nbands = 1;
+  int64_t n_sorted = 0;    // events with a non-empty box (the binned ones)
+  float* d_part = nullptr;
+  float* d_lam_sorted = nullptr;
+  size_t cap_part = 0;
   int* d_flag = nullptr;  // device overflow flag of the fixed-point image
   // G(k) = 2^(-(k du)^2) of the fused walks, for the current sk and pw
   float* d_gtab = nullptr;
@@ -205,9 +218,14 @@ constexpr int kFixed64Bits = 40;
 constexpr int kGradBits = 26;
 
 // Along a box row the reference's float ellipse test has no holes when the
-// quadratic's column-to-column curvature C*pw^2 dominates the rounding of q
-// (|dq/dz| * ulp(z) + ulp(9)); then each row's ellipse pixels are one
-// interval, which the fused kernel relies on.
+// quadratic's column-to-column curvature dominates the rounding of q.  With
+// q convex in the column, q(c) <= (q(c-1) + q(c+1))/2 - C pw^2 exactly; if
+// the float q is within E of the exact one, a column rejected between two
+// accepted ones needs 2E > C pw^2.  E is bounded by |dq/dz| times the
+// rounding of dz (|dq/dz| <= 2 sqrt(9 C) + 1 on the ellipse, dz rounded to
+// about two units in the last place of the largest |z|) plus the rounding
+// of q near 9; rows are one interval when C pw^2 > 4 E, twice the margin
+// the argument needs.  The fused kernel and the footprint counts rely on it.
 bool rows_are_intervals(const StripConst& s) {
   double C = 2.0 * s.isz;
   double pw = s.pw;
@@ -215,7 +233,7 @@ bool rows_are_intervals(const StripConst& s) {
   double ulp_z = zmax * 1.2e-7;
   double dq = 2.0 * std::sqrt(9.0 * C) + 1.0;
   double err = dq * ulp_z + 9.0 * 2.4e-7;
-  return C * pw * pw > 50.0 * err;
+  return C * pw * pw > 4.0 * err;
 }
 
 // The per-column tables of the fused walks (see strip_fused.hip): G(k) =
@@ -317,6 +335,8 @@ int prepare_fused(petstrip_handle* h) {
   }
   h->max_rows = max_rows;
   h->max_ww = max_ww;
+  h->n_sorted = acc;
+  h->banded = false;
   h->col_stride = petstrip::fused_col_stride(max_wh);
   // Every mode accumulates its LDS window in exact fixed point (ds_add_u64
   // runs at several times the rate of ds_add_f32 on gfx950); the
@@ -325,6 +345,48 @@ int prepare_fused(petstrip_handle* h) {
   h->lds_bytes = h->col_stride
                      ? petstrip::fused_lds_bytes(max_ww, h->col_stride, petstrip::MODE_MLEM)
                      : (size_t)-1;
+  if ((h->lds_bytes > h->lds_limit || h->opts.force_bands) && !items.empty()) {
+    // banded form: each chunk's window cut into bands of kBandRows image rows
+    const int br = petstrip::kBandRows;
+    std::vector<petstrip::WorkItem> bands;
+    for (const auto& it : items) {
+      if (it.wh <= 0 || it.ww <= 0) continue;
+      for (int b = it.wy0 / br; b <= (it.wy0 + it.wh - 1) / br; ++b) {
+        petstrip::WorkItem bi = it;
+        bi.wy0 = std::max(it.wy0, b * br);
+        bi.wh = std::min(it.wy0 + it.wh, (b + 1) * br) - bi.wy0;
+        bi.pad = b;
+        bands.push_back(bi);
+      }
+    }
+    const size_t lds = petstrip::fused_lds_bytes(max_ww, br, petstrip::MODE_MLEM);
+    if (lds <= h->lds_limit) {
+      h->banded = true;
+      h->col_stride = br;
+      h->lds_bytes = lds;
+      h->nbands = (max_rows + br - 1) / br + 1;
+      const size_t need = (size_t)std::max<int64_t>(acc, 1) * h->nbands * 4;
+      if (need > h->cap_part) {
+        dfree(h->d_part);
+        h->cap_part = 0;
+        if ((rc = dalloc(&h->d_part, need))) return rc;
+        h->cap_part = need;
+      }
+      dfree(h->d_lam_sorted);
+      if ((rc = dalloc(&h->d_lam_sorted, (size_t)std::max<int64_t>(acc, 1)))) return rc;
+      items.swap(bands);
+      h->n_items = (int)items.size();
+      if (items.size() > h->cap_items) {
+        dfree(h->d_items);
+        if ((rc = dalloc(&h->d_items, items.size()))) return rc;
+        h->cap_items = items.size();
+      }
+      HIP_TRY(hipMemcpyAsync(h->d_items, items.data(),
+                             items.size() * sizeof(petstrip::WorkItem),
+                             hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+  }
   h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
                 h->opts.update == PETSTRIP_UPDATE_CPU &&
                 h->opts.path == PETSTRIP_PATH_AUTO;
@@ -393,7 +455,28 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.du = h->du;
   a.gtab = h->d_gtab;
   a.mtab = h->d_mtab;
+  a.part = h->d_part;
+  a.nbands = h->nbands;
+  a.lam_sorted = h->d_lam_sorted;
   return a;
+}
+
+// The forward projection of the fused forms: lambda per event and, with
+// sigma, the log-likelihood terms and the weighted sigma derivative in the
+// scalars (weights g, or 1/lambda when a.g is null).  The caller zeroes
+// lambda and the scalars.
+int fused_forward(petstrip_handle* h, const petstrip::FusedArgs& a, bool sigma) {
+  if (!h->banded)
+    return hiperr(petstrip::launch_strip_fused(
+        h->stream, petstrip::MODE_FORWARD, h->col_stride, 0, a, h->n_items,
+        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_FORWARD)));
+  const size_t part = (size_t)std::max<int64_t>(h->n_sorted, 1) * h->nbands * 4;
+  HIP_TRY(hipMemsetAsync(h->d_part, 0, part * sizeof(float), h->stream));
+  HIP_TRY(petstrip::launch_strip_banded(
+      h->stream, sigma ? petstrip::MODE_FORWARD : petstrip::MODE_MLEM, 1, a, h->n_items,
+      petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_FORWARD)));
+  HIP_TRY(petstrip::launch_band_combine(h->stream, a, h->n_sorted, sigma ? 1 : 0));
+  return PETSTRIP_OK;
 }
 
 int one_iteration(petstrip_handle* h) {
@@ -412,9 +495,16 @@ int one_iteration(petstrip_handle* h) {
     }
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
-    HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_MLEM, h->col_stride,
-                                         accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
-                                         h->n_items, h->lds_bytes));
+    if (h->banded) {
+      int rc;
+      if ((rc = fused_forward(h, a, false))) return rc;
+      HIP_TRY(petstrip::launch_strip_banded(h->stream, petstrip::MODE_MLEM, 2, a,
+                                            h->n_items, h->lds_bytes));
+    } else {
+      HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_MLEM, h->col_stride,
+                                           accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
+                                           h->n_items, h->lds_bytes));
+    }
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     // float accumulators already hold image units; the u64 one holds 2^-40
     double unit = a.acc_kind == petstrip::GLOBAL_U64 ? a.inv_fixed_scale : 1.0;
@@ -461,7 +551,8 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   if (rc) return rc;
   if (opts && (opts->accumulation < 0 || opts->accumulation > 2 ||
                opts->update < 0 || opts->update > 1 || opts->path < 0 ||
-               opts->path > 1 || opts->row_scan < 0 || opts->row_scan > 1))
+               opts->path > 1 || opts->row_scan < 0 || opts->row_scan > 1 ||
+               opts->force_bands < 0 || opts->force_bands > 1))
     return fail(PETSTRIP_EINVAL, "invalid options");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -541,6 +632,8 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_flag);
   dfree(h->d_gtab);
   dfree(h->d_mtab);
+  dfree(h->d_part);
+  dfree(h->d_lam_sorted);
   if (h->d_acc) (void)hipFree(h->d_acc);
   h->d_acc = nullptr;
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -568,7 +661,7 @@ int petstrip_set_options(petstrip_handle* h, const petstrip_options* opts) {
   if (!h || !opts) return fail(PETSTRIP_EINVAL, "bad argument");
   if (opts->accumulation < 0 || opts->accumulation > 2 || opts->update < 0 ||
       opts->update > 1 || opts->path < 0 || opts->path > 1 || opts->row_scan < 0 ||
-      opts->row_scan > 1)
+      opts->row_scan > 1 || opts->force_bands < 0 || opts->force_bands > 1)
     return fail(PETSTRIP_EINVAL, "invalid options");
   h->opts = *opts;
   h->lambda_valid = false;
@@ -731,7 +824,7 @@ int petstrip_sensitivity(petstrip_handle* h, int mode, float* out) {
 int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
   if (!h || !out) return fail(PETSTRIP_EINVAL, "bad argument");
   *out = h->stats;
-  out->fused = h->fused_ok ? 1 : 0;
+  out->fused = h->fused_ok ? (h->banded ? 2 : 1) : 0;
   out->n_work_items = h->n_items;
   out->lds_bytes = (int64_t)h->lds_bytes;
   return PETSTRIP_OK;
@@ -761,9 +854,7 @@ int petstrip_forward(petstrip_handle* h, float* lambda) {
     petstrip::FusedArgs a = fused_args(h);
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
     HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
-    HIP_TRY(petstrip::launch_strip_fused(
-        h->stream, petstrip::MODE_FORWARD, h->col_stride, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_FORWARD)));
+    if ((rc = fused_forward(h, a, false))) return rc;
   } else {
     HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
                                          h->d_rho, nullptr, h->d_lambda, h->d_inv));
@@ -832,9 +923,17 @@ int gradient_fused(petstrip_handle* h, float* grad_rho, double grad_sigma[2],
   HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
   HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
   HIP_TRY(hipEventRecord(h->evk0, h->stream));
-  HIP_TRY(petstrip::launch_strip_fused(
-      h->stream, petstrip::MODE_GRAD, h->col_stride, 0, a, h->n_items,
-      petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD)));
+  if (h->banded) {
+    int rc;
+    if ((rc = fused_forward(h, a, true))) return rc;
+    HIP_TRY(petstrip::launch_strip_banded(
+        h->stream, petstrip::MODE_GRAD, 2, a, h->n_items,
+        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD)));
+  } else {
+    HIP_TRY(petstrip::launch_strip_fused(
+        h->stream, petstrip::MODE_GRAD, h->col_stride, 0, a, h->n_items,
+        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD)));
+  }
   HIP_TRY(hipEventRecord(h->evk1, h->stream));
   HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0, -1.0f,
                                     0, nullptr, h->d_out, (int64_t)h->n_pix));
@@ -876,9 +975,10 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
   }
   HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
   HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
-  HIP_TRY(petstrip::launch_strip_fused(
-      h->stream, petstrip::MODE_FORWARD, h->col_stride, 0, a, h->n_items,
-      petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_FORWARD)));
+  {
+    int rc;
+    if ((rc = fused_forward(h, a, true))) return rc;
+  }
   double scal[8];
   HIP_TRY(hipMemcpyAsync(scal, h->d_scalars, sizeof(scal), hipMemcpyDeviceToHost,
                          h->stream));
@@ -904,9 +1004,14 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
     a.inv_fixed_scale = std::ldexp(1.0, -k);
     a.acc_kind = petstrip::GLOBAL_U64;
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
-    HIP_TRY(petstrip::launch_strip_fused(
-        h->stream, petstrip::MODE_BACKWARD, h->col_stride, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_BACKWARD)));
+    if (h->banded)
+      HIP_TRY(petstrip::launch_strip_banded(
+          h->stream, petstrip::MODE_BACKWARD, 2, a, h->n_items,
+          petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_BACKWARD)));
+    else
+      HIP_TRY(petstrip::launch_strip_fused(
+          h->stream, petstrip::MODE_BACKWARD, h->col_stride, 0, a, h->n_items,
+          petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_BACKWARD)));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_U64,
                                       a.inv_fixed_scale, bias, 0, nullptr, h->d_out,

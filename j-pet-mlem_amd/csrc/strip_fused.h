@@ -15,7 +15,7 @@ struct WorkItem {
   int32_t wx0, wy0;    // window origin (column, row)
   int32_t ww, wh;      // window size
   int32_t max_rows;    // largest box height in the chunk
-  int32_t pad;
+  int32_t pad;         // banded items: the band index (rows / kBandRows)
 };
 
 enum { GLOBAL_F32 = 0, GLOBAL_F64 = 1, GLOBAL_U64 = 2 };
@@ -40,6 +40,10 @@ struct FusedArgs {
   float du;                  // sg.pw s_pix, the u step per column
   const float* gtab;         // G(k) = 2^(-(k du)^2) for k < the image width + 8
   const float* mtab;         // (k, k^2) pairs for the same k
+  float* part;               // banded: per sorted event and band, (lambda,
+                             // sigma_z sum, sigma_dl sum, largest row factor)
+  int nbands;                // banded: band slots per event
+  float* lam_sorted;         // banded: lambda per sorted event
   double* scalars;           // FORWARD: {sum log lambda, dl/dsz, dl/dsdl,
                              //           max_e |g_e| K_e, sum_e |g_e| K_e}
                              // with K_e the event's largest kernel value
@@ -55,6 +59,7 @@ hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
                                WorkItem* items, int n_items);
 
 constexpr int kWaves = 16;           // waves per workgroup of the fused kernel
+constexpr int kBandRows = 64;        // rows per band of the banded form
 constexpr int kMaxColStride = 256;   // largest LDS column stride (window height)
 
 // n_box and n_ell per event from the row intervals of the fused walks
@@ -71,6 +76,16 @@ size_t fused_lds_bytes(int max_ww, int col_stride, int mode);
 int fused_col_stride(int max_wh);
 hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int compensated,
                               const FusedArgs& args, int n_items, size_t lds_bytes);
+// Banded form (column stride kBandRows): band = 1 runs pass 1 into the band
+// partials (mode MODE_FORWARD with sigma sums, else lambda only), band = 2
+// runs pass 2 of mode (MLEM, GRAD or BACKWARD) with the combined lambda.
+hipError_t launch_strip_banded(hipStream_t st, int mode, int band, const FusedArgs& args,
+                               int n_items, size_t lds_bytes);
+// Band partials -> lambda (sorted and original order); sigma = 1 also adds
+// the log-likelihood terms, weighted sigma derivatives, max and sum of |g| K
+// to the scalars (as the FORWARD mode does).
+hipError_t launch_band_combine(hipStream_t st, const FusedArgs& args, int64_t n_sorted,
+                               int sigma);
 
 // out = acc * unit + bias (acc read as kind; 64-bit values as signed), the
 // accumulator cleared; *overflow (nullable) set to 1 if a 64-bit value is

@@ -498,12 +498,19 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
 // different rows, so pixel (row, col) sits in bank (row + const) mod 64
 // whatever column each lane has reached, and every LDS access of the walks
 // is free of bank conflicts.
-template <int MODE, bool COMP, int HP>
+//
+// BAND = 1, 2: the banded form for windows too tall for LDS.  Work items then
+// cover one band of kBandRows image rows, and an event is walked in every
+// band its box reaches.  BAND = 1 runs pass 1 and stores the band's partial
+// (lambda, sigma_z and sigma_dl sums, largest row factor) per event and
+// band; k_band_combine adds them in a fixed order; BAND = 2 runs pass 2
+// with the complete lambda.
+template <int MODE, bool COMP, int HP, int BAND>
 __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
-  constexpr bool kPass1 = MODE != MODE_BACKWARD;
-  constexpr bool kPass2 = MODE != MODE_FORWARD;
-  constexpr bool kRho = MODE != MODE_BACKWARD;  // rho window in LDS
-  constexpr bool kSigma = MODE == MODE_FORWARD || MODE == MODE_GRAD;
+  constexpr bool kPass1 = BAND == 1 || (BAND == 0 && MODE != MODE_BACKWARD);
+  constexpr bool kPass2 = BAND == 2 || (BAND == 0 && MODE != MODE_FORWARD);
+  constexpr bool kRho = kPass1 || (kPass2 && MODE == MODE_MLEM);  // rho window in LDS
+  constexpr bool kSigma = kPass1 && (MODE == MODE_FORWARD || MODE == MODE_GRAD);
   constexpr int kThreads = 64 * kWaves;
   using U = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -555,14 +562,21 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
 
   for (int e = it.begin + wave; e < it.end; e += kWaves) {
     const EventGeom g = a.sgeom[e];
-    const int nrows = g.y1 - g.y0;
+    // box rows walked here: all of them, or those in this item's band
+    int ry0 = g.y0, ry1 = g.y1;
+    if constexpr (BAND != 0) {
+      ry0 = max(g.y0, it.wy0);
+      ry1 = min(g.y1, it.wy0 + wh);
+      if (ry1 <= ry0 || g.x1 <= g.x0) continue;
+    }
+    const int nrows = ry1 - ry0;
     const int ngroups = (nrows + 63) >> 6;
-    const int row0 = (g.y0 - it.wy0) - it.wx0 * HP;  // window index of (g.y0, 0)
+    const int row0 = (ry0 - it.wy0) - it.wx0 * HP;  // window index of (ry0, 0)
     const int orig = a.sidx[e];
     const EventRows ev = event_rows(sg, g, inv_pw);
     // group 0 takes the 64 rows around the ellipse centre, later groups the
     // rows above and below it in order
-    const int mid = (int)((sg.tl_y - g.y) * inv_ph) - g.y0;
+    const int mid = (int)((sg.tl_y - g.y) * inv_ph) - ry0;
     const int m0 = nrows > 64 ? min(max(mid - 32, 0), nrows - 64) : 0;
     RowSeg cache[kCachedGroups];
 
@@ -579,7 +593,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
           if (gg * 64 + lane < nrows)
-            s = row_setup<kSigma>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix, scan);
+            s = row_setup<kSigma>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan);
           if (gi < kCachedGroups) cache[gi] = s;
           float s0 = 0.0f, s1 = 0.0f;
           forward_rows<HP, COMP, kSigma>(s, du, gtab, mtab, rho_w, s0, comp, s1);
@@ -599,6 +613,20 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         }
       }
       lam = COMP ? wave_sum_comp(acc, comp) : wave_sum(acc);
+      if constexpr (BAND == 1) {
+        // this band's partial sums of the event
+        const float wsz = kSigma ? wave_sum(sz) : 0.0f;
+        const float wsdl = kSigma ? wave_sum(sdl) : 0.0f;
+        const float lmx = kSigma ? wave_max_f(lmax) : 0.0f;
+        if (lane == 0) {
+          float* p = a.part + ((size_t)e * a.nbands + (it.pad - g.y0 / kBandRows)) * 4;
+          p[0] = lam;
+          p[1] = wsz;
+          p[2] = wsdl;
+          p[3] = lmx;
+        }
+        continue;
+      }
       if (lane == 0) a.lambda[orig] = lam;
       if constexpr (kSigma) {
         // weight of this event: upstream g_e or 1/lambda_e (log-likelihood)
@@ -623,6 +651,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
 
     // ---- pass 2: backprojection into the LDS window --------------------------
     if constexpr (kPass2) {
+      if constexpr (BAND == 2) lam = a.lam_sorted[e];
       // lw = log2 of the event multiplier in fixed-point units
       float lw;
       bool neg = false;
@@ -651,7 +680,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           } else {
             s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
             if (gg * 64 + lane < nrows)
-              s = row_setup<false>(sg, sk, g, ev, g.y0 + r, row0 + r, HP, s_pix, scan);
+              s = row_setup<false>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan);
           }
           if constexpr (MODE == MODE_MLEM) {
             // K rho 2^bits / lambda <= 2^bits, but the chain's K 2^bits /
@@ -688,7 +717,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             } else if (s.len > 0) {
               const int col0 = it.wx0 + (s.w0 - (row0 + r + it.wx0 * HP)) / HP;
               double* img = reinterpret_cast<double*>(a.acc) +
-                            (size_t)(g.y0 + r) * sg.W + col0;
+                            (size_t)(ry0 + r) * sg.W + col0;
               for (int k = 0; k < s.len; ++k) {
                 const float uu = fmaf((float)k, du, s.u0);
                 const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc));
@@ -802,9 +831,9 @@ int fused_col_stride(int max_wh) {
 
 namespace {
 
-template <int MODE, bool COMP, int HP>
+template <int MODE, bool COMP, int HP, int BAND>
 hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds) {
-  auto kern = k_strip_fused<MODE, COMP, HP>;
+  auto kern = k_strip_fused<MODE, COMP, HP, BAND>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -816,11 +845,80 @@ template <int MODE, bool COMP>
 hipError_t launch_h(int col_stride, hipStream_t st, const FusedArgs& a, int n_items,
                     size_t lds) {
   switch (col_stride) {
-    case 64: return launch_t<MODE, COMP, 64>(st, a, n_items, lds);
-    case 128: return launch_t<MODE, COMP, 128>(st, a, n_items, lds);
-    case 192: return launch_t<MODE, COMP, 192>(st, a, n_items, lds);
-    case 256: return launch_t<MODE, COMP, 256>(st, a, n_items, lds);
+    case 64: return launch_t<MODE, COMP, 64, 0>(st, a, n_items, lds);
+    case 128: return launch_t<MODE, COMP, 128, 0>(st, a, n_items, lds);
+    case 192: return launch_t<MODE, COMP, 192, 0>(st, a, n_items, lds);
+    case 256: return launch_t<MODE, COMP, 256, 0>(st, a, n_items, lds);
     default: return hipErrorInvalidValue;
+  }
+}
+
+// Sums each event's band partials in band order: lambda (sorted and original
+// order) and, with sigma, the log-likelihood terms, the sigma derivative
+// weighted by g_e (or 1/lambda_e) and the largest |g_e| K; block sums go to
+// the scalars in doubles.
+__global__ __launch_bounds__(256) void k_band_combine(const float* part, int64_t n,
+                                                      int nbands, const int32_t* sidx,
+                                                      const float* g, int sigma,
+                                                      float* lam_sorted, float* lambda,
+                                                      double* scalars) {
+  __shared__ double red[4][5];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0, msum = 0.0;
+  if (e < n) {
+    float lam = 0.0f, sz = 0.0f, sdl = 0.0f, lmx = -1e30f;
+    const float* p = part + (size_t)e * nbands * 4;
+    for (int j = 0; j < nbands; ++j) {
+      lam += p[4 * j];
+      sz += p[4 * j + 1];
+      sdl += p[4 * j + 2];
+      // slots of bands the event does not reach hold zeros
+      if (p[4 * j] != 0.0f || p[4 * j + 3] != 0.0f) lmx = fmaxf(lmx, p[4 * j + 3]);
+    }
+    const int orig = sidx[e];
+    lam_sorted[e] = lam;
+    lambda[orig] = lam;
+    if (sigma) {
+      const float ge = g ? g[orig] : (lam > 0.0f ? 1.0f / lam : 0.0f);
+      if (lam > 0.0f) ll = log((double)lam);
+      gsz = (double)ge * sz;
+      gsdl = (double)ge * sdl;
+      mmax = (double)fabsf(ge) * (double)exp2f(lmx);
+      msum = mmax;
+    }
+  }
+  if (sigma) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      ll += __shfl_xor(ll, off, 64);
+      gsz += __shfl_xor(gsz, off, 64);
+      gsdl += __shfl_xor(gsdl, off, 64);
+      mmax = fmax(mmax, __shfl_xor(mmax, off, 64));
+      msum += __shfl_xor(msum, off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      red[w][0] = ll;
+      red[w][1] = gsz;
+      red[w][2] = gsdl;
+      red[w][3] = mmax;
+      red[w][4] = msum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < 4; ++i) {
+        red[0][0] += red[i][0];
+        red[0][1] += red[i][1];
+        red[0][2] += red[i][2];
+        red[0][3] = fmax(red[0][3], red[i][3]);
+        red[0][4] += red[i][4];
+      }
+      unsafeAtomicAdd(&scalars[0], red[0][0]);
+      unsafeAtomicAdd(&scalars[1], red[0][1]);
+      unsafeAtomicAdd(&scalars[2], red[0][2]);
+      atomicMax(reinterpret_cast<unsigned long long*>(&scalars[3]),
+                (unsigned long long)__double_as_longlong(red[0][3]));
+      unsafeAtomicAdd(&scalars[4], red[0][4]);
+    }
   }
 }
 
@@ -842,6 +940,29 @@ hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int comp
     default:
       return launch_h<MODE_BACKWARD, false>(col_stride, st, a, n_items, lds_bytes);
   }
+}
+
+hipError_t launch_strip_banded(hipStream_t st, int mode, int band, const FusedArgs& a,
+                               int n_items, size_t lds_bytes) {
+  if (n_items <= 0) return hipSuccess;
+  if (band == 1) {
+    return mode == MODE_FORWARD
+               ? launch_t<MODE_FORWARD, false, kBandRows, 1>(st, a, n_items, lds_bytes)
+               : launch_t<MODE_MLEM, false, kBandRows, 1>(st, a, n_items, lds_bytes);
+  }
+  switch (mode) {
+    case MODE_MLEM: return launch_t<MODE_MLEM, false, kBandRows, 2>(st, a, n_items, lds_bytes);
+    case MODE_GRAD: return launch_t<MODE_GRAD, false, kBandRows, 2>(st, a, n_items, lds_bytes);
+    default: return launch_t<MODE_BACKWARD, false, kBandRows, 2>(st, a, n_items, lds_bytes);
+  }
+}
+
+hipError_t launch_band_combine(hipStream_t st, const FusedArgs& a, int64_t n, int sigma) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_band_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     a.part, n, a.nbands, a.sidx, a.g, sigma, a.lam_sorted, a.lambda,
+                     a.scalars);
+  return hipGetLastError();
 }
 
 }  // namespace petstrip

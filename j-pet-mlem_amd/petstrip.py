@@ -43,7 +43,7 @@ class CScanner(C.Structure):
 class COptions(C.Structure):
     _fields_ = [("accumulation", C.c_int32), ("update", C.c_int32),
                 ("path", C.c_int32), ("row_scan", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("force_bands", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class CStats(C.Structure):
@@ -182,11 +182,12 @@ class Reconstruction:
     """
 
     def __init__(self, scanner, device=0, accumulation=ACCUM_FP32,
-                 update=UPDATE_CPU, path=PATH_AUTO, row_scan=False):
+                 update=UPDATE_CPU, path=PATH_AUTO, row_scan=False, force_bands=False):
         self.lib = load_library()
         self.scanner = scanner
         self._h = C.c_void_p()
-        opts = COptions(accumulation, update, path, 1 if row_scan else 0)
+        opts = COptions(accumulation, update, path, 1 if row_scan else 0,
+                        1 if force_bands else 0)
         sc = scanner.c()
         _check(self.lib.petstrip_create(C.byref(sc), device, C.byref(opts),
                                         C.byref(self._h)))
@@ -224,8 +225,9 @@ class Reconstruction:
         return s.value
 
     def set_options(self, accumulation=ACCUM_FP32, update=UPDATE_CPU, path=PATH_AUTO,
-                    row_scan=False):
-        o = COptions(accumulation, update, path, 1 if row_scan else 0)
+                    row_scan=False, force_bands=False):
+        o = COptions(accumulation, update, path, 1 if row_scan else 0,
+                     1 if force_bands else 0)
         _check(self.lib.petstrip_set_options(self._h, C.byref(o)))
 
     def set_events(self, responses):

@@ -237,3 +237,68 @@ def test_workload_160px_1e6(gpu_pkg, acc):
         print(f"iter {i + 1}: rel L2 vs f32 {e32:.2e} vs f64 {e64:.2e}", flush=True)
         assert e32 < 1e-5
         assert e64 < 1e-4
+
+
+def test_banded_matches_unbanded(gpu_pkg):
+    """The banded form (per-band partial denominators, combined in a fixed
+    order) against the one-window form on the same events: MLEM images,
+    denominators and the gradient."""
+    sc, ev = small(gpu_pkg, n=20000)
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.5, 1.5, (sc.n_y_pixels, sc.n_z_pixels)).astype(np.float32)
+    out = {}
+    for fb in (False, True):
+        with gpu_pkg.Reconstruction(sc, force_bands=fb) as rec:
+            rec.set_events(ev)
+            st = rec.stats()
+            rec.rho = rho
+            lam = rec.forward()
+            grad, gsig, ll = rec.backward(want_loglik=True)
+            rec.rho = rho
+            rec(3)
+            out[fb] = (st["fused"], lam, grad, gsig, ll, rec.rho)
+    print(f"case: 64x64 @ 8 mm, fused {out[False][0]} vs {out[True][0]}", flush=True)
+    assert out[False][0] == 1 and out[True][0] == 2
+    a, b = out[False], out[True]
+    assert np.allclose(a[1], b[1], rtol=2e-6, atol=0)
+    assert rel_l2(b[2], a[2]) < 1e-6
+    assert b[3][0] == pytest.approx(a[3][0], rel=1e-5)
+    assert b[3][1] == pytest.approx(a[3][1], rel=1e-5)
+    assert b[4] == pytest.approx(a[4], rel=1e-7)
+    assert rel_l2(b[5], a[5]) < 1e-6
+
+
+def test_banded_tall_boxes_vs_oracle(gpu_pkg):
+    """0.5 mm pixels: the boxes are ~234 rows tall, the windows do not fit
+    LDS and the fused passes run in bands; 3 MLEM iterations against the
+    FP32 and FP64 oracles, and the log-likelihood gradient against FP64."""
+    sc = gpu_pkg.Scanner(0.43, 0.5, 200, 200, 0.0005, 0.0005, 0.01, 0.04)
+    ev = gpu_pkg.phantom_events(sc, gpu_pkg.workloads.PHANTOM_3, 6000, seed=21)
+    with gpu_pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        st = rec.stats()
+        imgs = []
+        for _ in range(3):
+            rec(1)
+            imgs.append(rec.rho)
+        rho = imgs[-1]
+        grad, gsig, _ = rec.backward()
+    print(f"case: 200x200 @ 0.5 mm, 6000 events, fused {st['fused']}, "
+          f"lds {st['lds_bytes']}", flush=True)
+    assert st["fused"] == 2
+    with gpu_pkg.Reconstruction(sc) as rec2:
+        rec2.set_events(ev)
+        nb, ne = rec2.footprints()
+    onb, one = ol.footprint(sc, ev, n_threads=16)
+    assert np.array_equal(nb, onb) and np.array_equal(ne, one)
+    r32 = ol.mlem_f32(sc, ev, 3, n_threads=16)
+    r64 = ol.mlem_f64(sc, ev, 3, n_threads=16)
+    for i in range(3):
+        e32, e64 = rel_l2(imgs[i], r32[i]), rel_l2(imgs[i], r64[i])
+        print(f"iter {i + 1}: rel L2 vs f32 {e32:.2e} vs f64 {e64:.2e}", flush=True)
+        assert e32 < 1e-5 and e64 < 1e-4
+    _, _, g_ref, gs_ref = ol.loglik_f64(sc, ev, rho, n_threads=16)
+    m = rho > 1e-3 * rho.max()
+    assert rel_l2(grad[m], g_ref[m]) < 1e-5
+    assert gsig[0] == pytest.approx(gs_ref[0], rel=1e-3)
+    assert gsig[1] == pytest.approx(gs_ref[1], rel=1e-3)
