@@ -149,6 +149,15 @@ def main():
                       "ellipse_px_per_event": st["n_ellipse_pixels"] / max(n_ev, 1)},
         "setup_s": {"generate": t_gen, "upload_and_bin": t_setup},
     }
+    # the same events generated on the device (Philox stream of the host
+    # generator) straight into a fresh handle's buffers, binned
+    phantom, seed = pkg.WORKLOADS[name][4], pkg.WORKLOADS[name][5]
+    with pkg.Reconstruction(sc, device=0, accumulation=acc) as rec_dev:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rec_dev.set_phantom_events(phantom, n_ev, seed=seed)
+        torch.cuda.synchronize()
+        out["setup_s"]["generate_on_device_and_bin"] = time.perf_counter() - t0
     if not args.no_grad:
         # full image + (sigma_z, sigma_dl) gradient of the Poisson list-mode
         # log-likelihood at the current image: one forward pass (lambda, sigma

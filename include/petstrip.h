@@ -228,6 +228,25 @@ int petstrip_phantom_events(const petstrip_scanner* scanner,
                             uint64_t seed, int64_t n, float* out,
                             int n_threads);
 
+/* The same events generated on device `device` into d_out (device memory,
+ * n triples), one lane per slot.  Host and device runs of one seed agree
+ * except where the two libm implementations differ in the last bit of a
+ * double.  No counterpart in the reference (its phantom is host-only,
+ * src/2d/strip/phantom_cmd.cpp:214-394, with a shared random engine). */
+int petstrip_phantom_events_device(const petstrip_scanner* scanner,
+                                   const double* regions, int32_t n_regions,
+                                   uint64_t seed, int64_t n, float* d_out,
+                                   int device);
+
+/* Copies the handle's n_events responses (z_u, z_d, dl triples) to the host. */
+int petstrip_get_events(petstrip_handle* h, float* responses);
+
+/* Generates n phantom events on the handle's device straight into its event
+ * buffer (the scanner's sigmas set the noise) and prepares them as
+ * petstrip_set_events does. */
+int petstrip_set_phantom_events(petstrip_handle* h, const double* regions,
+                                int32_t n_regions, uint64_t seed, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

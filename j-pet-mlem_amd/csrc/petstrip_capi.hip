@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/petstrip.h"
+#include "phantom.h"
 #include "strip_fused.h"
 #include "strip_geometry.h"
 #include "strip_kernels.h"
@@ -112,6 +113,10 @@ void dfree(T*& p) {
 
 namespace petstrip {
 int set_error(int code, const char* msg) { return fail(code, msg); }
+int phantom_spec(const petstrip_scanner* sc, const double* regions, int32_t n_regions,
+                 uint64_t seed, PhantomSpec& p);
+int phantom_device(const PhantomSpec& p, int64_t n, float* out, hipStream_t st,
+                   bool* failed);
 }  // namespace petstrip
 
 struct petstrip_handle {
@@ -177,6 +182,9 @@ int ensure_device(petstrip_handle* h) {
 
 // Event arrays sized for n events.
 int reserve_events(petstrip_handle* h, int64_t n) {
+  // sorted event indices are int32
+  if (n > (int64_t)INT32_MAX)
+    return fail(PETSTRIP_EINVAL, "more than 2^31-1 events in one handle");
   if ((size_t)n <= h->cap_events && h->d_geom) return PETSTRIP_OK;
   dfree(h->d_resp);
   dfree(h->d_geom);
@@ -518,10 +526,12 @@ int one_iteration(petstrip_handle* h) {
   HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
                                        h->d_rho, denominator_weight(h),
                                        h->d_lambda, h->d_inv));
-  HIP_TRY(petstrip::launch_fill(h->stream, h->d_out, 0.0f, (int64_t)h->n_pix));
+  // the FP64 accumulator is zero between uses (finalize clears it)
   HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n,
-                                       h->d_rho, h->d_inv, h->d_out));
+                                       h->d_rho, h->d_inv, (double*)h->d_acc));
   HIP_TRY(hipEventRecord(h->evk1, h->stream));
+  HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0, 0.0f,
+                                    0, nullptr, h->d_out, (int64_t)h->n_pix));
   std::swap(h->d_rho, h->d_out);
   h->lambda_valid = false;
   return PETSTRIP_OK;
@@ -675,7 +685,6 @@ int petstrip_set_events(petstrip_handle* h, const float* responses, int64_t n) {
   if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
   if (n < 0 || (n > 0 && !responses))
     return fail(PETSTRIP_EINVAL, "bad responses/n");
-  if (n > ((int64_t)1 << 40)) return fail(PETSTRIP_EINVAL, "n too large");
   int rc;
   if ((rc = ensure_device(h))) return rc;
   if ((rc = reserve_events(h, n))) return rc;
@@ -698,6 +707,36 @@ int petstrip_set_events_device(petstrip_handle* h, const float* d_responses,
   if (n > 0)
     HIP_TRY(hipMemcpyAsync(h->d_resp, d_responses, (size_t)n * 3 * sizeof(float),
                            hipMemcpyDeviceToDevice, h->stream));
+  return prepare_events(h);
+}
+
+int petstrip_get_events(petstrip_handle* h, float* responses) {
+  if (!h || (h->n_events > 0 && !responses)) return fail(PETSTRIP_EINVAL, "bad argument");
+  int rc;
+  if ((rc = ensure_device(h))) return rc;
+  if (h->n_events > 0)
+    HIP_TRY(hipMemcpyAsync(responses, h->d_resp, (size_t)h->n_events * 3 * sizeof(float),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return PETSTRIP_OK;
+}
+
+int petstrip_set_phantom_events(petstrip_handle* h, const double* regions,
+                                int32_t n_regions, uint64_t seed, int64_t n) {
+  if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
+  if (n < 0) return fail(PETSTRIP_EINVAL, "bad n");
+  petstrip::PhantomSpec p;
+  int rc;
+  if ((rc = petstrip::phantom_spec(&h->scanner, regions, n_regions, seed, p))) return rc;
+  if ((rc = ensure_device(h))) return rc;
+  if ((rc = reserve_events(h, n))) return rc;
+  h->n_events = n;
+  bool failed = false;
+  if ((rc = petstrip::phantom_device(p, n, h->d_resp, h->stream, &failed))) return rc;
+  if (failed) {
+    h->n_events = 0;
+    return fail(PETSTRIP_ESTATE, "no detected emission after 2^16 draws in a slot");
+  }
   return prepare_events(h);
 }
 
@@ -886,11 +925,12 @@ int backward_per_event(petstrip_handle* h, const float* g, float* grad_rho,
                                             h->d_scalars + 2));
   }
   if (grad_rho) {
-    HIP_TRY(petstrip::launch_fill(h->stream, h->d_out, 0.0f, (int64_t)h->n_pix));
     HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n, nullptr,
-                                         h->d_inv, h->d_out));
-    if (!g)
-      HIP_TRY(petstrip::launch_add_scalar(h->stream, h->d_out, -1.0f, (int64_t)h->n_pix));
+                                         h->d_inv, (double*)h->d_acc));
+    // the log-likelihood gradient subtracts the sensitivity term, 1 per pixel
+    HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0,
+                                      g ? 0.0f : -1.0f, 0, nullptr, h->d_out,
+                                      (int64_t)h->n_pix));
     HIP_TRY(hipMemcpyAsync(grad_rho, h->d_out, h->n_pix * sizeof(float),
                            hipMemcpyDeviceToHost, h->stream));
   }

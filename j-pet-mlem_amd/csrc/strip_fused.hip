@@ -49,6 +49,8 @@
 // ellipse pixel and advanced by one pixel width per column (u = u0 + k du).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "strip_fused.h"
 #include "strip_geometry.h"
 
@@ -806,10 +808,17 @@ __global__ __launch_bounds__(256) void k_row_counts(StripConst sg, const EventGe
 
 hipError_t launch_row_counts(hipStream_t st, const StripConst& sg, const EventGeom* geom,
                              int64_t n, int scan, int32_t* nbox, int32_t* nell) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, sg,
-                     geom, n, scan, nbox, nell);
-  return hipGetLastError();
+  // one wave per event: slices of 2^24 events keep the grid's work-item
+  // count within 32 bits
+  constexpr int64_t kSlice = (int64_t)1 << 24;
+  for (int64_t e0 = 0; e0 < n; e0 += kSlice) {
+    const int64_t m = std::min(kSlice, n - e0);
+    hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st, sg,
+                       geom + e0, m, scan, nbox + e0, nell + e0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------
@@ -837,8 +846,16 @@ hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds)
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(n_items), dim3(64 * kWaves), lds, st, a);
-  return hipGetLastError();
+  // 2^21 items of 1024 work-items per launch keep the grid within 32 bits
+  constexpr int kSlice = 1 << 21;
+  for (int i0 = 0; i0 < n_items; i0 += kSlice) {
+    FusedArgs b = a;
+    b.items = a.items + i0;
+    hipLaunchKernelGGL(kern, dim3((unsigned)std::min(kSlice, n_items - i0)),
+                       dim3(64 * kWaves), lds, st, b);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <int MODE, bool COMP>

@@ -19,7 +19,7 @@ hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
 hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* w,
-                              float* out);
+                              double* out);
 hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
                              const StripConst& sk, const EventGeom* geom,
                              int64_t n, const float* rho, const float* w,

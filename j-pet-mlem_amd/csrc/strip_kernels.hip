@@ -13,6 +13,8 @@
 // device exp/sqrt, within a few ulp of the host libm.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "strip_geometry.h"
 #include "strip_kernels.h"
 
@@ -158,13 +160,14 @@ __global__ __launch_bounds__(256) void k_denominator(StripConst sg, StripConst s
 // ---------------------------------------------------------------------------
 // Pass 2: backprojection out[p] += K rho[p] * w_e (w_e = inv_e for MLEM,
 // rho == nullptr gives out[p] += K * w_e, the adjoint of the projector).
+// The sums are FP64: a pixel can take millions of adds at 1e8 events.
 
 __global__ __launch_bounds__(256) void k_backproject(StripConst sg, StripConst sk,
                                                      const EventGeom* __restrict__ geom,
                                                      int64_t n,
                                                      const float* __restrict__ rho,
                                                      const float* __restrict__ w,
-                                                     float* __restrict__ out) {
+                                                     double* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
   if (e >= n) return;
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(256) void k_backproject(StripConst sg, StripConst s
       int p = iy * sg.W + iz;
       float kv = kernel_value(sk, g, dy, dz);
       if (rho) kv *= rho[p];
-      unsafeAtomicAdd(&out[p], kv * we);
+      unsafeAtomicAdd(&out[p], (double)(kv * we));
     }
   }
 }
@@ -309,6 +312,11 @@ static inline unsigned blocks_for(int64_t n, int per_block) {
   return (unsigned)((n + per_block - 1) / per_block);
 }
 
+// A dispatch's grid is counted in work-items in 32 bits, so a launch of one
+// wave per event covers at most 2^24 events (2^30 work-items); larger event
+// sets go in slices, each kernel seeing its slice's pointers.
+constexpr int64_t kWaveSlice = (int64_t)1 << 24;
+
 hipError_t launch_event_setup(hipStream_t st, const StripConst& s,
                               const float* resp, int64_t n, EventGeom* geom) {
   if (n <= 0) return hipSuccess;
@@ -321,9 +329,14 @@ hipError_t launch_footprint(hipStream_t st, const StripConst& s,
                             const EventGeom* geom, int64_t n, int32_t* nbox,
                             int32_t* nell) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_footprint, dim3(blocks_for(n, 4)), dim3(256), 0, st, s,
-                     geom, n, nbox, nell);
-  return hipGetLastError();
+  for (int64_t e0 = 0; e0 < n; e0 += kWaveSlice) {
+    const int64_t m = std::min(kWaveSlice, n - e0);
+    hipLaunchKernelGGL(k_footprint, dim3(blocks_for(m, 4)), dim3(256), 0, st, s,
+                       geom + e0, m, nbox + e0, nell + e0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
@@ -331,19 +344,29 @@ hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
                               int64_t n, const float* rho, const float* weight,
                               float* lambda, float* inv) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_denominator, dim3(blocks_for(n, 4)), dim3(256), 0, st,
-                     sg, sk, geom, n, rho, weight, lambda, inv);
-  return hipGetLastError();
+  for (int64_t e0 = 0; e0 < n; e0 += kWaveSlice) {
+    const int64_t m = std::min(kWaveSlice, n - e0);
+    hipLaunchKernelGGL(k_denominator, dim3(blocks_for(m, 4)), dim3(256), 0, st,
+                       sg, sk, geom + e0, m, rho, weight, lambda + e0, inv + e0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* w,
-                              float* out) {
+                              double* out) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_backproject, dim3(blocks_for(n, 4)), dim3(256), 0, st,
-                     sg, sk, geom, n, rho, w, out);
-  return hipGetLastError();
+  for (int64_t e0 = 0; e0 < n; e0 += kWaveSlice) {
+    const int64_t m = std::min(kWaveSlice, n - e0);
+    hipLaunchKernelGGL(k_backproject, dim3(blocks_for(m, 4)), dim3(256), 0, st,
+                       sg, sk, geom + e0, m, rho, w + e0, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
@@ -351,9 +374,14 @@ hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
                              int64_t n, const float* rho, const float* w,
                              double* out2) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sigma_grad, dim3(blocks_for(n, 4)), dim3(256), 0, st,
-                     sg, sk, geom, n, rho, w, out2);
-  return hipGetLastError();
+  for (int64_t e0 = 0; e0 < n; e0 += kWaveSlice) {
+    const int64_t m = std::min(kWaveSlice, n - e0);
+    hipLaunchKernelGGL(k_sigma_grad, dim3(blocks_for(m, 4)), dim3(256), 0, st,
+                       sg, sk, geom + e0, m, rho, w + e0, out2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_fill(hipStream_t st, float* p, float v, int64_t n) {

@@ -100,6 +100,10 @@ def load_library(path=LIB_PATH):
         "petstrip_set_sigma": ([vp, dp, dp, C.c_int], C.c_int),
         "petstrip_phantom_events": ([C.POINTER(CScanner), fp, i32, C.c_uint64,
                                      i64, fp, C.c_int], C.c_int),
+        "petstrip_phantom_events_device": ([C.POINTER(CScanner), fp, i32, C.c_uint64,
+                                            i64, vp, C.c_int], C.c_int),
+        "petstrip_set_phantom_events": ([vp, fp, i32, C.c_uint64, i64], C.c_int),
+        "petstrip_get_events": ([vp, fp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -234,6 +238,20 @@ class Reconstruction:
         r = np.ascontiguousarray(responses, dtype=np.float32).reshape(-1, 3)
         _check(self.lib.petstrip_set_events(self._h, _ptr(r), C.c_int64(r.shape[0])))
         self.n_events = r.shape[0]
+
+    def set_phantom_events(self, regions, n, seed=1):
+        """Generates n seeded phantom events on the device (the generator of
+        phantom_events, one lane per slot) and prepares them as set_events."""
+        reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1, 6))
+        _check(self.lib.petstrip_set_phantom_events(self._h, _ptr(reg), reg.shape[0],
+                                                    C.c_uint64(seed), C.c_int64(int(n))))
+        self.n_events = int(n)
+
+    def events(self):
+        """The responses on the device, n x 3 float32."""
+        out = np.empty((self.n_events, 3), dtype=np.float32)
+        _check(self.lib.petstrip_get_events(self._h, _ptr(out)))
+        return out
 
     def set_events_device(self, ptr, n):
         _check(self.lib.petstrip_set_events_device(self._h, C.c_void_p(ptr),
