@@ -15,6 +15,9 @@ mean launch time measured with HIP events on the library's stream.
 cpu_baseline: the reference CPU algorithm (the reference's own sources
 compiled by oracle/Makefile when present, else the oracle's restatement) on a
 bounded prefix of the same events, with 16 threads.
+parity: the GPU image over the same prefix against that CPU image, and over a
+shorter prefix against FP64 runs of the same algorithm (relative L2 and the
+largest per-pixel relative error, per iteration).
 """
 import argparse
 import json
@@ -39,11 +42,16 @@ def parse():
     p.add_argument("--workload", default="strip-512px-1mm-2e7ev")
     p.add_argument("--accumulation", default="fp32",
                    choices=["fp32", "compensated", "fixed64"])
-    p.add_argument("--cpu-sample", type=float, default=6e6,
-                   help="events in the CPU baseline sample")
+    p.add_argument("--cpu-sample", type=float, default=None,
+                   help="events in the CPU baseline sample (default: about "
+                        "3.4e10 box pixels of work, 6e6 events at 512 px / 1 mm)")
     p.add_argument("--cpu-threads", type=int, default=16)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-grad", action="store_true")
+    p.add_argument("--calibrate", type=int, default=0, metavar="ROUNDS",
+                   help="also run ROUNDS rounds of sigma calibration (5 MLEM "
+                        "iterations + one gradient step on (sigma_z, sigma_dl) "
+                        "each) from (0.015, 0.06) on the same events")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="also run one device copy of MIB MiB (for HBM counter "
                         "calibration under rocprofv3 --pmc)")
@@ -64,8 +72,21 @@ def measured_traffic(workload):
     return None if entry is None else entry.get("bytes_per_launch")
 
 
-def cpu_baseline(sc, ev, n_sample, threads):
-    """Reference CPU path on a prefix of the events, one iteration."""
+def _errors(img, ref):
+    """Relative L2 error and largest per-pixel relative error (over pixels
+    at least 1e-3 of the reference maximum) of img against ref."""
+    import numpy as np
+    a = np.asarray(img, np.float64).ravel()
+    b = np.asarray(ref, np.float64).ravel()
+    rel_l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    m = b >= 1e-3 * b.max()
+    return rel_l2, float(np.max(np.abs(a[m] - b[m]) / b[m]))
+
+
+def cpu_baseline(sc, ev, n_sample, threads, gpu_iterations, box_px):
+    """Reference CPU path on a prefix of the events, one iteration, timed;
+    then, as the checker, the GPU's images on the same prefixes against that
+    CPU image and against FP64 runs of the same algorithm."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
 
@@ -75,13 +96,51 @@ def cpu_baseline(sc, ev, n_sample, threads):
     fn = ol.ref_mlem if kind == "reference" else ol.mlem_f32
     fn(sc, sample[:min(len(sample), 20000)], 1, n_threads=threads)  # warm
     t0 = time.perf_counter()
-    fn(sc, sample, 1, n_threads=threads)
+    cpu_img = fn(sc, sample, 1, n_threads=threads)[0]
     dt = time.perf_counter() - t0
-    return {"value": len(sample) / dt, "unit": "events/s/iter", "cores": threads,
+    base = {"value": len(sample) / dt, "unit": "events/s/iter", "cores": threads,
             "kind": kind,
             "sample": f"first {len(sample)} events of the same event list, "
                       f"1 iteration from rho=1, {threads} OpenMP threads, "
                       f"{dt:.2f} s"}
+    rel_cpu, max_cpu = _errors(gpu_iterations(sample, 1)[0], cpu_img)
+    # FP64 runs: a prefix of about 3e9 box pixels, 2 iterations
+    n64 = int(min(5e5, 3e9 / max(box_px, 1.0), len(sample)))
+    it64 = ol.mlem_f64(sc, sample[:n64], 2, n_threads=threads)
+    itg = gpu_iterations(sample[:n64], 2)
+    errs = [_errors(g, r) for g, r in zip(itg, it64)]
+    parity = {
+        "rel_l2_vs_cpu": rel_cpu, "max_pixel_rel_vs_cpu": max_cpu,
+        "cpu_sample": f"1 iteration over the first {len(sample)} events, against "
+                      f"the {kind} CPU image",
+        "rel_l2_vs_fp64": [e[0] for e in errs],
+        "max_pixel_rel_vs_fp64": [e[1] for e in errs],
+        "fp64_sample": f"iterations 1-2 over the first {n64} events; max-pixel error "
+                       "over pixels >= 1e-3 of the FP64 maximum",
+    }
+    return base, parity
+
+
+def calibration_parity(pkg, name, ev, state, threads, box_px):
+    """The checker for the calibrated state: one iteration from the final
+    image at the recovered sigmas, GPU against FP64, on a prefix of the
+    events."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    sz, sdl, rho = state
+    sc = pkg.workload_scanner(name, sigma_z=sz, sigma_dl=sdl)
+    n = int(min(5e5, 1.5e9 / max(box_px, 1.0), len(ev)))
+    sample = ev[:n]
+    ref = ol.mlem_f64(sc, sample, 1, n_threads=threads, rho0=rho)[0]
+    with pkg.Reconstruction(sc, device=0) as r:
+        r.set_events(sample)
+        r.rho = rho
+        r(1)
+        got = r.rho.copy()
+    l2, mx = _errors(got, ref)
+    return {"rel_l2_vs_fp64": l2, "max_pixel_rel_vs_fp64": mx,
+            "sample": f"one iteration from the final image at the recovered sigmas "
+                      f"over the first {n} events"}
 
 
 def main():
@@ -89,7 +148,7 @@ def main():
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
               "supported (the path would shard events across GPUs and all-reduce "
-              "the image, see DESIGN.md)", file=sys.stderr)
+              "the image)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -173,6 +232,28 @@ def main():
             times.append((time.perf_counter() - g0) * 1e3)
         out["grad_ms"] = float(np.median(times))
     rec.close()
+    cal_state = None
+    if args.calibrate > 0:
+        with pkg.Reconstruction(sc, device=0, accumulation=acc) as cal:
+            cal.set_events(ev)
+            res = pkg.calibrate_sigma(
+                cal, (0.015, 0.06), rounds=args.calibrate,
+                log=lambda h: print(f"calibration round {h['round']}: sigma "
+                                    f"({h['sigma_z']:.6f}, {h['sigma_dl']:.6f}) "
+                                    f"loglik {h['loglik']:.6e}", file=sys.stderr,
+                                    flush=True))
+            # the image before one more iteration at the recovered sigmas, for
+            # the checker below
+            cal_state = (res["sigma_z"], res["sigma_dl"], cal.rho.copy())
+        out["calibration"] = {
+            "start": [0.015, 0.06], "simulated": [sc.sigma_z, sc.sigma_dl],
+            "recovered": [res["sigma_z"], res["sigma_dl"]],
+            "rounds": args.calibrate, "mlem_iterations": res["mlem_iterations"],
+            "seconds": res["seconds"],
+            "mlem_events_per_s_per_iter": n_ev * res["mlem_iterations"] / res["seconds_mlem"],
+            "gradient_s_per_round": res["seconds_gradient"] / args.calibrate,
+            "trajectory": [[h["sigma_z"], h["sigma_dl"]] for h in res["history"]],
+        }
     if args.calibrate_copy > 0:
         n = args.calibrate_copy * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -183,7 +264,23 @@ def main():
         out["calibration_copy_bytes"] = 2 * n * 4
         del a, b
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, ev, args.cpu_sample, args.cpu_threads)
+        def gpu_iterations(events, n_iter):
+            with pkg.Reconstruction(sc, device=0, accumulation=acc) as r:
+                r.set_events(events)
+                imgs = []
+                for _ in range(n_iter):
+                    r(1)
+                    imgs.append(r.rho.copy())
+            return imgs
+
+        box_px = st["n_box_pixels"] / max(n_ev, 1)
+        n_cpu = args.cpu_sample if args.cpu_sample else 3.4e10 / max(box_px, 1.0)
+        out["cpu_baseline"], out["parity"] = cpu_baseline(
+            sc, ev, n_cpu, args.cpu_threads, gpu_iterations, box_px)
+        if cal_state is not None:
+            out["calibration"]["parity"] = calibration_parity(
+                pkg, name, ev, cal_state, args.cpu_threads,
+                st["n_box_pixels"] / max(n_ev, 1))
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 

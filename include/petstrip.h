@@ -96,7 +96,9 @@ typedef struct petstrip_options {
                            its quadratic; results are identical, slower     */
   int32_t force_bands;  /* verification switch: 1 = run the fused passes in
                            64-row bands (the form used when a window does
-                           not fit LDS) even when windows fit              */
+                           not fit LDS) even when windows fit; 2 = in bands
+                           of 64 rows x 128 columns (the form used when a
+                           row band does not fit either)                   */
   int32_t reserved[3];  /* zero                                            */
 } petstrip_options;
 
@@ -112,7 +114,8 @@ typedef struct petstrip_stats {
   double last_iteration_ms;  /* device time of the last iteration        */
   double last_kernel_ms;     /* device time of its event kernel(s)       */
   int32_t fused;             /* 1: iterations use the binned kernel, 2: in
-                                64-row bands, 0: the per-event kernels */
+                                64-row bands, 3: in bands of 64 rows x 128
+                                columns, 0: the per-event kernels       */
   int32_t n_work_items;      /* workgroups of the binned kernel          */
   int64_t lds_bytes;         /* LDS per workgroup of the binned kernel   */
 } petstrip_stats;

@@ -9,3 +9,4 @@ from .petstrip import (  # noqa: F401
     device_count, load_library, phantom_events,
 )
 from .workloads import WORKLOADS, workload_scanner, workload_events  # noqa: F401
+from .calibration import calibrate_sigma  # noqa: F401

@@ -159,6 +159,8 @@ struct petstrip_handle {
   // banded form (windows taller than LDS allows): per-band partial sums
   bool banded = false;
   int nbands = 1;
+  int ncb = 1;      // column-band slots per event (1: row bands only)
+  int cshift = 30;  // log2 of the column-band width (30: no column bands)
   int64_t n_sorted = 0;    // events with a non-empty box (the binned ones)
   float* d_part = nullptr;
   float* d_lam_sorted = nullptr;
@@ -353,26 +355,44 @@ int prepare_fused(petstrip_handle* h) {
   h->lds_bytes = h->col_stride
                      ? petstrip::fused_lds_bytes(max_ww, h->col_stride, petstrip::MODE_MLEM)
                      : (size_t)-1;
+  h->ncb = 1;
+  h->cshift = 30;
   if ((h->lds_bytes > h->lds_limit || h->opts.force_bands) && !items.empty()) {
-    // banded form: each chunk's window cut into bands of kBandRows image rows
+    // banded form: each chunk's window cut into bands of kBandRows image
+    // rows, and into bands of 2^kColBandShift columns when a row band of the
+    // widest window still exceeds LDS
     const int br = petstrip::kBandRows;
+    const bool cols =
+        petstrip::fused_lds_bytes(max_ww, br, petstrip::MODE_MLEM) > h->lds_limit ||
+        h->opts.force_bands == 2;
+    const int cshift = cols ? petstrip::kColBandShift : 30;
+    const int cb = 1 << cshift;
     std::vector<petstrip::WorkItem> bands;
     for (const auto& it : items) {
       if (it.wh <= 0 || it.ww <= 0) continue;
       for (int b = it.wy0 / br; b <= (it.wy0 + it.wh - 1) / br; ++b) {
-        petstrip::WorkItem bi = it;
-        bi.wy0 = std::max(it.wy0, b * br);
-        bi.wh = std::min(it.wy0 + it.wh, (b + 1) * br) - bi.wy0;
-        bi.pad = b;
-        bands.push_back(bi);
+        for (int c = it.wx0 >> cshift; c <= (it.wx0 + it.ww - 1) >> cshift; ++c) {
+          petstrip::WorkItem bi = it;
+          bi.wy0 = std::max(it.wy0, b * br);
+          bi.wh = std::min(it.wy0 + it.wh, (b + 1) * br) - bi.wy0;
+          bi.wx0 = (int32_t)std::max<int64_t>(it.wx0, (int64_t)c * cb);
+          bi.ww = (int32_t)(std::min<int64_t>(it.wx0 + it.ww, (int64_t)(c + 1) * cb) - bi.wx0);
+          bi.pad = b;
+          bands.push_back(bi);
+        }
       }
     }
-    const size_t lds = petstrip::fused_lds_bytes(max_ww, br, petstrip::MODE_MLEM);
+    const int band_ww = cols ? std::min(max_ww, cb) : max_ww;
+    const size_t lds = petstrip::fused_lds_bytes(band_ww, br, petstrip::MODE_MLEM);
     if (lds <= h->lds_limit) {
       h->banded = true;
       h->col_stride = br;
       h->lds_bytes = lds;
-      h->nbands = (max_rows + br - 1) / br + 1;
+      h->max_ww = band_ww;
+      // slots per event: every row band and column band its box can reach
+      h->ncb = cols ? (max_ww + cb - 1) / cb + 1 : 1;
+      h->cshift = cshift;
+      h->nbands = ((max_rows + br - 1) / br + 1) * h->ncb;
       const size_t need = (size_t)std::max<int64_t>(acc, 1) * h->nbands * 4;
       if (need > h->cap_part) {
         dfree(h->d_part);
@@ -465,6 +485,8 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.mtab = h->d_mtab;
   a.part = h->d_part;
   a.nbands = h->nbands;
+  a.ncb = h->ncb;
+  a.cshift = h->cshift;
   a.lam_sorted = h->d_lam_sorted;
   return a;
 }
@@ -562,7 +584,7 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   if (opts && (opts->accumulation < 0 || opts->accumulation > 2 ||
                opts->update < 0 || opts->update > 1 || opts->path < 0 ||
                opts->path > 1 || opts->row_scan < 0 || opts->row_scan > 1 ||
-               opts->force_bands < 0 || opts->force_bands > 1))
+               opts->force_bands < 0 || opts->force_bands > 2))
     return fail(PETSTRIP_EINVAL, "invalid options");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -671,7 +693,7 @@ int petstrip_set_options(petstrip_handle* h, const petstrip_options* opts) {
   if (!h || !opts) return fail(PETSTRIP_EINVAL, "bad argument");
   if (opts->accumulation < 0 || opts->accumulation > 2 || opts->update < 0 ||
       opts->update > 1 || opts->path < 0 || opts->path > 1 || opts->row_scan < 0 ||
-      opts->row_scan > 1 || opts->force_bands < 0 || opts->force_bands > 1)
+      opts->row_scan > 1 || opts->force_bands < 0 || opts->force_bands > 2)
     return fail(PETSTRIP_EINVAL, "invalid options");
   h->opts = *opts;
   h->lambda_valid = false;
@@ -863,7 +885,7 @@ int petstrip_sensitivity(petstrip_handle* h, int mode, float* out) {
 int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
   if (!h || !out) return fail(PETSTRIP_EINVAL, "bad argument");
   *out = h->stats;
-  out->fused = h->fused_ok ? (h->banded ? 2 : 1) : 0;
+  out->fused = h->fused_ok ? (h->banded ? (h->ncb > 1 ? 3 : 2) : 1) : 0;
   out->n_work_items = h->n_items;
   out->lds_bytes = (int64_t)h->lds_bytes;
   return PETSTRIP_OK;

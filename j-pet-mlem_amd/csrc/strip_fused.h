@@ -15,7 +15,7 @@ struct WorkItem {
   int32_t wx0, wy0;    // window origin (column, row)
   int32_t ww, wh;      // window size
   int32_t max_rows;    // largest box height in the chunk
-  int32_t pad;         // banded items: the band index (rows / kBandRows)
+  int32_t pad;         // banded items: the row band index (rows / kBandRows)
 };
 
 enum { GLOBAL_F32 = 0, GLOBAL_F64 = 1, GLOBAL_U64 = 2 };
@@ -42,7 +42,9 @@ struct FusedArgs {
   const float* mtab;         // (k, k^2) pairs for the same k
   float* part;               // banded: per sorted event and band, (lambda,
                              // sigma_z sum, sigma_dl sum, largest row factor)
-  int nbands;                // banded: band slots per event
+  int nbands;                // banded: band slots per event (row bands x ncb)
+  int ncb;                   // banded: column-band slots per event (1: none)
+  int cshift;                // banded: log2 of the column-band width (30: none)
   float* lam_sorted;         // banded: lambda per sorted event
   double* scalars;           // FORWARD: {sum log lambda, dl/dsz, dl/dsdl,
                              //           max_e |g_e| K_e, sum_e |g_e| K_e}
@@ -60,6 +62,8 @@ hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
 
 constexpr int kWaves = 16;           // waves per workgroup of the fused kernel
 constexpr int kBandRows = 64;        // rows per band of the banded form
+constexpr int kColBandShift = 7;     // columns per column band: 128 (windows
+                                     // too wide for LDS even in row bands)
 constexpr int kMaxColStride = 256;   // largest LDS column stride (window height)
 
 // n_box and n_ell per event from the row intervals of the fused walks

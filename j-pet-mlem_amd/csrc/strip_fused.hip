@@ -165,7 +165,7 @@ template <bool SIGMA>
 __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
                                             const EventGeom& g, const EventRows& ev,
                                             int iy, int row_w, int cstride, float s_pix,
-                                            bool scan) {
+                                            bool scan, int cx0, int cx1) {
   RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
   const float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
   const float Ady2 = (dy * g.A) * dy;
@@ -208,8 +208,9 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   } else if (none) {
     return s;
   }
-  a = max(a, g.x0);
-  b = min(b, g.x1 - 1);
+  // the box, and the work item's columns [cx0, cx1) in the column-banded form
+  a = max(a, max(g.x0, cx0));
+  b = min(b, min(g.x1, cx1) - 1);
   if (a > b) return s;
   s.len = b - a + 1;
   s.w0 = row_w + a * cstride;
@@ -502,8 +503,9 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
 // is free of bank conflicts.
 //
 // BAND = 1, 2: the banded form for windows too tall for LDS.  Work items then
-// cover one band of kBandRows image rows, and an event is walked in every
-// band its box reaches.  BAND = 1 runs pass 1 and stores the band's partial
+// cover one band of kBandRows image rows (and, for windows too wide as well,
+// one band of 2^kColBandShift columns), and an event is walked in every band
+// its box reaches, its rows clipped to the band's columns.  BAND = 1 runs pass 1 and stores the band's partial
 // (lambda, sigma_z and sigma_dl sums, largest row factor) per event and
 // band; k_band_combine adds them in a fixed order; BAND = 2 runs pass 2
 // with the complete lambda.
@@ -565,11 +567,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   for (int e = it.begin + wave; e < it.end; e += kWaves) {
     const EventGeom g = a.sgeom[e];
     // box rows walked here: all of them, or those in this item's band
-    int ry0 = g.y0, ry1 = g.y1;
+    // and columns: the box's, or those in this item's column band
+    int ry0 = g.y0, ry1 = g.y1, cx0 = g.x0, cx1 = g.x1;
     if constexpr (BAND != 0) {
       ry0 = max(g.y0, it.wy0);
       ry1 = min(g.y1, it.wy0 + wh);
-      if (ry1 <= ry0 || g.x1 <= g.x0) continue;
+      cx0 = max(g.x0, it.wx0);
+      cx1 = min(g.x1, it.wx0 + ww);
+      if (ry1 <= ry0 || cx1 <= cx0) continue;
     }
     const int nrows = ry1 - ry0;
     const int ngroups = (nrows + 63) >> 6;
@@ -595,7 +600,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
           if (gg * 64 + lane < nrows)
-            s = row_setup<kSigma>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan);
+            s = row_setup<kSigma>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan, cx0,
+                                  cx1);
           if (gi < kCachedGroups) cache[gi] = s;
           float s0 = 0.0f, s1 = 0.0f;
           forward_rows<HP, COMP, kSigma>(s, du, gtab, mtab, rho_w, s0, comp, s1);
@@ -621,7 +627,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         const float wsdl = kSigma ? wave_sum(sdl) : 0.0f;
         const float lmx = kSigma ? wave_max_f(lmax) : 0.0f;
         if (lane == 0) {
-          float* p = a.part + ((size_t)e * a.nbands + (it.pad - g.y0 / kBandRows)) * 4;
+          const int slot = (it.pad - g.y0 / kBandRows) * a.ncb +
+                           ((it.wx0 >> a.cshift) - (g.x0 >> a.cshift));
+          float* p = a.part + ((size_t)e * a.nbands + slot) * 4;
           p[0] = lam;
           p[1] = wsz;
           p[2] = wsdl;
@@ -682,7 +690,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           } else {
             s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
             if (gg * 64 + lane < nrows)
-              s = row_setup<false>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan);
+              s = row_setup<false>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan, cx0,
+                                   cx1);
           }
           if constexpr (MODE == MODE_MLEM) {
             // K rho 2^bits / lambda <= 2^bits, but the chain's K 2^bits /
@@ -797,7 +806,7 @@ __global__ __launch_bounds__(256) void k_row_counts(StripConst sg, const EventGe
   int cnt = 0;
   if (ncols > 0) {
     for (int r = lane; r < nrows; r += 64)
-      cnt += row_setup<false>(sg, sg, g, ev, g.y0 + r, 0, 0, 1.0f, scan != 0).len;
+      cnt += row_setup<false>(sg, sg, g, ev, g.y0 + r, 0, 0, 1.0f, scan != 0, g.x0, g.x1).len;
   }
   for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
   if (lane == 0) {

@@ -191,7 +191,7 @@ class Reconstruction:
         self.scanner = scanner
         self._h = C.c_void_p()
         opts = COptions(accumulation, update, path, 1 if row_scan else 0,
-                        1 if force_bands else 0)
+                        int(force_bands))
         sc = scanner.c()
         _check(self.lib.petstrip_create(C.byref(sc), device, C.byref(opts),
                                         C.byref(self._h)))
@@ -231,7 +231,7 @@ class Reconstruction:
     def set_options(self, accumulation=ACCUM_FP32, update=UPDATE_CPU, path=PATH_AUTO,
                     row_scan=False, force_bands=False):
         o = COptions(accumulation, update, path, 1 if row_scan else 0,
-                     1 if force_bands else 0)
+                     int(force_bands))
         _check(self.lib.petstrip_set_options(self._h, C.byref(o)))
 
     def set_events(self, responses):

@@ -167,3 +167,91 @@ def test_workload_256px_gradient(gpu_pkg):
     assert gsig[0] == pytest.approx(gs_ref[0], rel=1e-3, abs=1e-5 * n / 0.01)
     assert gsig[1] == pytest.approx(gs_ref[1], rel=1e-3, abs=1e-5 * n / 0.04)
     assert ll == pytest.approx(ll_ref, rel=1e-6)
+
+
+class _EventEllipses:
+    """FP64 centres and ellipse coefficients of every event (response.h:22-37,
+    gaussian_kernel.h:108-137), to find the events whose ellipse may hold a
+    pixel."""
+
+    def __init__(self, sc, ev):
+        e = np.asarray(ev, np.float64)
+        R, sz, sdl = sc.radius, sc.sigma_z, sc.sigma_dl
+        t = (e[:, 0] - e[:, 1]) / (2 * R)
+        self.y = -0.5 * e[:, 2] / np.sqrt(1 + t * t)
+        self.z = 0.5 * (e[:, 0] + e[:, 1] + 2 * self.y * t)
+        self.A = 4 * (1 + t * t) / sdl ** 2 + 2 * t * t / sz ** 2
+        self.B = -4 * t / sz ** 2
+        self.C = 2 / sz ** 2
+        self.sc = sc
+
+    def near(self, iy, iz):
+        """A superset (the ellipse test with a 1e-3 margin) of the events
+        whose ellipse holds pixel (iy, iz); the others have a lambda that
+        does not depend on the pixel."""
+        sc = self.sc
+        pw, ph = sc.pixel_width, sc.pixel_height
+        zp = sc.center_z - sc.n_z_pixels * pw / 2 + pw / 2 + iz * pw
+        yp = sc.center_y + sc.n_y_pixels * ph / 2 - ph / 2 - iy * ph
+        dy, dz = yp - self.y, zp - self.z
+        q = self.A * dy * dy + self.B * dy * dz + self.C * dz * dz
+        return np.nonzero(q <= 9 * (1 + 1e-3))[0]
+
+
+def test_workload_256px_fd_64_pixels(gpu_pkg):
+    """256x256 pixels of 2 mm, 5e6 events of the three-ellipse phantom, the
+    image after two MLEM iterations: the GPU image gradient on 64 seeded
+    random pixels against FP64 central differences of the log-likelihood
+    (step 1e-4 of the pixel value; only the events whose box may hold the
+    pixel change, so the difference is taken over them), the GPU sigma
+    gradient against FP64 central differences in sigma (step 1e-4 relative,
+    support frozen), and the MLEM identity (grad + 1) rho = rho' over the
+    whole image."""
+    pkg = gpu_pkg
+    name = "strip-256px-2mm-5e6ev"
+    sc = pkg.workload_scanner(name)
+    ev = pkg.workload_events(name)
+    print(f"case: {name}, {len(ev)} events", flush=True)
+    s0 = (sc.sigma_z, sc.sigma_dl)
+    with pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        rec(2)
+        rho = rec.rho.copy()
+        grad, gsig, ll = rec.backward(want_loglik=True)
+        rec(1)
+        nxt = rec.rho.copy()
+    m = rho > 0
+    assert rel((grad + 1)[m] * rho[m], nxt[m]) < 1e-5
+    rng = np.random.default_rng(4)
+    cand = np.argwhere(rho > 1e-2 * rho.max())
+    pix = cand[rng.choice(len(cand), 64, replace=False)]
+    r64 = rho.astype(np.float64)
+    ells = _EventEllipses(sc, ev)
+    worst = 0.0
+    for iy, iz in pix:
+        sub = np.ascontiguousarray(ev[ells.near(iy, iz)])
+        h = 1e-4 * r64[iy, iz]
+        vals = []
+        for sgn in (+1, -1):
+            rp = r64.copy()
+            rp[iy, iz] += sgn * h
+            vals.append(ol.loglik_f64(sc, sub, rp, want_grad=False, n_threads=16)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        err = abs(grad[iy, iz] - fd) / max(abs(fd), 1.0)
+        worst = max(worst, err)
+        assert err < 1e-4, (iy, iz, grad[iy, iz], fd)
+    print(f"64 pixels: worst |grad - fd| / max(|fd|, 1) = {worst:.2e}", flush=True)
+    fd_sig = []
+    for k in range(2):
+        h = 1e-4 * s0[k]
+        vals = []
+        for sgn in (+1, -1):
+            sk = list(s0)
+            sk[k] += sgn * h
+            vals.append(ol.loglik_f64(sc, ev, rho, sk=tuple(sk), ss=s0, want_grad=False,
+                                      n_threads=16)[0])
+        fd_sig.append((vals[0] - vals[1]) / (2 * h))
+    print(f"sigma grad {gsig} FP64 fd {fd_sig}", flush=True)
+    n = len(ev)
+    assert gsig[0] == pytest.approx(fd_sig[0], rel=1e-3, abs=1e-5 * n / s0[0])
+    assert gsig[1] == pytest.approx(fd_sig[1], rel=1e-3, abs=1e-5 * n / s0[1])

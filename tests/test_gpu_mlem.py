@@ -239,15 +239,18 @@ def test_workload_160px_1e6(gpu_pkg, acc):
         assert e64 < 1e-4
 
 
-def test_banded_matches_unbanded(gpu_pkg):
-    """The banded form (per-band partial denominators, combined in a fixed
-    order) against the one-window form on the same events: MLEM images,
-    denominators and the gradient."""
-    sc, ev = small(gpu_pkg, n=20000)
+@pytest.mark.parametrize("n_pix,size", [(64, 0.008), (256, 0.002)])
+def test_banded_matches_unbanded(gpu_pkg, n_pix, size):
+    """The banded forms (per-band partial denominators, combined in a fixed
+    order), in row bands and in row x column bands, against the one-window
+    form on the same events: MLEM images, denominators and the gradient.
+    At 256 pixels the windows of the bins next to column 128 are cut by the
+    column bands."""
+    sc, ev = small(gpu_pkg, n_pix=n_pix, size=size, n=20000)
     rng = np.random.default_rng(11)
     rho = rng.uniform(0.5, 1.5, (sc.n_y_pixels, sc.n_z_pixels)).astype(np.float32)
     out = {}
-    for fb in (False, True):
+    for fb in (0, 1, 2):
         with gpu_pkg.Reconstruction(sc, force_bands=fb) as rec:
             rec.set_events(ev)
             st = rec.stats()
@@ -257,15 +260,18 @@ def test_banded_matches_unbanded(gpu_pkg):
             rec.rho = rho
             rec(3)
             out[fb] = (st["fused"], lam, grad, gsig, ll, rec.rho)
-    print(f"case: 64x64 @ 8 mm, fused {out[False][0]} vs {out[True][0]}", flush=True)
-    assert out[False][0] == 1 and out[True][0] == 2
-    a, b = out[False], out[True]
-    assert np.allclose(a[1], b[1], rtol=2e-6, atol=0)
-    assert rel_l2(b[2], a[2]) < 1e-6
-    assert b[3][0] == pytest.approx(a[3][0], rel=1e-5)
-    assert b[3][1] == pytest.approx(a[3][1], rel=1e-5)
-    assert b[4] == pytest.approx(a[4], rel=1e-7)
-    assert rel_l2(b[5], a[5]) < 1e-6
+    print(f"case: {n_pix}x{n_pix} @ {size * 1e3:g} mm, fused "
+          f"{[out[f][0] for f in (0, 1, 2)]}", flush=True)
+    assert [out[f][0] for f in (0, 1, 2)] == [1, 2, 3]
+    a = out[0]
+    for fb in (1, 2):
+        b = out[fb]
+        assert np.allclose(a[1], b[1], rtol=2e-6, atol=0)
+        assert rel_l2(b[2], a[2]) < 1e-6
+        assert b[3][0] == pytest.approx(a[3][0], rel=1e-5)
+        assert b[3][1] == pytest.approx(a[3][1], rel=1e-5)
+        assert b[4] == pytest.approx(a[4], rel=1e-7)
+        assert rel_l2(b[5], a[5]) < 1e-6
 
 
 def test_banded_tall_boxes_vs_oracle(gpu_pkg):
@@ -294,6 +300,45 @@ def test_banded_tall_boxes_vs_oracle(gpu_pkg):
     r32 = ol.mlem_f32(sc, ev, 3, n_threads=16)
     r64 = ol.mlem_f64(sc, ev, 3, n_threads=16)
     for i in range(3):
+        e32, e64 = rel_l2(imgs[i], r32[i]), rel_l2(imgs[i], r64[i])
+        print(f"iter {i + 1}: rel L2 vs f32 {e32:.2e} vs f64 {e64:.2e}", flush=True)
+        assert e32 < 1e-5 and e64 < 1e-4
+    _, _, g_ref, gs_ref = ol.loglik_f64(sc, ev, rho, n_threads=16)
+    m = rho > 1e-3 * rho.max()
+    assert rel_l2(grad[m], g_ref[m]) < 1e-5
+    assert gsig[0] == pytest.approx(gs_ref[0], rel=1e-3)
+    assert gsig[1] == pytest.approx(gs_ref[1], rel=1e-3)
+
+
+def test_column_bands_wide_boxes_vs_oracle(gpu_pkg):
+    """0.5 mm pixels at 1.5x the detector widths (sigma_z = 0.015 m,
+    sigma_dl = 0.06 m, the start of a width calibration): boxes up to ~240
+    columns wide, so even 64-row bands of the windows exceed LDS and the
+    fused passes run in bands of 64 rows x 128 columns.  Footprints exact,
+    2 MLEM iterations against the FP32 and FP64 oracles, the log-likelihood
+    gradient against FP64."""
+    pkg = gpu_pkg
+    sc = pkg.Scanner(0.43, 0.5, 320, 320, 0.0005, 0.0005, 0.015, 0.06)
+    phantom = [(0.0, 0.0, 0.05, 0.04, 20.0, 1.0), (0.02, 0.01, 0.015, 0.015, 0.0, 3.0)]
+    ev = pkg.phantom_events(sc, phantom, 3000, seed=23)
+    with pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        st = rec.stats()
+        nb, ne = rec.footprints()
+        imgs = []
+        for _ in range(2):
+            rec(1)
+            imgs.append(rec.rho)
+        rho = imgs[-1]
+        grad, gsig, _ = rec.backward()
+    print(f"case: 320x320 @ 0.5 mm, sigma 1.5x, 3000 events, fused {st['fused']}, "
+          f"lds {st['lds_bytes']}, items {st['n_work_items']}", flush=True)
+    assert st["fused"] == 3
+    onb, one = ol.footprint(sc, ev, n_threads=16)
+    assert np.array_equal(nb, onb) and np.array_equal(ne, one)
+    r32 = ol.mlem_f32(sc, ev, 2, n_threads=16)
+    r64 = ol.mlem_f64(sc, ev, 2, n_threads=16)
+    for i in range(2):
         e32, e64 = rel_l2(imgs[i], r32[i]), rel_l2(imgs[i], r64[i])
         print(f"iter {i + 1}: rel L2 vs f32 {e32:.2e} vs f64 {e64:.2e}", flush=True)
         assert e32 < 1e-5 and e64 < 1e-4
