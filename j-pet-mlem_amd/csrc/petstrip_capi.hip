@@ -168,7 +168,7 @@ struct petstrip_handle {
   int* d_flag = nullptr;  // device overflow flag of the fixed-point image
   // G(k) = 2^(-(k du)^2) of the fused walks, for the current sk and pw
   float* d_gtab = nullptr;
-  float* d_mtab = nullptr;  // (k, k^2) pairs
+  float* d_mtab = nullptr;  // (k G(k), k^2 G(k)) pairs
   size_t gtab_n = 0;
   float s_pix = 0.0f, du = -1.0f;
   bool fused_ok = false;
@@ -247,18 +247,21 @@ bool rows_are_intervals(const StripConst& s) {
 }
 
 // The per-column tables of the fused walks (see strip_fused.hip): G(k) =
-// 2^(-(k du)^2), du = pw sqrt(isz log2 e) with the kernel's sigma_z, rounded
-// once from double, and the (k, k^2) pairs of the sigma moments; rebuilt
-// when sigma_z or the width change.
+// 2^(-(k du)^2), du = pw sqrt(isz log2 e) with the kernel's sigma_z, and the
+// (k G(k), k^2 G(k)) pairs of the sigma moments, each rounded once from
+// double; rebuilt when sigma_z or the width change.
 int update_kernel_table(petstrip_handle* h) {
   const float s_pix = sqrtf(h->sk.isz * 1.44269504088896341f);
   const float du = h->sg.pw * s_pix;
   const size_t n = (size_t)h->sg.W + 8;
   if (h->d_gtab && du == h->du && n == h->gtab_n) return PETSTRIP_OK;
-  std::vector<float> t(n);
+  std::vector<float> t(n), m(2 * n);
   for (size_t k = 0; k < n; ++k) {
     const double x = (double)k * (double)du;
-    t[k] = (float)std::exp2(-x * x);
+    const double gk = std::exp2(-x * x);
+    t[k] = (float)gk;
+    m[2 * k] = (float)((double)k * gk);
+    m[2 * k + 1] = (float)((double)k * (double)k * gk);
   }
   if (n != h->gtab_n) {
     dfree(h->d_gtab);
@@ -267,16 +270,11 @@ int update_kernel_table(petstrip_handle* h) {
     int rc;
     if ((rc = dalloc(&h->d_gtab, n))) return rc;
     if ((rc = dalloc(&h->d_mtab, 2 * n))) return rc;
-    std::vector<float> m(2 * n);
-    for (size_t k = 0; k < n; ++k) {
-      m[2 * k] = (float)k;
-      m[2 * k + 1] = (float)((double)k * (double)k);
-    }
-    HIP_TRY(hipMemcpy(h->d_mtab, m.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice));
     h->gtab_n = n;
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
   HIP_TRY(hipMemcpy(h->d_gtab, t.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_mtab, m.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice));
   h->s_pix = s_pix;
   h->du = du;
   return PETSTRIP_OK;

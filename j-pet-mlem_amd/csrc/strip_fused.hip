@@ -296,8 +296,10 @@ __device__ __forceinline__ void read4(const float* rp, float (&rv)[kUnroll]) {
 // to zero, which drops those pixels from every sum without a mask.
 // s0 += sum K rho (two_sum error into comp when COMP); SIGMA: s1 += sum K
 // rho u^2, from the moments M1 = sum k K rho and M2 = sum k^2 K rho of the
-// pixel index k (k and k^2 from a table through the scalar cache for the
-// full groups), combined per row as u0^2 S + 2 u0 du M1 + du^2 M2.
+// pixel index k, combined per row as u0^2 S + 2 u0 du M1 + du^2 M2.  In the
+// full groups the chain value times rho is weighted by G(k), k G(k) and
+// k^2 G(k), all three from tables through the scalar cache: three FMAs per
+// pixel give S, M1 and M2.
 template <int HP, bool COMP, bool SIGMA>
 __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* gtab,
                                              ConstF* mtab, const float* rho_w, float& s0,
@@ -331,16 +333,15 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* 
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int k = gk * kUnroll + u;
-      const float kv = c.p[u] * gtab[k];
-      c.p[u] *= b4;
       if constexpr (SIGMA) {
-        const float kr = kv * rv[u];
-        acc += kr;
-        m1 = fmaf(kr, mtab[2 * k], m1);
-        m2 = fmaf(kr, mtab[2 * k + 1], m2);
+        const float pr = c.p[u] * rv[u];
+        acc = fmaf(pr, gtab[k], acc);
+        m1 = fmaf(pr, mtab[2 * k], m1);
+        m2 = fmaf(pr, mtab[2 * k + 1], m2);
       } else {
-        acc = fmaf(kv, rv[u], acc);
+        acc = fmaf(c.p[u] * gtab[k], rv[u], acc);
       }
+      c.p[u] *= b4;
     }
     if constexpr (COMP) add(t0);
   }
@@ -374,12 +375,19 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* 
 }
 
 // x -> the integer round(x) as (lo, hi); SPLIT: kv = x 2^-32 < 2^30 and the
-// halves are converted apart, else x < 2^32.
-template <bool SPLIT>
+// halves are converted apart; RPI: x < 2^31 is rounded by the conversion
+// itself (v_cvt_rpi_i32_f32, floor(x + 0.5)); else x + 0.5 < 2^32 arrives
+// with its 0.5 added and is truncated.
+template <bool SPLIT, bool RPI = false>
 __device__ __forceinline__ void to_fixed(float x_or_kv, unsigned& lo, unsigned& hi) {
   if constexpr (SPLIT) {
     hi = (unsigned)x_or_kv;
     lo = (unsigned)fmaf(__builtin_amdgcn_fractf(x_or_kv), 4294967296.0f, 0.5f);
+  } else if constexpr (RPI) {
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x_or_kv));
+    lo = (unsigned)r;
+    hi = 0u;
   } else {
     lo = (unsigned)x_or_kv;  // x in [0.5, 2^32)
     hi = 0u;
@@ -392,9 +400,9 @@ __device__ __forceinline__ void to_fixed(float x_or_kv, unsigned& lo, unsigned& 
 // One pass-2 value from the kernel value kv (which carries the event
 // multiplier): K w rho or K w, + 0.5 for rounding unless SPLIT (whose
 // conversion rounds the low half itself).
-template <bool RHO, bool SPLIT>
+template <bool RHO, bool SPLIT, bool RPI = false>
 __device__ __forceinline__ float fixed_arg(float kv, float rho) {
-  if constexpr (SPLIT) return RHO ? kv * rho : kv;
+  if constexpr (SPLIT || RPI) return RHO ? kv * rho : kv;
   else if constexpr (RHO) return fmaf(kv, rho, 0.5f);
   else return kv + 0.5f;
 }
@@ -424,8 +432,13 @@ __device__ __forceinline__ void set_pair(unsigned long long& v, unsigned lo, uns
 // multiplier rides in the kernel values (chain and direct); otherwise (MLEM
 // events whose folded values could overflow a float) x = (K rho / lambda)
 // 2^bits is formed per pixel without the large multiplier.  SPLIT: values
-// reach 2^62 and carry a further 2^-32 until converted.
-template <int HP, bool RHO, bool FOLD, bool SPLIT, bool SUB>
+// reach 2^62 and carry a further 2^-32 until converted.  RPI (with FOLD and
+// without SPLIT): values below 2^31, rounded by the conversion in the full
+// groups, whose values come from a multiplication.  The conversion is inline
+// assembly, and the compiler does not insert the wait state that an
+// inline-assembly reader of a transcendental result needs on gfx950, so the
+// tail (values straight from v_exp_f32) keeps the add-and-truncate form.
+template <int HP, bool RHO, bool FOLD, bool SPLIT, bool SUB, bool RPI = false>
 __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, float lw,
                                                  float lam, float unit_scale, ConstF* gtab,
                                                  const float* rho_w,
@@ -465,12 +478,12 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
     for (int u = 0; u < kUnroll; ++u) {
       float x;
       if constexpr (FOLD) {
-        x = fixed_arg<RHO, SPLIT>(c.p[u] * gtab[gk * kUnroll + u], rv[u]);
+        x = fixed_arg<RHO, SPLIT, RPI>(c.p[u] * gtab[gk * kUnroll + u], rv[u]);
         c.p[u] *= c.b4;
       } else {
         x = direct(kf, u, rv[u]);
       }
-      to_fixed<SPLIT>(x, lo[u], hi[u]);
+      to_fixed<SPLIT, RPI>(x, lo[u], hi[u]);
       set_pair(pv[u], lo[u], hi[u], !SPLIT);
     }
     if (gk < nfull) {
@@ -714,14 +727,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
                                                               gtab, rho_w, acc_w);
             }
           } else if constexpr (MODE == MODE_GRAD) {
-            // K / lambda has no bound: rows whose values stay below 2^31.5
-            // (or 2^52) fixed-point units take the single (split)
+            // K / lambda has no bound: rows whose values stay below 2^30.5
+            // (or 2^52) fixed-point units take the single rounding (split)
             // conversion, the rare rest is added to the global image in
             // doubles
             const float vmax = s.lsc + lw;
-            if (!__any(vmax > 31.5f)) {
-              backproject_rows<HP, false, true, false, false>(s, du, lw, 0.0f, unit_scale,
-                                                              gtab, rho_w, acc_w);
+            if (!__any(vmax > 30.5f)) {
+              backproject_rows<HP, false, true, false, false, true>(
+                  s, du, lw, 0.0f, unit_scale, gtab, rho_w, acc_w);
             } else if (!__any(vmax > 52.0f)) {
               backproject_rows<HP, false, true, true, false>(s, du, lw, 0.0f, unit_scale,
                                                              gtab, rho_w, acc_w);
