@@ -75,10 +75,19 @@ enum {
    * rho'[p] = sum_e K_e(p) rho[p] / sum_q K_e(q) rho[q]
    * (src/2d/strip/reconstruction.h:220-298). */
   PETSTRIP_UPDATE_CPU = 0,
-  /* The reference --gpu variant: denominator weighted by the 5-point pixel
+  /* The reference --gpu update: denominator weighted by the 5-point pixel
    * sensitivity (src/2d/strip/cuda/reconstruction/warp_granularity.cuh:
-   * 101-108,145-150), Gaussian kernel, guarded inverse. */
+   * 101-108,145-150), guarded inverse; with PETSTRIP_KERNEL_ANALYTIC it is
+   * the reference --gpu iteration (cuda/reconstruction.cu:45-52). */
   PETSTRIP_UPDATE_SENSITIVITY_DENOMINATOR = 1
+};
+
+/* Which kernel value is computed inside the (Gaussian) 3-sigma support. */
+enum {
+  PETSTRIP_KERNEL_GAUSSIAN = 0,  /* GaussianKernel (gaussian_kernel.h:51-79)  */
+  /* AnalyticKernel (src/2d/strip/analytic_kernel.h:28-62); runs on the
+   * per-event kernels; no sigma derivative */
+  PETSTRIP_KERNEL_ANALYTIC = 1
 };
 
 /* Which kernels run an iteration. */
@@ -99,7 +108,8 @@ typedef struct petstrip_options {
                            not fit LDS) even when windows fit; 2 = in bands
                            of 64 rows x 128 columns (the form used when a
                            row band does not fit either)                   */
-  int32_t reserved[3];  /* zero                                            */
+  int32_t kernel;       /* PETSTRIP_KERNEL_*                               */
+  int32_t reserved[2];  /* zero                                            */
 } petstrip_options;
 
 typedef struct petstrip_handle petstrip_handle;

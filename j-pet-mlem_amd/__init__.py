@@ -4,7 +4,8 @@ The compute path is libpetstrip_hip.so (hand-written HIP for gfx950 behind
 the C ABI in include/petstrip.h); this package is its Python host side.
 """
 from .petstrip import (  # noqa: F401
-    ACCUM_COMPENSATED, ACCUM_FIXED64, ACCUM_FP32, PATH_AUTO, PATH_PER_EVENT, UPDATE_CPU,
+    ACCUM_COMPENSATED, ACCUM_FIXED64, ACCUM_FP32, KERNEL_ANALYTIC, KERNEL_GAUSSIAN,
+    PATH_AUTO, PATH_PER_EVENT, UPDATE_CPU,
     UPDATE_SENSITIVITY_DENOMINATOR, PetstripError, Reconstruction, Scanner,
     device_count, load_library, phantom_events,
 )

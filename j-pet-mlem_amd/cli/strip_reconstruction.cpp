@@ -68,6 +68,9 @@ const OptSpec kOptions[] = {
     {"rho", 0, true, "", true, "start rho (eg. existing iteration)"},
     {"rho-is-int", 0, false, "", true, "start rho uses integer representation (not float)"},
     {"accumulation", 0, true, "fp32", true, "fp32 | compensated | fixed64"},
+    {"update", 0, true, "cpu", true,
+     "cpu (Gaussian kernel, the CPU update) | gpu (analytic kernel, 5-point "
+     "sensitivity in the denominator: the reference --gpu iteration)"},
     {"help", 'h', false, "", false, "print this message"},
 };
 
@@ -410,6 +413,13 @@ int run(int argc, char** argv) {
   opts.accumulation = accm == "fixed64" ? PETSTRIP_ACCUM_FIXED64
                       : accm == "compensated" ? PETSTRIP_ACCUM_COMPENSATED
                                               : PETSTRIP_ACCUM_FP32;
+  std::string upd = a.get("update");
+  if (upd == "gpu") {
+    opts.update = PETSTRIP_UPDATE_SENSITIVITY_DENOMINATOR;
+    opts.kernel = PETSTRIP_KERNEL_ANALYTIC;
+  } else if (upd != "cpu") {
+    throw std::runtime_error("--update must be cpu or gpu");
+  }
   petstrip_handle* h = nullptr;
   check(petstrip_create(&sc, (int)to_long(a.get("cuda-device")), &opts, &h));
   struct Guard {

@@ -415,6 +415,7 @@ int prepare_fused(petstrip_handle* h) {
   }
   h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
                 h->opts.update == PETSTRIP_UPDATE_CPU &&
+                h->opts.kernel == PETSTRIP_KERNEL_GAUSSIAN &&
                 h->opts.path == PETSTRIP_PATH_AUTO;
   return PETSTRIP_OK;
 }
@@ -545,10 +546,11 @@ int one_iteration(petstrip_handle* h) {
   HIP_TRY(hipEventRecord(h->evk0, h->stream));
   HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
                                        h->d_rho, denominator_weight(h),
-                                       h->d_lambda, h->d_inv));
+                                       h->d_lambda, h->d_inv, h->opts.kernel));
   // the FP64 accumulator is zero between uses (finalize clears it)
   HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n,
-                                       h->d_rho, h->d_inv, (double*)h->d_acc));
+                                       h->d_rho, h->d_inv, (double*)h->d_acc,
+                                       h->opts.kernel));
   HIP_TRY(hipEventRecord(h->evk1, h->stream));
   HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0, 0.0f,
                                     0, nullptr, h->d_out, (int64_t)h->n_pix));
@@ -582,7 +584,8 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   if (opts && (opts->accumulation < 0 || opts->accumulation > 2 ||
                opts->update < 0 || opts->update > 1 || opts->path < 0 ||
                opts->path > 1 || opts->row_scan < 0 || opts->row_scan > 1 ||
-               opts->force_bands < 0 || opts->force_bands > 2))
+               opts->force_bands < 0 || opts->force_bands > 2 || opts->kernel < 0 ||
+               opts->kernel > 1))
     return fail(PETSTRIP_EINVAL, "invalid options");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -691,7 +694,8 @@ int petstrip_set_options(petstrip_handle* h, const petstrip_options* opts) {
   if (!h || !opts) return fail(PETSTRIP_EINVAL, "bad argument");
   if (opts->accumulation < 0 || opts->accumulation > 2 || opts->update < 0 ||
       opts->update > 1 || opts->path < 0 || opts->path > 1 || opts->row_scan < 0 ||
-      opts->row_scan > 1 || opts->force_bands < 0 || opts->force_bands > 2)
+      opts->row_scan > 1 || opts->force_bands < 0 || opts->force_bands > 2 ||
+      opts->kernel < 0 || opts->kernel > 1)
     return fail(PETSTRIP_EINVAL, "invalid options");
   h->opts = *opts;
   h->lambda_valid = false;
@@ -916,7 +920,8 @@ int petstrip_forward(petstrip_handle* h, float* lambda) {
     if ((rc = fused_forward(h, a, false))) return rc;
   } else {
     HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
-                                         h->d_rho, nullptr, h->d_lambda, h->d_inv));
+                                         h->d_rho, nullptr, h->d_lambda, h->d_inv,
+                                         h->opts.kernel));
   }
   if (lambda && n > 0)
     HIP_TRY(hipMemcpyAsync(lambda, h->d_lambda, n * sizeof(float),
@@ -946,7 +951,7 @@ int backward_per_event(petstrip_handle* h, const float* g, float* grad_rho,
   }
   if (grad_rho) {
     HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n, nullptr,
-                                         h->d_inv, (double*)h->d_acc));
+                                         h->d_inv, (double*)h->d_acc, h->opts.kernel));
     // the log-likelihood gradient subtracts the sensitivity term, 1 per pixel
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0,
                                       g ? 0.0f : -1.0f, 0, nullptr, h->d_out,
@@ -1089,6 +1094,8 @@ int petstrip_backward(petstrip_handle* h, const float* g, float* grad_rho,
                       double grad_sigma[2], double* loglik) {
   if (!h) return fail(PETSTRIP_EINVAL, "handle is NULL");
   if (loglik && g) return fail(PETSTRIP_EINVAL, "loglik needs g == NULL");
+  if (grad_sigma && h->opts.kernel == PETSTRIP_KERNEL_ANALYTIC)
+    return fail(PETSTRIP_EINVAL, "no sigma derivative for the analytic kernel");
   int rc;
   if ((rc = ensure_device(h))) return rc;
   double ll_terms = 0.0;

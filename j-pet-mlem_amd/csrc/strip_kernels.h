@@ -15,11 +15,11 @@ hipError_t launch_footprint(hipStream_t st, const StripConst& s,
 hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* weight,
-                              float* lambda, float* inv);
+                              float* lambda, float* inv, int analytic = 0);
 hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* w,
-                              double* out);
+                              double* out, int analytic = 0);
 hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
                              const StripConst& sk, const EventGeom* geom,
                              int64_t n, const float* rho, const float* w,

@@ -69,6 +69,36 @@ __device__ __forceinline__ float kernel_value(const StripConst& k,
   return (k.c / sqrtf(norm)) * expf(bcb * -0.5f);
 }
 
+// AnalyticKernel::operator()(y, tan, sec, R, distance)
+// (src/2d/strip/analytic_kernel.h:28-62) in FP32: with a as for the Gaussian
+// kernel, o = (a_p tan, a_q tan, -yp sec (1 + 2 tan^2)) and b, the kernel is
+// c / sqrt(norm) exp(-(b S b - (b S a)^2 / norm) / 2), norm = a S a + 2 o S b.
+// The support (box and 3-sigma ellipse) stays the Gaussian one, as in the
+// reference.
+__device__ __forceinline__ float analytic_value(const StripConst& k,
+                                                const EventGeom& g, float dy,
+                                                float dz) {
+  const float sec_sq = g.sec * g.sec;
+  const float yp = dy + g.y;
+  const float ap = -(yp - k.R) * sec_sq, aq = -(yp + k.R) * sec_sq;
+  const float ar = -2.0f * yp * g.sec * g.t;
+  const float op = ap * g.t, oq = aq * g.t;
+  const float orr = -yp * g.sec * (1.0f + 2.0f * g.t * g.t);
+  const float pq = dz - dy * g.t;
+  const float br = -2.0f * dy * g.sec;
+  const float aia = ap * k.isz * ap + aq * k.isz * aq + ar * k.isdl * ar;
+  const float bia = pq * k.isz * ap + pq * k.isz * aq + br * k.isdl * ar;
+  const float bib = pq * k.isz * pq + pq * k.isz * pq + br * k.isdl * br;
+  const float oib = op * k.isz * pq + oq * k.isz * pq + orr * k.isdl * br;
+  const float norm = aia + 2.0f * oib;
+  return k.c / sqrtf(norm) * expf(-0.5f * (bib - bia * bia / norm));
+}
+
+__device__ __forceinline__ float event_kernel(const StripConst& k, const EventGeom& g,
+                                              float dy, float dz, int analytic) {
+  return analytic ? analytic_value(k, g, dy, dz) : kernel_value(k, g, dy, dz);
+}
+
 __device__ __forceinline__ EventGeom load_geom(const EventGeom* geom, int64_t e) {
   const float4* p = reinterpret_cast<const float4*>(geom + e);
   float4 a = p[0], b = p[1];
@@ -130,7 +160,8 @@ __global__ __launch_bounds__(256) void k_denominator(StripConst sg, StripConst s
                                                      const float* __restrict__ rho,
                                                      const float* __restrict__ weight,
                                                      float* __restrict__ lambda,
-                                                     float* __restrict__ inv) {
+                                                     float* __restrict__ inv,
+                                                     int analytic) {
   const int lane = threadIdx.x & 63;
   int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
   if (e >= n) return;
@@ -147,7 +178,7 @@ __global__ __launch_bounds__(256) void k_denominator(StripConst sg, StripConst s
       int p = iy * sg.W + iz;
       float r = rho[p];
       if (weight) r *= weight[p];
-      acc += kernel_value(sk, g, dy, dz) * r;
+      acc += event_kernel(sk, g, dy, dz, analytic) * r;
     }
   }
   float d = wave_sum(acc);
@@ -167,7 +198,8 @@ __global__ __launch_bounds__(256) void k_backproject(StripConst sg, StripConst s
                                                      int64_t n,
                                                      const float* __restrict__ rho,
                                                      const float* __restrict__ w,
-                                                     double* __restrict__ out) {
+                                                     double* __restrict__ out,
+                                                     int analytic) {
   const int lane = threadIdx.x & 63;
   int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
   if (e >= n) return;
@@ -183,7 +215,7 @@ __global__ __launch_bounds__(256) void k_backproject(StripConst sg, StripConst s
     float dz = col_dz(sg, g, iz);
     if (in_ellipse(g, dy, dz)) {
       int p = iy * sg.W + iz;
-      float kv = kernel_value(sk, g, dy, dz);
+      float kv = event_kernel(sk, g, dy, dz, analytic);
       if (rho) kv *= rho[p];
       unsafeAtomicAdd(&out[p], (double)(kv * we));
     }
@@ -342,12 +374,12 @@ hipError_t launch_footprint(hipStream_t st, const StripConst& s,
 hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* weight,
-                              float* lambda, float* inv) {
+                              float* lambda, float* inv, int analytic) {
   if (n <= 0) return hipSuccess;
   for (int64_t e0 = 0; e0 < n; e0 += kWaveSlice) {
     const int64_t m = std::min(kWaveSlice, n - e0);
     hipLaunchKernelGGL(k_denominator, dim3(blocks_for(m, 4)), dim3(256), 0, st,
-                       sg, sk, geom + e0, m, rho, weight, lambda + e0, inv + e0);
+                       sg, sk, geom + e0, m, rho, weight, lambda + e0, inv + e0, analytic);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -357,12 +389,12 @@ hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
 hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* w,
-                              double* out) {
+                              double* out, int analytic) {
   if (n <= 0) return hipSuccess;
   for (int64_t e0 = 0; e0 < n; e0 += kWaveSlice) {
     const int64_t m = std::min(kWaveSlice, n - e0);
     hipLaunchKernelGGL(k_backproject, dim3(blocks_for(m, 4)), dim3(256), 0, st,
-                       sg, sk, geom + e0, m, rho, w + e0, out);
+                       sg, sk, geom + e0, m, rho, w + e0, out, analytic);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libpetstrip_hip.so")
 ACCUM_FP32, ACCUM_COMPENSATED, ACCUM_FIXED64 = 0, 1, 2
 UPDATE_CPU, UPDATE_SENSITIVITY_DENOMINATOR = 0, 1
 PATH_AUTO, PATH_PER_EVENT = 0, 1
+KERNEL_GAUSSIAN, KERNEL_ANALYTIC = 0, 1
 
 
 class PetstripError(RuntimeError):
@@ -43,7 +44,8 @@ class CScanner(C.Structure):
 class COptions(C.Structure):
     _fields_ = [("accumulation", C.c_int32), ("update", C.c_int32),
                 ("path", C.c_int32), ("row_scan", C.c_int32),
-                ("force_bands", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("force_bands", C.c_int32), ("kernel", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class CStats(C.Structure):
@@ -186,12 +188,13 @@ class Reconstruction:
     """
 
     def __init__(self, scanner, device=0, accumulation=ACCUM_FP32,
-                 update=UPDATE_CPU, path=PATH_AUTO, row_scan=False, force_bands=False):
+                 update=UPDATE_CPU, path=PATH_AUTO, row_scan=False, force_bands=False,
+                 kernel=KERNEL_GAUSSIAN):
         self.lib = load_library()
         self.scanner = scanner
         self._h = C.c_void_p()
         opts = COptions(accumulation, update, path, 1 if row_scan else 0,
-                        int(force_bands))
+                        int(force_bands), kernel)
         sc = scanner.c()
         _check(self.lib.petstrip_create(C.byref(sc), device, C.byref(opts),
                                         C.byref(self._h)))
@@ -229,9 +232,9 @@ class Reconstruction:
         return s.value
 
     def set_options(self, accumulation=ACCUM_FP32, update=UPDATE_CPU, path=PATH_AUTO,
-                    row_scan=False, force_bands=False):
+                    row_scan=False, force_bands=False, kernel=KERNEL_GAUSSIAN):
         o = COptions(accumulation, update, path, 1 if row_scan else 0,
-                     int(force_bands))
+                     int(force_bands), kernel)
         _check(self.lib.petstrip_set_options(self._h, C.byref(o)))
 
     def set_events(self, responses):

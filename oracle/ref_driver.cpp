@@ -7,6 +7,7 @@
 //   - Strip::GaussianKernel               src/2d/strip/gaussian_kernel.h:12-161
 //   - Strip::Scanner                      src/2d/strip/scanner.h:26-291
 //   - Strip::Response::calculate_tan_y_z  src/2d/strip/response.h:22-37
+//   - Strip::AnalyticKernel               src/2d/strip/analytic_kernel.h:16-87
 // Only tests and the golden-vector script load the built library.
 #include <cstdint>
 #include <cstring>
@@ -14,6 +15,7 @@
 
 #include "2d/strip/reconstruction.h"
 #include "2d/strip/gaussian_kernel.h"
+#include "2d/strip/analytic_kernel.h"
 
 using F = float;
 using S = short;
@@ -95,6 +97,21 @@ double ref_strip_kernel_f64(double sigma_z, double sigma_dl, double y,
                             double tan, double sec, double R, double dz,
                             double dy) {
   PET2D::Strip::GaussianKernel<double> k(sigma_z, sigma_dl);
+  return k(y, tan, sec, R, PET2D::Vector<double>(dz, dy));
+}
+
+// AnalyticKernel::operator()(y, tan, sec, R, distance) in float or double.
+double ref_strip_analytic_kernel_f32(double sigma_z, double sigma_dl, double y,
+                                     double tan, double sec, double R, double dz,
+                                     double dy) {
+  PET2D::Strip::AnalyticKernel<F> k((F)sigma_z, (F)sigma_dl);
+  return k((F)y, (F)tan, (F)sec, (F)R, PET2D::Vector<F>((F)dz, (F)dy));
+}
+
+double ref_strip_analytic_kernel_f64(double sigma_z, double sigma_dl, double y,
+                                     double tan, double sec, double R, double dz,
+                                     double dy) {
+  PET2D::Strip::AnalyticKernel<double> k(sigma_z, sigma_dl);
   return k(y, tan, sec, R, PET2D::Vector<double>(dz, dy));
 }
 

@@ -175,6 +175,45 @@ static inline double kernel_d(double isz, double isdl, double c, double y,
   return (c / sqrt(norm)) * exp(bcb * -0.5);
 }
 
+/* AnalyticKernel::operator()(y, tan, sec, R, distance)
+ * (analytic_kernel.h:28-62): a, o, b as written there, then
+ *   norm = a.S.a + 2 o.S.b,  K = c / sqrt(norm) exp(-(b.S.b - (b.S.a)^2/norm)/2)
+ * with u.S.v = (u_p v_p + u_q v_q) isz + u_r v_r isdl in the reference's
+ * order (multiply_inv_cor_mat, gaussian_kernel.h:152-156). */
+static inline double analytic_d(double isz, double isdl, double c, double y,
+                                double t, double sec, double R, double dz,
+                                double dy) {
+  double sec_sq = sec * sec;
+  double yp = dy + y;
+  double ap = -(yp - R) * sec_sq, aq = -(yp + R) * sec_sq;
+  double ar = -2 * yp * sec * t;
+  double op = ap * t, oq = aq * t, orr = -yp * sec * (1 + 2 * t * t);
+  double pq = dz - dy * t;
+  double br = -2 * dy * sec;
+  double aia = ap * isz * ap + aq * isz * aq + ar * isdl * ar;
+  double bia = pq * isz * ap + pq * isz * aq + br * isdl * ar;
+  double bib = pq * isz * pq + pq * isz * pq + br * isdl * br;
+  double oib = op * isz * pq + oq * isz * pq + orr * isdl * br;
+  double norm = aia + 2 * oib;
+  return c / sqrt(norm) * exp(-0.5 * (bib - bia * bia / norm));
+}
+static inline float analytic_f(const s32* k, float y, float t, float sec,
+                               float R, float dz, float dy) {
+  float sec_sq = sec * sec;
+  float yp = dy + y;
+  float ap = -(yp - R) * sec_sq, aq = -(yp + R) * sec_sq;
+  float ar = -2 * yp * sec * t;
+  float op = ap * t, oq = aq * t, orr = -yp * sec * (1 + 2 * t * t);
+  float pq = dz - dy * t;
+  float br = -2 * dy * sec;
+  float aia = ap * k->isz * ap + aq * k->isz * aq + ar * k->isdl * ar;
+  float bia = pq * k->isz * ap + pq * k->isz * aq + br * k->isdl * ar;
+  float bib = pq * k->isz * pq + pq * k->isz * pq + br * k->isdl * br;
+  float oib = op * k->isz * pq + oq * k->isz * pq + orr * k->isdl * br;
+  float norm = aia + 2 * oib;
+  return k->c / sqrtf(norm) * expf(-0.5f * (bib - bia * bia / norm));
+}
+
 /* ------------------------------------------------------------------------ */
 /* Event geometry: centre, ellipse and the clamped exclusive box
  * (reconstruction.h:222-246). */
@@ -457,8 +496,96 @@ int oracle_mlem_f64(const oracle_scanner* sc_in, const float* events,
   return 0;
 }
 
+/* The reference --gpu iteration (cuda/reconstruction.cu:59-113,
+ * cuda/reconstruction/warp_granularity.cuh:42-153) in FP64: per event the
+ * denominator D = sum_p K rho_p s_p over the same box and 3-sigma ellipse,
+ * then rho'_p += K rho_p / D, with the Gaussian (kernel 0) or the analytic
+ * kernel (kernel 1) and the sensitivity s (NULL: 1).  D = 0 contributes
+ * nothing (the reference divides unguarded). */
+static void bb_update_variant_f64(const s64* s, const float* e, const double* rho,
+                                  int kernel, const double* sens, double* out,
+                                  scratch64* sc) {
+  ev64 v;
+  event_d(s, e, &v);
+  size_t box = (v.x1 > v.x0 && v.y1 > v.y0)
+                   ? (size_t)(v.x1 - v.x0) * (size_t)(v.y1 - v.y0)
+                   : 0;
+  ensure_cap64(sc, box + 1);
+  size_t n_ell = 0;
+  double denom = 0;
+  for (int iy = v.y0; iy < v.y1; ++iy) {
+    double py = fma(-(double)iy, s->ph, s->tl_y_half_h);
+    double dy = py - v.y;
+    double Ady2 = (dy * v.A) * dy;
+    double Bdy = dy * v.B;
+    for (int iz = v.x0; iz < v.x1; ++iz) {
+      double px = fma((double)iz, s->pw, s->tl_z_half_w);
+      double dz = px - v.z;
+      double q = fma(v.C * dz, dz, fma(Bdy, dz, Ady2));
+      if (!(q <= 9.0)) continue;
+      int32_t p = iy * s->n_z + iz;
+      double K = kernel ? analytic_d(s->isz, s->isdl, s->c, v.y, v.t, v.sec, s->R, dz, dy)
+                        : kernel_d(s->isz, s->isdl, s->c, v.y, v.t, v.sec, s->R, dz, dy);
+      double kr = K * rho[p];
+      denom += kr * (sens ? sens[p] : 1.0);
+      sc->idx[n_ell] = p;
+      sc->kr[n_ell] = kr;
+      ++n_ell;
+    }
+  }
+  double inv = (denom > 0) ? 1 / denom : 0;
+  for (size_t i = 0; i < n_ell; ++i) out[sc->idx[i]] += sc->kr[i] * inv;
+}
+
+int oracle_mlem_variant_f64(const oracle_scanner* sc_in, const float* events,
+                            int64_t n, double* rho, int n_iter, int kernel,
+                            const double* sens, int n_threads, double* rho_iters) {
+  s64 s;
+  make_s64(sc_in, &s, sc_in->sigma_z, sc_in->sigma_dl);
+  if (n_threads < 1) n_threads = 1;
+  const size_t np = (size_t)s.n_y * (size_t)s.n_z;
+  double* priv = (double*)malloc((size_t)n_threads * np * sizeof(double));
+  if (!priv) return -1;
+  for (int it = 0; it < n_iter; ++it) {
+    memset(priv, 0, (size_t)n_threads * np * sizeof(double));
+#pragma omp parallel num_threads(n_threads)
+    {
+      scratch64 sc = {0, 0, 0};
+      double* mine = priv + (size_t)omp_get_thread_num() * np;
+#pragma omp for schedule(dynamic)
+      for (int64_t e = 0; e < n; ++e)
+        bb_update_variant_f64(&s, events + 3 * e, rho, kernel, sens, mine, &sc);
+      free(sc.idx);
+      free(sc.kr);
+    }
+    for (size_t i = 0; i < np; ++i) rho[i] = 0;
+    for (int t = 0; t < n_threads; ++t)
+      for (size_t i = 0; i < np; ++i) rho[i] += priv[(size_t)t * np + i];
+    if (rho_iters)
+      memcpy(rho_iters + (size_t)it * np, rho, np * sizeof(double));
+  }
+  free(priv);
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Point helpers and kernel probes for the unit-test goldens. */
+float oracle_analytic_kernel_f32(double sigma_z, double sigma_dl, float y, float tan,
+                                 float sec, float R, float dz, float dy) {
+  oracle_scanner sc = {1, 1, 1, 1, 1, 1, sigma_z, sigma_dl, 0, 0};
+  s32 s;
+  make_s32(&sc, &s);
+  return analytic_f(&s, y, tan, sec, R, dz, dy);
+}
+
+double oracle_analytic_kernel_f64(double sigma_z, double sigma_dl, double y,
+                                  double tan, double sec, double R, double dz,
+                                  double dy) {
+  double isz = 1 / (sigma_z * sigma_z), isdl = 1 / (sigma_dl * sigma_dl);
+  double c = 1 / (2 * M_PI * M_PI) * sqrt(isz * isz * isdl);
+  return analytic_d(isz, isdl, c, y, tan, sec, R, dz, dy);
+}
+
 float oracle_kernel_f32(double sigma_z, double sigma_dl, float y, float tan,
                         float sec, float R, float dz, float dy) {
   oracle_scanner sc = {1, 1, 1, 1, 1, 1, sigma_z, sigma_dl, 0, 0};

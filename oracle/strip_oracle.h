@@ -13,6 +13,9 @@
  *   Strip::GaussianKernel               src/2d/strip/gaussian_kernel.h:19-160
  *   Strip::Scanner                      src/2d/strip/scanner.h:39-67,136-185
  *   PET2D::Pixel::clamp/index           src/2d/geometry/pixel.h:29,51-54
+ *   Strip::AnalyticKernel               src/2d/strip/analytic_kernel.h:28-62
+ *   the --gpu iteration                 src/2d/strip/cuda/reconstruction.cu:59-113,
+ *                                       cuda/reconstruction/warp_granularity.cuh:42-153
  *
  * Floating-point contraction: the reference is built by GCC with
  * -O3 -march=native (CMakeLists.txt:76-81).  In C++ mode GCC contracts
@@ -83,6 +86,25 @@ float oracle_kernel_f32(double sigma_z, double sigma_dl, float y, float tan,
                         float sec, float R, float dz, float dy);
 double oracle_kernel_f64(double sigma_z, double sigma_dl, double y, double tan,
                          double sec, double R, double dz, double dy);
+
+/* AnalyticKernel::operator()(y, tan, sec, R, distance)
+ * (analytic_kernel.h:28-62). */
+float oracle_analytic_kernel_f32(double sigma_z, double sigma_dl, float y,
+                                 float tan, float sec, float R, float dz,
+                                 float dy);
+double oracle_analytic_kernel_f64(double sigma_z, double sigma_dl, double y,
+                                  double tan, double sec, double R, double dz,
+                                  double dy);
+
+/* The reference --gpu iteration in FP64 (cuda/reconstruction.cu:59-113,
+ * warp_granularity.cuh:42-153): denominator sum K rho sens, update
+ * rho' = sum_e K rho / D_e; kernel 0 = Gaussian, 1 = analytic; sens (W*H,
+ * nullable = 1) is the caller's map, normally the 5-point pixel
+ * sensitivity.  Same box and ellipse as oracle_mlem_f64. */
+int oracle_mlem_variant_f64(const oracle_scanner* s, const float* events,
+                            int64_t n, double* rho, int n_iter, int kernel,
+                            const double* sens, int n_threads,
+                            double* rho_iters);
 
 /* GaussianKernel::ellipse_bb -> {sec, A, B, C, bb_y, bb_z}. */
 void oracle_ellipse_bb_f32(double sigma_z, double sigma_dl, float tan,

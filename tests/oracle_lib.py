@@ -59,6 +59,12 @@ def oracle():
         o.oracle_sensitivity_point_f64.restype = C.c_double
         o.oracle_loglik_f64.argtypes = [S, vp, i64, vp, vp, C.c_double, C.c_double,
                                         C.c_double, C.c_double, vp, vp, vp, vp, C.c_int]
+        o.oracle_analytic_kernel_f32.argtypes = [C.c_double, C.c_double] + [C.c_float] * 6
+        o.oracle_analytic_kernel_f32.restype = C.c_float
+        o.oracle_analytic_kernel_f64.argtypes = [C.c_double] * 8
+        o.oracle_analytic_kernel_f64.restype = C.c_double
+        o.oracle_mlem_variant_f64.argtypes = [S, vp, i64, vp, C.c_int, C.c_int, vp,
+                                              C.c_int, vp]
         _o = o
     return _o
 
@@ -81,6 +87,10 @@ def ref():
         r.ref_strip_kernel_f64.argtypes = [d] * 8
         r.ref_strip_kernel_f64.restype = d
         r.ref_strip_ellipse_bb.argtypes = [d, d, d, vp]
+        r.ref_strip_analytic_kernel_f32.argtypes = [d] * 8
+        r.ref_strip_analytic_kernel_f32.restype = d
+        r.ref_strip_analytic_kernel_f64.argtypes = [d] * 8
+        r.ref_strip_analytic_kernel_f64.restype = d
         r.ref_strip_tan_y_z.argtypes = [d, vp, vp]
         r.ref_strip_pixel_at.argtypes = [d, d, C.c_int, C.c_int, d, C.c_float,
                                          C.c_float, vp]
@@ -129,6 +139,22 @@ def mlem_f64(sc, events, n_iter, n_threads=1, rho0=None):
     iters = np.zeros((n_iter, np_), np.float64)
     o.oracle_mlem_f64(C.byref(s), P(ev), ev.shape[0], P(rho), n_iter, n_threads,
                       P(iters))
+    return iters.reshape(n_iter, sc.n_y_pixels, sc.n_z_pixels)
+
+
+def mlem_variant_f64(sc, events, n_iter, kernel=1, sens=None, n_threads=8, rho0=None):
+    """The reference --gpu iteration in FP64: kernel 0 = Gaussian,
+    1 = analytic; sens = the map in the denominator (None: 1)."""
+    o = oracle()
+    s = oscanner(sc)
+    np_ = sc.n_y_pixels * sc.n_z_pixels
+    ev = np.ascontiguousarray(events, dtype=np.float32).reshape(-1, 3)
+    rho = np.ones(np_, np.float64) if rho0 is None else \
+        np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1).copy()
+    sv = None if sens is None else np.ascontiguousarray(sens, np.float64).reshape(-1)
+    iters = np.zeros((n_iter, np_), np.float64)
+    o.oracle_mlem_variant_f64(C.byref(s), P(ev), ev.shape[0], P(rho), n_iter, kernel,
+                              P(sv), n_threads, P(iters))
     return iters.reshape(n_iter, sc.n_y_pixels, sc.n_z_pixels)
 
 

@@ -140,3 +140,23 @@ def test_rho_resume(tmp_path, gpu_pkg):
     img = np.fromfile(tmp_path / "b_1.bin", np.float32).reshape(64, 64)
     ref = ol.mlem_f32(sc, ev, 3, n_threads=8)[-1]
     assert np.linalg.norm(img - ref) / np.linalg.norm(ref) < 1e-5
+
+
+@needs_cli
+@pytest.mark.gpu
+def test_update_gpu_is_reference_gpu_iteration(tmp_path, gpu_pkg):
+    """--update gpu: analytic kernel with the 5-point sensitivity in the
+    denominator, against the oracle's FP64 statement of the reference --gpu
+    iteration."""
+    sc, ev, path = make_input(tmp_path, gpu_pkg)
+    out = tmp_path / "g.bin"
+    r = run([str(path), "-G", "-i", "1", "-I", "3", "-o", str(out), "--update", "gpu",
+             "-n", "64", "-p", "0.008", "-r", "0.43", "-l", "0.5", "--s-z", "0.01",
+             "--s-dl", "0.04"])
+    assert r.returncode == 0, r.stderr
+    img = np.fromfile(tmp_path / "g_3.bin", np.float32).reshape(64, 64)
+    sens = ol.sensitivity_f32(sc, 1).astype(np.float64)
+    ref = ol.mlem_variant_f64(sc, ev, 3, kernel=1, sens=sens, n_threads=8)[-1]
+    assert np.linalg.norm(img - ref) / np.linalg.norm(ref) < 1e-5
+    bad = run([str(path), "--update", "sideways", "-n", "64"])
+    assert bad.returncode != 0
