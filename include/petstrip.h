@@ -251,6 +251,61 @@ int petstrip_phantom_events_device(const petstrip_scanner* scanner,
                                    uint64_t seed, int64_t n, float* d_out,
                                    int device);
 
+/* The phantom command's generator (src/2d/strip/phantom_cmd.cpp:136-444,
+ * src/common/phantom_monte_carlo.h:33-109).  Regions are septuples
+ * (kind, x, y, a, b, angle_deg, intensity) with kind 0 = ellipse (semi-axes
+ * a, b, rotated by angle), 1 = rectangle (centre x, y, half-sides a, b),
+ * 2 = point (x, y); weights intensity x area, intensity for points
+ * (src/2d/geometry/phantom.h:44-67).  Flags: PETSTRIP_PHANTOM_ADDITIVE
+ * (regions overlap additively, phantom.h:193-203) and
+ * PETSTRIP_PHANTOM_DETECTED (n counts detected events, as --detected;
+ * otherwise n counts emissions and only the detected ones are returned).
+ * The same Philox stream as petstrip_phantom_events: in the detected mode
+ * with ellipses only, the responses equal that function's.  device < 0
+ * runs on host threads (the same events). */
+#define PETSTRIP_PHANTOM_ADDITIVE 1
+#define PETSTRIP_PHANTOM_DETECTED 2
+
+/* Image grid of the histograms: PixelGrid (src/2d/geometry/pixel_grid.h),
+ * n_columns along z, row 0 at the bottom (lower_left_y); tan planes of the
+ * tan-bins map as VariableVoxelSizeVoxelGrid (src/3d/geometry/voxel_grid.h
+ * :78-121) from -max_tan in steps of 2 max_tan / tan_bins. */
+typedef struct petstrip_phantom_grid {
+  int32_t n_columns, n_rows;
+  double pixel_size;
+  double lower_left_x, lower_left_y;
+  int32_t tan_bins;
+  double max_tan;
+} petstrip_phantom_grid;
+
+/* Host outputs; nullable ones are skipped.  The images are overwritten. */
+typedef struct petstrip_phantom_output {
+  float* responses;      /* capacity n x 3: (z_u, z_d, dl) with detector error */
+  float* exact;          /* n x 3: the responses without error               */
+  float* events;         /* n x 4: emission point and (sin phi, cos phi)     */
+  int32_t* emitted;      /* n_columns x n_rows: origins of every emission    */
+  int32_t* detected;     /* origins of the detected emissions                */
+  int32_t* naive;        /* centres of the responses with error              */
+  int32_t* tan_bins_map; /* n_columns x n_rows x tan_bins, those centres by tan */
+  int64_t n_responses;   /* out: responses written                           */
+  int64_t n_emitted;     /* out: emissions made                              */
+} petstrip_phantom_output;
+
+int petstrip_phantom_generate(const petstrip_scanner* scanner, const double* regions,
+                              int32_t n_regions, int32_t flags, uint64_t seed,
+                              int64_t n, const petstrip_phantom_grid* grid,
+                              petstrip_phantom_output* out, int device);
+
+/* The phantom command's --tan-bins kernel map (phantom_cmd.cpp:396-440):
+ * per voxel of the grid, the analytic kernel (analytic_kernel.h:28-62) at
+ * the kernel point seen from the voxel centre with the plane's tan, times
+ * 4 R pixel^2 dtan sqrt(1 + tan^2) and scale (the reference's
+ * 1/sensitivity(kernel point) with --detected, else 1), FP32 on device. */
+int petstrip_phantom_tan_kernel_map(const petstrip_scanner* scanner,
+                                    const petstrip_phantom_grid* grid,
+                                    double kernel_point_x, double kernel_point_y,
+                                    double scale, float* out, int device);
+
 /* Copies the handle's n_events responses (z_u, z_d, dl triples) to the host. */
 int petstrip_get_events(petstrip_handle* h, float* responses);
 

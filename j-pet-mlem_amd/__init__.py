@@ -7,7 +7,8 @@ from .petstrip import (  # noqa: F401
     ACCUM_COMPENSATED, ACCUM_FIXED64, ACCUM_FP32, KERNEL_ANALYTIC, KERNEL_GAUSSIAN,
     PATH_AUTO, PATH_PER_EVENT, UPDATE_CPU,
     UPDATE_SENSITIVITY_DENOMINATOR, PetstripError, Reconstruction, Scanner,
-    device_count, load_library, phantom_events,
+    device_count, load_library, phantom_events, phantom_generate, phantom_grid,
+    phantom_tan_kernel_map, REGION_ELLIPSE, REGION_POINT, REGION_RECTANGLE,
 )
 from .workloads import WORKLOADS, workload_scanner, workload_events  # noqa: F401
 from .calibration import calibrate_sigma  # noqa: F401

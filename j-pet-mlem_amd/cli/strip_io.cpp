@@ -170,6 +170,27 @@ void write_nrrd(const std::string& fn, const std::string& raw_name, int width, i
       << "encoding: " << (ascii ? "ascii" : "raw") << '\n';
 }
 
+void write_nrrd_typed(const std::string& fn, const std::string& raw_name, const char* type,
+                      const std::vector<size_t>& sizes, bool ascii) {
+  std::ofstream out(fn);
+  if (!out.is_open()) throw std::runtime_error("cannot write " + fn);
+  out << "NRRD0001\n"
+      << "type: " << type << '\n'
+      << "endian: little\n"
+      << "dimension: " << sizes.size() << '\n'
+      << "sizes:";
+  for (size_t v : sizes) out << ' ' << v;
+  out << '\n'
+      << "data file: ./" << raw_name << '\n'
+      << "encoding: " << (ascii ? "ascii" : "raw") << '\n';
+}
+
+void write_raw_bytes(const std::string& fn, const void* data, size_t bytes) {
+  std::ofstream out(fn, std::ios::binary);
+  if (!out.is_open()) throw std::runtime_error("cannot write " + fn);
+  out.write(reinterpret_cast<const char*>(data), (std::streamsize)bytes);
+}
+
 void write_png(const std::string& fn, const float* img, int width, int height,
                double scale_max) {
   std::ofstream out(fn, std::ios::binary);

@@ -34,6 +34,11 @@ void write_text_matrix(const std::string& fn, const float* img, int width, int h
 void write_text_tuples(const std::string& fn, const float* img, int width, int height);
 void write_nrrd(const std::string& fn, const std::string& raw_name, int width,
                 int height, bool ascii);
+// NRRD header of any type ("int", "float") and dimension (sizes fastest
+// first), and raw bytes (src/util/nrrd_writer.h:18-27, bstream.h).
+void write_nrrd_typed(const std::string& fn, const std::string& raw_name, const char* type,
+                      const std::vector<size_t>& sizes, bool ascii);
+void write_raw_bytes(const std::string& fn, const void* data, size_t bytes);
 // scale_max = 0: the image maximum maps to black (inverted grayscale).
 void write_png(const std::string& fn, const float* img, int width, int height,
                double scale_max);

@@ -26,198 +26,48 @@
 #include <vector>
 
 #include "../../include/petstrip.h"
+#include "cmdline.h"
 #include "strip_io.h"
 
 namespace {
 
-struct OptSpec {
-  const char* name;
-  char short_name;
-  bool takes_value;
-  const char* def;  // default value ("" = none)
-  bool from_file;   // may be set from a config file
-  const char* help;
-};
+using namespace strip_cli;
 
-const OptSpec kOptions[] = {
-    {"config", 'c', true, "", true, "load config file"},
-    {"radius", 'r', true, "0.5", true, "R distance between scintillator"},
-    {"length", 'l', true, "1", true, "scintillator length"},
-    {"s-pixel", 'p', true, "0.01", true, "pixel size"},
-    {"n-pixels", 'n', true, "", true, "number of pixels"},
-    {"n-z-pixels", 0, true, "", true, "number of z pixels"},
-    {"n-y-pixels", 0, true, "", true, "number of y pixels"},
-    {"s-z", 0, true, "0.015", true, "TOF sigma along z axis"},
-    {"s-dl", 0, true, "0.06", true, "TOF sigma delta-l"},
-    {"n-emissions", 'e', true, "0", false, "number of emissions"},
-    {"verbose", 'v', false, "", true, "print progress information (-v) or benchmark (-vv)"},
-    {"n-threads", 'T', true, "", true, "number of host threads"},
-    {"3d-response", 0, false, "", true, "input response contains additional LOR information"},
-    {"dl-is-time", 0, false, "", true, "delta is time, convert using speed of light"},
-    {"speed-of-light", 0, true, "299792458", true, "speed of light"},
-    {"scale-length", 0, true, "1", true, "scale input length values"},
-    {"scale-time", 0, true, "1", true, "scale input time values"},
-    {"blocks", 'i', true, "0", true, "number of iteration blocks"},
-    {"iterations", 'I', true, "1", true, "number of iterations (per block)"},
-    {"output", 'o', true, "", false, "output files prefix (png)"},
-    {"png-max", 0, true, "0", true, "maximum value mapped to 255 in PNG"},
-    {"gpu", 'G', false, "", true, "GPU output conventions (always runs on the GPU)"},
-    {"cuda-device", 'D', true, "0", true, "GPU device"},
-    {"cuda-blocks", 'B', true, "64", true, "ignored (kept for compatibility)"},
-    {"cuda-threads", 'W', true, "512", true, "ignored (kept for compatibility)"},
-    {"rho", 0, true, "", true, "start rho (eg. existing iteration)"},
-    {"rho-is-int", 0, false, "", true, "start rho uses integer representation (not float)"},
-    {"accumulation", 0, true, "fp32", true, "fp32 | compensated | fixed64"},
-    {"update", 0, true, "cpu", true,
+const std::vector<OptSpec> kOptions = {
+    {"config", 'c', true, "", true, kSave, "load config file"},
+    {"radius", 'r', true, "0.5", true, kSave, "R distance between scintillator"},
+    {"length", 'l', true, "1", true, kSave, "scintillator length"},
+    {"s-pixel", 'p', true, "0.01", true, kSave, "pixel size"},
+    {"n-pixels", 'n', true, "", true, kSave, "number of pixels"},
+    {"n-z-pixels", 0, true, "", true, kSave, "number of z pixels"},
+    {"n-y-pixels", 0, true, "", true, kSave, "number of y pixels"},
+    {"s-z", 0, true, "0.015", true, kSave, "TOF sigma along z axis"},
+    {"s-dl", 0, true, "0.06", true, kSave, "TOF sigma delta-l"},
+    {"n-emissions", 'e', true, "0", false, kSave, "number of emissions"},
+    {"verbose", 'v', false, "", true, kSave, "print progress information (-v) or benchmark (-vv)"},
+    {"n-threads", 'T', true, "", true, kSave, "number of host threads"},
+    {"3d-response", 0, false, "", true, kSave, "input response contains additional LOR information"},
+    {"dl-is-time", 0, false, "", true, kSave, "delta is time, convert using speed of light"},
+    {"speed-of-light", 0, true, "299792458", true, kSave, "speed of light"},
+    {"scale-length", 0, true, "1", true, kSave, "scale input length values"},
+    {"scale-time", 0, true, "1", true, kSave, "scale input time values"},
+    {"blocks", 'i', true, "0", true, kSave, "number of iteration blocks"},
+    {"iterations", 'I', true, "1", true, kSave, "number of iterations (per block)"},
+    {"output", 'o', true, "", false, kSave, "output files prefix (png)"},
+    {"png-max", 0, true, "0", true, kSave, "maximum value mapped to 255 in PNG"},
+    {"gpu", 'G', false, "", true, kSave, "GPU output conventions (always runs on the GPU)"},
+    {"cuda-device", 'D', true, "0", true, kSave, "GPU device"},
+    {"cuda-blocks", 'B', true, "64", true, kSave, "ignored (kept for compatibility)"},
+    {"cuda-threads", 'W', true, "512", true, kSave, "ignored (kept for compatibility)"},
+    {"rho", 0, true, "", true, kSave, "start rho (eg. existing iteration)"},
+    {"rho-is-int", 0, false, "", true, kSave, "start rho uses integer representation (not float)"},
+    {"accumulation", 0, true, "fp32", true, kSave, "fp32 | compensated | fixed64"},
+    {"update", 0, true, "cpu", true, kSave,
      "cpu (Gaussian kernel, the CPU update) | gpu (analytic kernel, 5-point "
      "sensitivity in the denominator: the reference --gpu iteration)"},
-    {"help", 'h', false, "", false, "print this message"},
+    {"help", 'h', false, "", false, kSave, "print this message"},
 };
 
-struct Args {
-  std::map<std::string, std::string> value;
-  std::map<std::string, int> count;  // flags and how often they were given
-  std::vector<std::string> rest;
-  std::vector<std::string> dir_stack;
-
-  const OptSpec* find(const std::string& n) const {
-    for (auto& o : kOptions)
-      if (n == o.name) return &o;
-    return nullptr;
-  }
-  const OptSpec* find_short(char c) const {
-    for (auto& o : kOptions)
-      if (o.short_name == c) return &o;
-    return nullptr;
-  }
-  bool exist(const std::string& n) const { return value.count(n) || count.count(n); }
-  std::string get(const std::string& n) const {
-    auto it = value.find(n);
-    if (it != value.end()) return it->second;
-    auto* o = find(n);
-    return o ? o->def : "";
-  }
-};
-
-// Integers accept k/m/g suffixes (src/util/cmdline_types.h:114-138).
-long to_long(const std::string& s) {
-  char* end = nullptr;
-  long v = std::strtol(s.c_str(), &end, 10);
-  if (end == s.c_str()) throw std::runtime_error("bad integer: " + s);
-  if (*end) {
-    if (end[1]) throw std::runtime_error("bad integer: " + s);
-    switch (std::tolower(*end)) {
-      case 'k': v *= 1000; break;
-      case 'm': v *= 1000000; break;
-      case 'g': v *= 1000000000; break;
-      default: throw std::runtime_error("bad integer: " + s);
-    }
-  }
-  return v;
-}
-
-double to_double(const std::string& s) {
-  char* end = nullptr;
-  double v = std::strtod(s.c_str(), &end);
-  if (end == s.c_str() || *end) throw std::runtime_error("bad number: " + s);
-  return v;
-}
-
-std::vector<double> to_doubles(const std::string& s) {
-  std::vector<double> v;
-  std::stringstream ss(s);
-  std::string el;
-  while (std::getline(ss, el, ',')) v.push_back(to_double(el));
-  return v;
-}
-
-void load_config(Args& a, const std::string& arg, bool verbose);
-
-void set_option(Args& a, const OptSpec* o, const std::string& val, bool from_file) {
-  if (from_file && !o->from_file) return;  // not_from_file options keep their value
-  if (!o->takes_value) {
-    a.count[o->name] += 1;
-    return;
-  }
-  if (std::string(o->name) == "config") {
-    load_config(a, val, a.exist("verbose"));
-    return;
-  }
-  a.value[o->name] = val;
-}
-
-void parse(Args& a, const std::vector<std::string>& argv, bool from_file) {
-  for (size_t i = 0; i < argv.size(); ++i) {
-    const std::string& s = argv[i];
-    if (s.size() > 2 && s[0] == '-' && s[1] == '-') {
-      size_t eq = s.find('=');
-      std::string name = s.substr(2, eq == std::string::npos ? std::string::npos : eq - 2);
-      const OptSpec* o = a.find(name);
-      if (!o) throw std::runtime_error("undefined option: --" + name);
-      if (!o->takes_value) {
-        set_option(a, o, "", from_file);
-      } else if (eq != std::string::npos) {
-        set_option(a, o, s.substr(eq + 1), from_file);
-      } else {
-        if (i + 1 >= argv.size()) throw std::runtime_error("option needs value: --" + name);
-        set_option(a, o, argv[++i], from_file);
-      }
-    } else if (s.size() > 1 && s[0] == '-' && !std::isdigit((unsigned char)s[1]) && s[1] != '.') {
-      for (size_t j = 1; j < s.size(); ++j) {
-        const OptSpec* o = a.find_short(s[j]);
-        if (!o) throw std::runtime_error(std::string("undefined short option: -") + s[j]);
-        if (!o->takes_value) {
-          set_option(a, o, "", from_file);
-          continue;
-        }
-        if (j + 1 < s.size()) {
-          set_option(a, o, s.substr(j + 1), from_file);
-        } else {
-          if (i + 1 >= argv.size())
-            throw std::runtime_error(std::string("option needs value: -") + s[j]);
-          set_option(a, o, argv[++i], from_file);
-        }
-        break;
-      }
-    } else {
-      a.rest.push_back(s);
-    }
-  }
-}
-
-// Config files: one argument per line, '#' lines are comments
-// (lib/cmdline/cmdline.h:508-517, src/util/cmdline_hooks.cpp:50-99).
-void load_config(Args& a, const std::string& arg, bool verbose) {
-  std::string path = arg;
-  if (!a.dir_stack.empty() && !path.empty() && path[0] != '/') path = a.dir_stack.back() + path;
-  if (verbose) std::cout << "# load: " << path << std::endl;
-  std::ifstream in(path);
-  if (!in.is_open()) throw std::runtime_error("cannot open input config file: " + path);
-  size_t sep = path.find_last_of('/');
-  a.dir_stack.push_back(sep == std::string::npos ? std::string() : path.substr(0, sep + 1));
-  std::vector<std::string> args;
-  std::string line;
-  while (std::getline(in, line))
-    if (!line.empty() && line[0] != '#') args.push_back(line);
-  size_t rest_before = a.rest.size();
-  parse(a, args, true);
-  a.rest.resize(rest_before);  // positional lines in config files are ignored
-  a.dir_stack.pop_back();
-}
-
-void usage() {
-  std::cerr << "usage: 2d_strip_reconstruction [options] response ...\n"
-               "options:\n";
-  for (auto& o : kOptions) {
-    std::cerr << "  ";
-    if (o.short_name) std::cerr << '-' << o.short_name << ", ";
-    else std::cerr << "    ";
-    std::cerr << "--" << std::left << std::setw(16) << o.name << o.help;
-    if (o.def[0]) std::cerr << " (=" << o.def << ")";
-    std::cerr << '\n';
-  }
-  std::cerr << "note: lengths are in the units of the responses (normally metres).\n";
-}
 
 // Benchmark / estimate progress in the reference's format (src/util/progress.h).
 struct Progress {
@@ -317,26 +167,17 @@ void check(int rc) {
 
 int run(int argc, char** argv) {
   Args a;
+  a.specs = kOptions;
   std::vector<std::string> args(argv + 1, argv + argc);
   parse(a, args, false);
   if (a.exist("help") || argc == 1) {
-    usage();
+    usage(a, "2d_strip_reconstruction [options] response ...",
+          "note: lengths are in the units of the responses (normally metres).");
     return 0;
   }
   if (a.rest.empty()) throw std::runtime_error("at least one responses input file expected, consult --help");
   // derived pixel counts (options.cpp:106-124), before the accompanying config
-  if (a.value.count("n-pixels")) {
-    a.value["n-z-pixels"] = a.value["n-pixels"];
-    a.value["n-y-pixels"] = a.value["n-pixels"];
-  } else {
-    auto radius = to_doubles(a.get("radius"));
-    if (radius.empty()) throw std::runtime_error("at least one --radius has to be specified");
-    if (radius.size() > 1) std::cerr << "warning: more than one radius specified with --radius\n";
-    double L = to_double(a.get("length")), s = to_double(a.get("s-pixel"));
-    if (!a.value.count("n-z-pixels")) a.value["n-z-pixels"] = std::to_string((long)std::ceil(L / s));
-    if (!a.value.count("n-y-pixels"))
-      a.value["n-y-pixels"] = std::to_string((long)std::ceil(2 * radius[0] / s));
-  }
+  derive_pixel_counts(a);
   // <response without extension>.cfg (cmdline_hooks.cpp:110-123)
   for (auto& fn : a.rest) {
     std::string cfg = strip_io::without_ext(fn) + ".cfg";

@@ -20,6 +20,11 @@ hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* w,
                               double* out, int analytic = 0);
+// Analytic-kernel map of tan bins (w x h x planes floats into out).
+hipError_t launch_tan_kernel_map(hipStream_t st, const StripConst& k, int w, int h,
+                                 int planes, float pixel, float llx, float lly,
+                                 float max_tan, float kpx, float kpy, float scale,
+                                 float* out);
 hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
                              const StripConst& sk, const EventGeom* geom,
                              int64_t n, const float* rho, const float* w,

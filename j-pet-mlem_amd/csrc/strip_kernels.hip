@@ -276,6 +276,38 @@ __global__ __launch_bounds__(256) void k_sigma_grad(StripConst sg, StripConst sk
 }
 
 // ---------------------------------------------------------------------------
+// Analytic-kernel map of the phantom command's tan bins
+// (src/2d/strip/phantom_cmd.cpp:396-440): voxel (x, y, plane) of a
+// column x row x tan-plane grid (PixelGrid / VariableVoxelSizeVoxelGrid
+// centres, row 0 at the bottom) holds
+//   K(centre as the event point, tan, sec; kernel point) x 4 R pixel^2 dtan sec
+//   x scale,
+// with dtan = 2 max_tan / planes and K the analytic kernel.
+
+__global__ __launch_bounds__(256) void k_tan_kernel_map(StripConst k, int w, int h,
+                                                        int planes, float pixel,
+                                                        float llx, float lly,
+                                                        float max_tan, float kpx,
+                                                        float kpy, float scale,
+                                                        float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)w * h * planes;
+  if (i >= n) return;
+  const int x = (int)(i % w), y = (int)((i / w) % h), z = (int)(i / ((int64_t)w * h));
+  const float dtan = max_tan / planes * 2;
+  const float t = (z + 0.5f) * dtan + -max_tan;
+  const float sec = sqrtf(1 + t * t);
+  const float cx = llx + pixel / 2 + x * pixel, cy = lly + pixel / 2 + y * pixel;
+  EventGeom g{};
+  g.y = cy;
+  g.t = t;
+  g.sec = sec;
+  const float kv = analytic_value(k, g, kpy - cy, kpx - cx);
+  const float volume = 4 * k.R * (pixel * pixel * dtan) * sqrtf(1 + t * t);
+  out[i] = kv * volume * scale;
+}
+
+// ---------------------------------------------------------------------------
 // Elementwise helpers.
 
 __global__ void k_fill(float* __restrict__ p, float v, int64_t n) {
@@ -414,6 +446,17 @@ hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t launch_tan_kernel_map(hipStream_t st, const StripConst& k, int w, int h,
+                                 int planes, float pixel, float llx, float lly,
+                                 float max_tan, float kpx, float kpy, float scale,
+                                 float* out) {
+  const int64_t n = (int64_t)w * h * planes;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tan_kernel_map, dim3(blocks_for(n, 256)), dim3(256), 0, st, k, w, h,
+                     planes, pixel, llx, lly, max_tan, kpx, kpy, scale, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill(hipStream_t st, float* p, float v, int64_t n) {

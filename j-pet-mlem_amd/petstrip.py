@@ -48,6 +48,20 @@ class COptions(C.Structure):
                 ("reserved", C.c_int32 * 2)]
 
 
+class CPhantomGrid(C.Structure):
+    _fields_ = [("n_columns", C.c_int32), ("n_rows", C.c_int32),
+                ("pixel_size", C.c_double), ("lower_left_x", C.c_double),
+                ("lower_left_y", C.c_double), ("tan_bins", C.c_int32),
+                ("max_tan", C.c_double)]
+
+
+class CPhantomOutput(C.Structure):
+    _fields_ = [("responses", C.c_void_p), ("exact", C.c_void_p), ("events", C.c_void_p),
+                ("emitted", C.c_void_p), ("detected", C.c_void_p), ("naive", C.c_void_p),
+                ("tan_bins_map", C.c_void_p), ("n_responses", C.c_int64),
+                ("n_emitted", C.c_int64)]
+
+
 class CStats(C.Structure):
     _fields_ = [
         ("n_events", C.c_int64), ("n_box_pixels", C.c_int64),
@@ -106,6 +120,11 @@ def load_library(path=LIB_PATH):
                                             i64, vp, C.c_int], C.c_int),
         "petstrip_set_phantom_events": ([vp, fp, i32, C.c_uint64, i64], C.c_int),
         "petstrip_get_events": ([vp, fp], C.c_int),
+        "petstrip_phantom_generate": ([C.POINTER(CScanner), fp, i32, i32, C.c_uint64, i64,
+                                       C.POINTER(CPhantomGrid), C.POINTER(CPhantomOutput),
+                                       C.c_int], C.c_int),
+        "petstrip_phantom_tan_kernel_map": ([C.POINTER(CScanner), C.POINTER(CPhantomGrid),
+                                             dp, dp, dp, fp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -178,6 +197,71 @@ def phantom_events(scanner, regions, n, seed=1, n_threads=None):
     _check(lib.petstrip_phantom_events(C.byref(sc), _ptr(reg), reg.shape[0],
                                        C.c_uint64(seed), C.c_int64(int(n)),
                                        _ptr(out), int(n_threads)))
+    return out
+
+
+REGION_ELLIPSE, REGION_RECTANGLE, REGION_POINT = 0, 1, 2
+PHANTOM_ADDITIVE, PHANTOM_DETECTED = 1, 2
+
+
+def phantom_grid(n_columns, n_rows, pixel_size, origin=(0.0, 0.0), tan_bins=0,
+                 max_tan=0.0):
+    """PixelGrid centred on origin (lower-left corner at
+    -pixel_size * (n_columns, n_rows) / 2 + origin), row 0 at the bottom."""
+    return CPhantomGrid(int(n_columns), int(n_rows), float(pixel_size),
+                        -pixel_size * n_columns / 2 + origin[0],
+                        -pixel_size * n_rows / 2 + origin[1], int(tan_bins), float(max_tan))
+
+
+def phantom_generate(scanner, regions, n, seed=1, additive=False, detected=False,
+                     grid=None, full=False, images=False, device=-1):
+    """The phantom command's generator (src/2d/strip/phantom_cmd.cpp:136-444).
+
+    regions: (kind, x, y, a, b, angle_deg, intensity) with kind REGION_*.
+    detected: n counts detected events (else emissions).  With grid and
+    images, also the emitted / detected / naive images (n_rows x n_columns,
+    row 0 at the bottom) and, with grid.tan_bins > 0, the tan-bins map.
+    device < 0: host threads.  Returns a dict."""
+    lib = load_library()
+    reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1, 7))
+    n = int(n)
+    res = {"responses": np.empty((n, 3), np.float32)}
+    if full:
+        res["exact"] = np.empty((n, 3), np.float32)
+        res["events"] = np.empty((n, 4), np.float32)
+    if images:
+        shape = (grid.n_rows, grid.n_columns)
+        for k in ("emitted", "detected", "naive"):
+            res[k] = np.empty(shape, np.int32)
+        if grid.tan_bins > 0:
+            res["tan_bins_map"] = np.empty((grid.tan_bins,) + shape, np.int32)
+    out = CPhantomOutput(*[_ptr(res.get(k)) for k in
+                           ("responses", "exact", "events", "emitted", "detected", "naive",
+                            "tan_bins_map")], 0, 0)
+    flags = (PHANTOM_ADDITIVE if additive else 0) | (PHANTOM_DETECTED if detected else 0)
+    sc = scanner.c()
+    _check(lib.petstrip_phantom_generate(C.byref(sc), _ptr(reg), reg.shape[0], flags,
+                                         C.c_uint64(seed), C.c_int64(n),
+                                         C.byref(grid) if grid is not None else None,
+                                         C.byref(out), int(device)))
+    m = out.n_responses
+    res["responses"] = res["responses"][:m]
+    if full:
+        res["exact"] = res["exact"][:m]
+        res["events"] = res["events"][:m]
+    res["n_emitted"] = out.n_emitted
+    return res
+
+
+def phantom_tan_kernel_map(scanner, grid, kernel_point=(0.0, 0.0), scale=1.0, device=0):
+    """The --tan-bins kernel map (tan_bins x n_rows x n_columns float32)."""
+    lib = load_library()
+    out = np.empty((grid.tan_bins, grid.n_rows, grid.n_columns), np.float32)
+    sc = scanner.c()
+    _check(lib.petstrip_phantom_tan_kernel_map(C.byref(sc), C.byref(grid),
+                                               float(kernel_point[0]),
+                                               float(kernel_point[1]), float(scale),
+                                               _ptr(out), int(device)))
     return out
 
 

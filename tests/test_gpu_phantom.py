@@ -51,3 +51,60 @@ def test_device_phantom_feeds_reconstruction(gpu_pkg):
     if not same:
         pytest.skip("the two event streams differ in a last bit on this device")
     assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+
+
+def test_device_generate_matches_host(gpu_pkg):
+    """The phantom command's generator on the device against host threads:
+    ellipses, a rectangle and a point, emission mode with images, full
+    outputs and the tan-bins map."""
+    pkg = gpu_pkg
+    sc = pkg.Scanner(0.43, 0.5, 64, 64, 0.004, 0.004, 0.01, 0.04)
+    regs = [(0, 0.0, 0.0, 0.03, 0.03, 0.0, 0.0), (1, 0.05, -0.03, 0.02, 0.01, 0.0, 3.0),
+            (2, -0.0513, 0.0517, 0.0, 0.0, 0.0, 0.01), (0, 0.0, 0.0, 0.15, 0.1, 20.0, 1.0)]
+    g = pkg.phantom_grid(128, 96, 0.003, tan_bins=6, max_tan=0.6)
+    out = [pkg.phantom_generate(sc, regs, 400000, seed=12, grid=g, images=True, full=True,
+                                device=d) for d in (-1, 0)]
+    h, d = out
+    print(f"case: emission mode, 4e5 emissions, {len(h['responses'])} detected", flush=True)
+    assert h["n_emitted"] == d["n_emitted"] == 400000
+    assert len(h["responses"]) == len(d["responses"])
+    same = np.all(h["responses"].view(np.uint32) == d["responses"].view(np.uint32), axis=1)
+    assert same.mean() > 0.9999
+    for k in ("exact", "events"):
+        assert np.max(np.abs(h[k] - d[k])) < 1e-6
+    for k in ("emitted", "detected", "naive", "tan_bins_map"):
+        assert np.abs(h[k].astype(np.int64) - d[k]).sum() <= 4, k
+    # detected mode: one event per slot
+    dm = pkg.phantom_generate(sc, regs, 50000, seed=12, detected=True, device=0)
+    assert len(dm["responses"]) == 50000 and dm["n_emitted"] > 50000
+
+
+def test_tan_kernel_map_vs_oracle(gpu_pkg):
+    """The --tan-bins kernel map against the oracle's FP64 analytic kernel
+    (analytic_kernel.h:28-62) times 4 R pixel^2 dtan sec, at sampled voxels."""
+    import oracle_lib as ol
+    pkg = gpu_pkg
+    sc = pkg.Scanner(0.43, 0.5, 64, 64, 0.004, 0.004, 0.01, 0.04)
+    g = pkg.phantom_grid(40, 30, 0.002, origin=(0.001, -0.002), tan_bins=5, max_tan=0.55)
+    kp = (0.003, -0.004)
+    m = pkg.phantom_tan_kernel_map(sc, g, kernel_point=kp, scale=2.0)
+    o = ol.oracle()
+    dt = 2 * 0.55 / 5
+    worst = 0.0
+    for z in range(5):
+        t = (z + 0.5) * dt - 0.55
+        sec = np.sqrt(1 + t * t)
+        for y in range(0, 30, 3):
+            for x in range(0, 40, 3):
+                cx = g.lower_left_x + (x + 0.5) * 0.002
+                cy = g.lower_left_y + (y + 0.5) * 0.002
+                k = o.oracle_analytic_kernel_f64(0.01, 0.04, cy, t, sec, 0.43,
+                                                 kp[0] - cx, kp[1] - cy)
+                want = k * 4 * 0.43 * 0.002 ** 2 * dt * sec * 2.0
+                got = float(m[z, y, x])
+                if want > 1e-30:
+                    worst = max(worst, abs(got / want - 1))
+                else:
+                    assert abs(got) < 1e-25
+    print(f"tan kernel map: worst relative error {worst:.2e}", flush=True)
+    assert worst < 1e-4
