@@ -271,12 +271,17 @@ struct Chain {
 };
 
 __device__ __forceinline__ Chain chain_start(const RowSeg& s, float du, float e_add) {
+  // two exponentials: the first pixel's value and the per-pixel ratio b1;
+  // the other starts and b4 = b1^4 by multiplication
   Chain c;
   const float e1 = -2.0f * s.u0 * du;
   const float e0 = fmaf(-s.u0, s.u0, s.lsc) + e_add;
+  const float b1 = __builtin_amdgcn_exp2f(e1);
+  c.p[0] = __builtin_amdgcn_exp2f(e0);
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) c.p[u] = __builtin_amdgcn_exp2f(fmaf((float)u, e1, e0));
-  c.b4 = __builtin_amdgcn_exp2f(4.0f * e1);
+  for (int u = 1; u < kUnroll; ++u) c.p[u] = c.p[u - 1] * b1;
+  const float b2 = b1 * b1;
+  c.b4 = b2 * b2;
   return c;
 }
 
