@@ -165,6 +165,17 @@ struct petstrip_handle {
   int ncb = 1;      // column-band slots per event (1: row bands only)
   int cshift = 30;  // log2 of the column-band width (30: no column bands)
   int64_t n_sorted = 0;    // events with a non-empty box (the binned ones)
+  // groups of whole chunks of the banded form: item range and sorted-event
+  // range, pipelined over two streams (pass 2 of a group beside pass 1 of the
+  // next)
+  struct BandGroup {
+    int item_begin, item_end;
+    int64_t event_begin, event_end;
+  };
+  std::vector<BandGroup> groups;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_group[8] = {};
+  hipEvent_t ev_join = nullptr;
   float* d_part = nullptr;
   float* d_lam_sorted = nullptr;
   size_t cap_part = 0;
@@ -221,6 +232,7 @@ int reserve_events(petstrip_handle* h, int64_t n) {
 constexpr int kBinW = 16;
 constexpr int kBinH = 8;
 constexpr int kChunk = 1024;      // events per work item
+constexpr int kMinGroupItems = 2048;  // banded items per pipelined group, at least
 // MLEM backprojection values K rho / lambda <= 1 are summed in 2^-31 units
 // (one conversion per value), or in 2^-40 units in the deterministic
 // accumulation mode (two conversions)
@@ -369,7 +381,10 @@ int prepare_fused(petstrip_handle* h) {
     const int cshift = cols ? petstrip::kColBandShift : 30;
     const int cb = 1 << cshift;
     std::vector<petstrip::WorkItem> bands;
+    std::vector<int> chunk_first;  // first band item of each chunk
+    chunk_first.reserve(items.size() + 1);
     for (const auto& it : items) {
+      chunk_first.push_back((int)bands.size());
       if (it.wh <= 0 || it.ww <= 0) continue;
       for (int b = it.wy0 / br; b <= (it.wy0 + it.wh - 1) / br; ++b) {
         for (int c = it.wx0 >> cshift; c <= (it.wx0 + it.ww - 1) >> cshift; ++c) {
@@ -403,6 +418,25 @@ int prepare_fused(petstrip_handle* h) {
       }
       dfree(h->d_lam_sorted);
       if ((rc = dalloc(&h->d_lam_sorted, (size_t)std::max<int64_t>(acc, 1)))) return rc;
+      // up to 8 groups of whole chunks, each enough items to fill the device
+      const int n_chunks = (int)items.size();
+      const int n_groups = (int)std::max<size_t>(
+          1, std::min<size_t>(8, bands.size() / (size_t)kMinGroupItems));
+      h->groups.clear();
+      int c0 = 0;
+      for (int g = 0; g < n_groups; ++g) {
+        const size_t target = bands.size() * (size_t)(g + 1) / n_groups;
+        int c1 = c0;
+        while (c1 < n_chunks && (g == n_groups - 1 || (size_t)chunk_first[c1] < target)) ++c1;
+        if (c1 == c0) continue;
+        petstrip_handle::BandGroup bg;
+        bg.item_begin = chunk_first[c0];
+        bg.item_end = c1 < n_chunks ? chunk_first[c1] : (int)bands.size();
+        bg.event_begin = items[c0].begin;
+        bg.event_end = c1 < n_chunks ? items[c1].begin : acc;
+        h->groups.push_back(bg);
+        c0 = c1;
+      }
       items.swap(bands);
       h->n_items = (int)items.size();
       if (items.size() > h->cap_items) {
@@ -511,6 +545,39 @@ int fused_forward(petstrip_handle* h, const petstrip::FusedArgs& a, bool sigma) 
   return PETSTRIP_OK;
 }
 
+// Pass 1 (mode1, BAND = 1), the combine and pass 2 (mode2, BAND = 2) of the
+// banded form, group by group: pass 1 and the combine of a group run on the
+// handle's stream and its pass 2 on the second stream once they are done,
+// so pass 2 of one group (bound by LDS atomics) runs beside pass 1 of the
+// next (bound by VALU issue).  The handle's stream waits for both at the end.
+int banded_pipeline(petstrip_handle* h, const petstrip::FusedArgs& a, int mode1,
+                    bool sigma, int mode2, size_t lds2) {
+  const size_t part = (size_t)std::max<int64_t>(h->n_sorted, 1) * h->nbands * 4;
+  HIP_TRY(hipMemsetAsync(h->d_part, 0, part * sizeof(float), h->stream));
+  const size_t lds1 =
+      petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_FORWARD);
+  int k = 0;
+  for (const auto& g : h->groups) {
+    petstrip::FusedArgs b = a;
+    b.items = a.items + g.item_begin;
+    const int ni = g.item_end - g.item_begin;
+    HIP_TRY(petstrip::launch_strip_banded(h->stream, mode1, 1, b, ni, lds1));
+    petstrip::FusedArgs c = a;  // the group's sorted events
+    c.part = a.part + (size_t)g.event_begin * h->nbands * 4;
+    c.sidx = a.sidx + g.event_begin;
+    c.lam_sorted = a.lam_sorted + g.event_begin;
+    HIP_TRY(petstrip::launch_band_combine(h->stream, c, g.event_end - g.event_begin,
+                                          sigma ? 1 : 0));
+    hipEvent_t ev = h->ev_group[k++ % 8];
+    HIP_TRY(hipEventRecord(ev, h->stream));
+    HIP_TRY(hipStreamWaitEvent(h->stream2, ev, 0));
+    HIP_TRY(petstrip::launch_strip_banded(h->stream2, mode2, 2, b, ni, lds2));
+  }
+  HIP_TRY(hipEventRecord(h->ev_join, h->stream2));
+  HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+  return PETSTRIP_OK;
+}
+
 int one_iteration(petstrip_handle* h) {
   const int64_t n = h->n_events;
   if (h->fused_ok) {
@@ -529,9 +596,9 @@ int one_iteration(petstrip_handle* h) {
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
     if (h->banded) {
       int rc;
-      if ((rc = fused_forward(h, a, false))) return rc;
-      HIP_TRY(petstrip::launch_strip_banded(h->stream, petstrip::MODE_MLEM, 2, a,
-                                            h->n_items, h->lds_bytes));
+      if ((rc = banded_pipeline(h, a, petstrip::MODE_MLEM, false, petstrip::MODE_MLEM,
+                                h->lds_bytes)))
+        return rc;
     } else {
       HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_MLEM, h->col_stride,
                                            accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
@@ -613,8 +680,14 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   if (hipGetDeviceProperties(&prop, device) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipGetDeviceProperties failed"));
   h->name = prop.name;
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipStreamCreate failed"));
+  for (auto& e : h->ev_group)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(PETSTRIP_EDEVICE, "hipEventCreate failed"));
+  if (hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(PETSTRIP_EDEVICE, "hipEventCreate failed"));
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
       hipEventCreate(&h->evk0) != hipSuccess || hipEventCreate(&h->evk1) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipEventCreate failed"));
@@ -650,6 +723,7 @@ int petstrip_destroy(petstrip_handle* h) {
   if (!h) return PETSTRIP_OK;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
   dfree(h->d_resp);
   dfree(h->d_geom);
   dfree(h->d_lambda);
@@ -673,10 +747,14 @@ int petstrip_destroy(petstrip_handle* h) {
   if (h->d_acc) (void)hipFree(h->d_acc);
   h->d_acc = nullptr;
   if (h->ev0) (void)hipEventDestroy(h->ev0);
+  for (auto& e : h->ev_group)
+    if (e) (void)hipEventDestroy(e);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   if (h->evk0) (void)hipEventDestroy(h->evk0);
   if (h->evk1) (void)hipEventDestroy(h->evk1);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   delete h;
   return PETSTRIP_OK;
 }
@@ -993,10 +1071,10 @@ int gradient_fused(petstrip_handle* h, float* grad_rho, double grad_sigma[2],
   HIP_TRY(hipEventRecord(h->evk0, h->stream));
   if (h->banded) {
     int rc;
-    if ((rc = fused_forward(h, a, true))) return rc;
-    HIP_TRY(petstrip::launch_strip_banded(
-        h->stream, petstrip::MODE_GRAD, 2, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD)));
+    if ((rc = banded_pipeline(
+             h, a, petstrip::MODE_FORWARD, true, petstrip::MODE_GRAD,
+             petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD))))
+      return rc;
   } else {
     HIP_TRY(petstrip::launch_strip_fused(
         h->stream, petstrip::MODE_GRAD, h->col_stride, 0, a, h->n_items,
