@@ -243,7 +243,7 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
 }
 
 constexpr int kUnroll = 4;
-constexpr int kSpareCols = kUnroll;  // rho window columns past the window's width
+constexpr int kSpareCols = kUnroll;  // zero rho window columns before the window
 constexpr int kCachedGroups = 4;  // row groups kept in registers between passes
 
 // k + sgn * log2(x) for x > 0 and integer k, formed as (k + sgn e) + sgn
@@ -255,7 +255,7 @@ __device__ __forceinline__ float log2_shift(float k, float x, float sgn) {
   return fmaf(sgn, (float)e, k) + sgn * __builtin_amdgcn_logf(m);
 }
 
-// Kernel values along a row.  With u = u0 + k du at the row's k-th pixel,
+// Kernel values along a row.  With u = u0 + k du at the walk's k-th pixel,
 //   K(k) = 2^(lsc - u^2) = 2^(lsc - u0^2 + e1 k) G(k),
 //   e1 = -2 u0 du,  G(k) = 2^(-(k du)^2),
 // the first factor is a geometric sequence in k for each lane and G(k) is
@@ -270,12 +270,12 @@ struct Chain {
   float b4;
 };
 
-__device__ __forceinline__ Chain chain_start(const RowSeg& s, float du, float e_add) {
+__device__ __forceinline__ Chain chain_start(float u0, float lsc, float du, float e_add) {
   // two exponentials: the first pixel's value and the per-pixel ratio b1;
   // the other starts and b4 = b1^4 by multiplication
   Chain c;
-  const float e1 = -2.0f * s.u0 * du;
-  const float e0 = fmaf(-s.u0, s.u0, s.lsc) + e_add;
+  const float e1 = -2.0f * u0 * du;
+  const float e0 = fmaf(-u0, u0, lsc) + e_add;
   const float b1 = __builtin_amdgcn_exp2f(e1);
   c.p[0] = __builtin_amdgcn_exp2f(e0);
 #pragma unroll
@@ -294,43 +294,43 @@ __device__ __forceinline__ void read4(const float* rp, float (&rv)[kUnroll]) {
   for (int u = 0; u < kUnroll; ++u) rv[u] = rp[u * HP];
 }
 
-// Pass 1 over one row group: the lane's full groups of four pixels (kernel
-// values from the chain), then its last len % 4 pixels at its own offset
-// (kernel values evaluated directly).  A lane whose full groups are done
-// re-reads its last one (so no read leaves the window) with its chain set
-// to zero, which drops those pixels from every sum without a mask.
+// Pass 1 over one row group.  A lane walks its row in ng = ceil(len / 4)
+// groups of four pixels that end at the row's last pixel: the first group
+// starts pad = 4 ng - len (0 to 3) pixels before the row, and those pixels
+// are dropped by zeroing their rho values (the window has spare zero columns
+// in front, so the reads stay inside it).  Every kernel value then comes from
+// the chain.  A lane whose groups are done re-reads its last one (so no read
+// leaves the window) with its chain set to zero, which drops those pixels
+// from every sum without a mask.
 // s0 += sum K rho (two_sum error into comp when COMP); SIGMA: s1 += sum K
-// rho u^2, from the moments M1 = sum k K rho and M2 = sum k^2 K rho of the
-// pixel index k, combined per row as u0^2 S + 2 u0 du M1 + du^2 M2.  In the
-// full groups the chain value times rho is weighted by G(k), k G(k) and
-// k^2 G(k), all three from tables through the scalar cache: three FMAs per
-// pixel give S, M1 and M2.
-template <int HP, bool COMP, bool SIGMA>
+// rho u^2, from the moments M1 = sum j K rho and M2 = sum j^2 K rho of the
+// walk index j, combined per row as u0^2 S + 2 u0 du M1 + du^2 M2 with u0 at
+// the walk's first pixel.  The chain value times rho is weighted by G(j),
+// j G(j) and j^2 G(j), all three from tables through the scalar cache: three
+// FMAs per pixel give S, M1 and M2.
+template <int HP, bool COMP, bool SIGMA, int UNR>
 __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* gtab,
-                                             ConstF* mtab, const float* rho_w, float& s0,
+                                             ConstF* mtab, const float* rho0, float& s0,
                                              float& comp, float& s1) {
-  const float* rp = rho_w + s.w0;
-  const int nfull = s.len >> 2;
-  const int lastg = max(nfull - 1, 0);
-  const int maxfull = uniform(wave_max_i(nfull));
+  const int ng = (s.len + kUnroll - 1) >> 2;
+  const int pad = ng * kUnroll - s.len;
+  const float u0 = fmaf(-(float)pad, du, s.u0);
+  // an empty row reads the front spare columns
+  const float* rp = s.len > 0 ? rho0 + (s.w0 - pad * HP) : rho0 - kSpareCols * HP;
+  const int lastg = max(ng - 1, 0);
+  const int maxng = uniform(wave_max_i(ng));
   float sum0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
-  auto add = [&](float t0) {
-    if constexpr (COMP) {
-      float sum, err;
-      two_sum(sum0, t0, sum, err);
-      sum0 = sum;
-      comp += err;
-    } else {
-      sum0 += t0;
-    }
-  };
-  Chain c = chain_start(s, du, 0.0f);
+  Chain c = chain_start(u0, s.lsc, du, 0.0f);
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) c.p[u] = nfull > 0 ? c.p[u] : 0.0f;
-  for (int gk = 0; gk < maxfull; ++gk) {
+  for (int u = 0; u < kUnroll; ++u) c.p[u] = ng > 0 ? c.p[u] : 0.0f;
+  auto group = [&](int gk, bool first) {
     float rv[kUnroll];
     read4<HP>(rp + min(gk, lastg) * (kUnroll * HP), rv);
-    const float b4 = gk + 1 < nfull ? c.b4 : 0.0f;
+    if (first) {
+#pragma unroll
+      for (int u = 0; u < kUnroll - 1; ++u) rv[u] = u < pad ? 0.0f : rv[u];
+    }
+    const float b4 = gk + 1 < ng ? c.b4 : 0.0f;
     // COMP sums the group before its two-sum; otherwise every pixel goes
     // straight into the row sum
     float t0 = 0.0f;
@@ -348,34 +348,21 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* 
       }
       c.p[u] *= b4;
     }
-    if constexpr (COMP) add(t0);
-  }
-  const int rem = s.len & (kUnroll - 1);
-  if (__any(rem > 0)) {
-    const int t = nfull * kUnroll;
-    float rv[kUnroll];
-    read4<HP>(rp + t * HP, rv);
-    float t0 = 0.0f;
-#pragma unroll
-    for (int u = 0; u < kUnroll - 1; ++u) {
-      const float kf = (float)(t + u);
-      const float uu = fmaf(kf, du, s.u0);
-      // pixels past the row are dropped by selecting the product, so
-      // whatever the window holds there cannot reach the sums
-      float kr = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * rv[u];
-      kr = (u < rem) ? kr : 0.0f;
-      t0 += kr;
-      if constexpr (SIGMA) {
-        m1 = fmaf(kr, kf, m1);
-        m2 = fmaf(kr * kf, kf, m2);
-      }
+    if constexpr (COMP) {
+      float sum, err;
+      two_sum(sum0, t0, sum, err);
+      sum0 = sum;
+      comp += err;
     }
-    add(t0);
-  }
+  };
+  if (maxng > 0) group(0, true);
+  // unrolled UNR times: one scalar load and one wait serve UNR groups
+#pragma unroll UNR
+  for (int gk = 1; gk < maxng; ++gk) group(gk, false);
   s0 += sum0;
   if constexpr (SIGMA) {
-    const float a1 = 2.0f * s.u0 * du, a2 = du * du;
-    s1 += fmaf(a2, m2, fmaf(a1, m1, s.u0 * s.u0 * sum0));
+    const float a1 = 2.0f * u0 * du, a2 = du * du;
+    s1 += fmaf(a2, m2, fmaf(a1, m1, u0 * u0 * sum0));
   }
 }
 
@@ -431,42 +418,33 @@ __device__ __forceinline__ void set_pair(unsigned long long& v, unsigned lo, uns
     v = ((unsigned long long)hi << 32) | lo;
 }
 
-// Pass 2 over one row group: the lane's full groups of four pixels under
-// one mask per group, then its last len % 4 pixels at its own offset.
+// Pass 2 over one row group, in the groups of pass 1 (ng groups of four
+// ending at the row's last pixel): the first group's pixels before the row
+// are skipped by a mask per pixel, the others go under one mask per group.
 // lw = log2 of the event multiplier in fixed-point units.  FOLD: the
-// multiplier rides in the kernel values (chain and direct); otherwise (MLEM
-// events whose folded values could overflow a float) x = (K rho / lambda)
-// 2^bits is formed per pixel without the large multiplier.  SPLIT: values
-// reach 2^62 and carry a further 2^-32 until converted.  RPI (with FOLD and
-// without SPLIT): values below 2^31, rounded by the conversion in the full
-// groups, whose values come from a multiplication.  The conversion is inline
-// assembly, and the compiler does not insert the wait state that an
-// inline-assembly reader of a transcendental result needs on gfx950, so the
-// tail (values straight from v_exp_f32) keeps the add-and-truncate form.
+// multiplier rides in the chain's kernel values; otherwise (MLEM events whose
+// folded values could overflow a float) x = (K rho / lambda) 2^bits is formed
+// per pixel from a direct exponential, without the large multiplier.  SPLIT:
+// values reach 2^62 and carry a further 2^-32 until converted.  RPI (with
+// FOLD and without SPLIT): values below 2^31, rounded by the conversion
+// itself.  The conversion is inline assembly, which the compiler does not
+// separate from a transcendental result by the wait state gfx950 needs; the
+// values it reads come from a multiplication by the table.
 template <int HP, bool RHO, bool FOLD, bool SPLIT, bool SUB, bool RPI = false>
 __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, float lw,
                                                  float lam, float unit_scale, ConstF* gtab,
-                                                 const float* rho_w,
+                                                 const float* rho0,
                                                  unsigned long long* acc_w) {
-  const float* rp = rho_w + s.w0;
-  unsigned long long* ap = acc_w + s.w0;
-  const float uo[kUnroll] = {s.u0, s.u0 + du, fmaf(2.0f, du, s.u0), fmaf(3.0f, du, s.u0)};
-  const int nfull = s.len >> 2;
-  const int lastg = max(nfull - 1, 0);
-  const int maxfull = uniform(wave_max_i(nfull));
+  static_assert(!RPI || (FOLD && !SPLIT), "the rounding conversion reads chain values");
+  const int ng = (s.len + kUnroll - 1) >> 2;
+  const int pad = ng * kUnroll - s.len;
+  const float u0 = fmaf(-(float)pad, du, s.u0);
+  const int w0 = s.w0 - pad * HP;
+  const float* rp = s.len > 0 ? rho0 + w0 : rho0 - kSpareCols * HP;
+  const int lastg = max(ng - 1, 0);
+  const int maxng = uniform(wave_max_i(ng));
   const float e_add = SPLIT ? lw - 32.0f : lw;
-  const float lscw = s.lsc + e_add;
-  // direct value of pixel k of the row (tail, and MLEM without folding)
-  auto direct = [&](float kf, int u, float rho) {
-    const float uu = fmaf(kf, du, uo[u]);
-    if constexpr (FOLD) {
-      return fixed_arg<RHO, SPLIT>(__builtin_amdgcn_exp2f(fmaf(-uu, uu, lscw)), rho);
-    } else {
-      // unit_scale = 2^bits (2^(bits - 32) when SPLIT)
-      const float kr = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * rho;
-      return SPLIT ? (kr / lam) * unit_scale : fmaf(kr / lam, unit_scale, 0.5f);
-    }
-  };
+  const float uo[kUnroll] = {u0, u0 + du, fmaf(2.0f, du, u0), fmaf(3.0f, du, u0)};
   unsigned lo[kUnroll], hi[kUnroll];
   unsigned long long pv[kUnroll];
 #pragma unroll
@@ -474,9 +452,8 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
     pv[u] = 0ull;
     asm volatile("" : "+v"(pv[u]));
   }
-  Chain c = chain_start(s, du, e_add);
-  float kf = 0.0f;
-  for (int gk = 0; gk < maxfull; ++gk, kf += (float)kUnroll) {
+  Chain c = chain_start(u0, s.lsc, du, e_add);
+  auto group = [&](int gk, bool first) {
     float rv[kUnroll] = {0.0f, 0.0f, 0.0f, 0.0f};
     if constexpr (RHO) read4<HP>(rp + min(gk, lastg) * (kUnroll * HP), rv);
 #pragma unroll
@@ -486,29 +463,23 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
         x = fixed_arg<RHO, SPLIT, RPI>(c.p[u] * gtab[gk * kUnroll + u], rv[u]);
         c.p[u] *= c.b4;
       } else {
-        x = direct(kf, u, rv[u]);
+        // unit_scale = 2^bits (2^(bits - 32) when SPLIT)
+        const float uu = fmaf((float)(gk * kUnroll), du, uo[u]);
+        const float kr = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc)) * rv[u];
+        x = SPLIT ? (kr / lam) * unit_scale : fmaf(kr / lam, unit_scale, 0.5f);
       }
       to_fixed<SPLIT, RPI>(x, lo[u], hi[u]);
       set_pair(pv[u], lo[u], hi[u], !SPLIT);
     }
-    if (gk < nfull) {
+    if (gk < ng) {
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
-        lds_accumulate<SUB>(ap + (gk * kUnroll + u) * HP, pv[u]);
+        if (!first || u == kUnroll - 1 || u >= pad)
+          lds_accumulate<SUB>(acc_w + (w0 + (gk * kUnroll + u) * HP), pv[u]);
     }
-  }
-  const int rem = s.len & (kUnroll - 1);
-  if (__any(rem > 0)) {
-    const int t0 = nfull * kUnroll;
-    float rv[kUnroll] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if constexpr (RHO) read4<HP>(rp + t0 * HP, rv);
-#pragma unroll
-    for (int u = 0; u < kUnroll - 1; ++u) {
-      to_fixed<SPLIT>(direct((float)t0, u, rv[u]), lo[u], hi[u]);
-      set_pair(pv[u], lo[u], hi[u], !SPLIT);
-      if (u < rem) lds_accumulate<SUB>(ap + (t0 + u) * HP, pv[u]);
-    }
-  }
+  };
+  if (maxng > 0) group(0, true);
+  for (int gk = 1; gk < maxng; ++gk) group(gk, false);
 }
 
 }  // namespace
@@ -534,6 +505,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   constexpr bool kRho = kPass1 || (kPass2 && MODE == MODE_MLEM);  // rho window in LDS
   constexpr bool kSigma = kPass1 && (MODE == MODE_FORWARD || MODE == MODE_GRAD);
   constexpr int kThreads = 64 * kWaves;
+  // pass 1's group loop unrolled four times, except with the sigma sums in
+  // the two-pass kernel, where the extra registers slowed the whole kernel
+  constexpr int kPass1Unroll = (kSigma && BAND == 0) ? 1 : 4;
   using U = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem[];
   const StripConst& sg = a.sg;
@@ -541,11 +515,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   const WorkItem it = a.items[blockIdx.x];
   const int ww = it.ww, wh = it.wh;
   const int npx = ww * HP;
-  // the rho window has four spare columns: the tail step reads four pixels
-  // from the end of a lane's full groups, up to three past its row's end
-  // even when the row has no tail (values discarded)
+  // the rho window has four spare zero columns in front: a walk's first
+  // group starts up to three pixels before its row, and an empty row reads
+  // the four of them (values discarded); rho0 is the window's column 0
   const int nrho = (ww + kSpareCols) * HP;
   float* rho_w = reinterpret_cast<float*>(smem);
+  const float* rho0 = rho_w + kSpareCols * HP;
   U* acc_w = reinterpret_cast<U*>(
       smem + (kRho ? (((size_t)nrho * 4 + 15) & ~(size_t)15) : 0));
   const int wave = uniform(threadIdx.x >> 6);
@@ -555,14 +530,15 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   // that the global reads and atomics of a wave cover 64-byte row segments.
   const int tc_lane = lane & 15, tr_lane = lane >> 4;
   if constexpr (kRho) {
-    const int ntc = (ww + kSpareCols + 15) >> 4, ntr = HP / 4;
+    // tile columns from -16: tile 0 holds the spare columns
+    const int ntc = ((ww + 15) >> 4) + 1, ntr = HP / 4;
     for (int t = wave; t < ntc * ntr; t += kWaves) {
       const int tr = t / ntc;
-      const int c = (t - tr * ntc) * 16 + tc_lane, r = tr * 4 + tr_lane;
-      if (c >= ww + kSpareCols) continue;
+      const int c = (t - tr * ntc) * 16 + tc_lane - 16, r = tr * 4 + tr_lane;
+      if (c < -kSpareCols || c >= ww) continue;
       float v = 0.0f;
-      if (r < wh && c < ww) v = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
-      rho_w[c * HP + r] = v;
+      if (r < wh && c >= 0) v = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
+      rho_w[(c + kSpareCols) * HP + r] = v;
     }
   }
   if constexpr (kPass2) {
@@ -622,7 +598,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
                                   cx1);
           if (gi < kCachedGroups) cache[gi] = s;
           float s0 = 0.0f, s1 = 0.0f;
-          forward_rows<HP, COMP, kSigma>(s, du, gtab, mtab, rho_w, s0, comp, s1);
+          forward_rows<HP, COMP, kSigma, kPass1Unroll>(s, du, gtab, mtab, rho0, s0, comp,
+                                                       s1);
           if constexpr (COMP) {
             float sum, err;
             two_sum(acc, s0, sum, err);
@@ -720,16 +697,16 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
               // 2^-40 units (the deterministic accumulation): values up to 2^40
               if (fold)
                 backproject_rows<HP, true, true, true, false>(s, du, lw, lam, unit_scale,
-                                                              gtab, rho_w, acc_w);
+                                                              gtab, rho0, acc_w);
               else
                 backproject_rows<HP, true, false, true, false>(s, du, lw, lam, unit_scale,
-                                                               gtab, rho_w, acc_w);
+                                                               gtab, rho0, acc_w);
             } else if (fold) {
               backproject_rows<HP, true, true, false, false>(s, du, lw, lam, unit_scale,
-                                                             gtab, rho_w, acc_w);
+                                                             gtab, rho0, acc_w);
             } else {
               backproject_rows<HP, true, false, false, false>(s, du, lw, lam, unit_scale,
-                                                              gtab, rho_w, acc_w);
+                                                              gtab, rho0, acc_w);
             }
           } else if constexpr (MODE == MODE_GRAD) {
             // K / lambda has no bound: rows whose values stay below 2^30.5
@@ -739,10 +716,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             const float vmax = s.lsc + lw;
             if (!__any(vmax > 30.5f)) {
               backproject_rows<HP, false, true, false, false, true>(
-                  s, du, lw, 0.0f, unit_scale, gtab, rho_w, acc_w);
+                  s, du, lw, 0.0f, unit_scale, gtab, rho0, acc_w);
             } else if (!__any(vmax > 52.0f)) {
               backproject_rows<HP, false, true, true, false>(s, du, lw, 0.0f, unit_scale,
-                                                             gtab, rho_w, acc_w);
+                                                             gtab, rho0, acc_w);
             } else if (s.len > 0) {
               const int col0 = it.wx0 + (s.w0 - (row0 + r + it.wx0 * HP)) / HP;
               double* img = reinterpret_cast<double*>(a.acc) +
@@ -757,18 +734,18 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             // |g| K 2^k <= 2^62 / n_events by the choice of k
             if (neg)
               backproject_rows<HP, false, true, true, true>(s, du, lw, 0.0f, unit_scale,
-                                                            gtab, rho_w, acc_w);
+                                                            gtab, rho0, acc_w);
             else
               backproject_rows<HP, false, true, true, false>(s, du, lw, 0.0f, unit_scale,
-                                                             gtab, rho_w, acc_w);
+                                                             gtab, rho0, acc_w);
           } else {
             // |g| K 2^k <= 2^31 by the choice of k
             if (neg)
               backproject_rows<HP, false, true, false, true>(s, du, lw, 0.0f, unit_scale,
-                                                             gtab, rho_w, acc_w);
+                                                             gtab, rho0, acc_w);
             else
               backproject_rows<HP, false, true, false, false>(s, du, lw, 0.0f, unit_scale,
-                                                              gtab, rho_w, acc_w);
+                                                              gtab, rho0, acc_w);
           }
         }
       }
