@@ -128,6 +128,12 @@ typedef struct petstrip_stats {
                                 columns, 0: the per-event kernels       */
   int32_t n_work_items;      /* workgroups of the binned kernel          */
   int64_t lds_bytes;         /* LDS per workgroup of the binned kernel   */
+  int64_t n_window_pixels;   /* binned kernel: sum over work items of their
+                                window pixels, the most global atomics one
+                                pass-2 launch makes (one per nonzero window
+                                pixel); every ellipse pixel's update is an
+                                LDS atomic.  0 for the per-event kernels,
+                                whose updates are all global atomics     */
 } petstrip_stats;
 
 /* Called after an output is ready.  iteration >= 1 for images, -1 for the

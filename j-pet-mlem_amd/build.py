@@ -1,4 +1,5 @@
-"""Builds libpetstrip_hip.so (HIP kernels + C ABI) for gfx950, in-tree.
+"""Builds libpetstrip_hip.so (HIP kernels + C ABI) for gfx950, in-tree, and
+libpetstrip_probe.so (the atomic-throughput probes bench.py reports).
 
 Every HIP source is compiled with -ffp-contract=off: the few fused
 multiply-adds the strip geometry needs are written out explicitly, so the
@@ -14,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpetstrip_hip.so")
+PROBE_LIB = os.path.join(HERE, "libpetstrip_probe.so")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -21,6 +23,7 @@ ARCH = os.environ.get("PETSTRIP_ARCH", "gfx950")
 
 SOURCES = ["strip_kernels.hip", "strip_bins.hip", "strip_fused.hip", "petstrip_capi.hip",
            "phantom.hip"]
+PROBE_SOURCES = ["atomic_probe.hip"]
 
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
@@ -39,22 +42,23 @@ def _run(cmd):
 
 
 def build(verbose=False):
-    """Compiles every source (always) and links the shared library."""
+    """Compiles every source (always) and links the two shared libraries."""
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
     cmds = []
-    for src in SOURCES:
+    objs = {}
+    for src in SOURCES + PROBE_SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        objs.append(obj)
+        objs[src] = obj
         cmds.append([HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(len(cmds), 8)) as ex:
         for r in ex.map(_run, cmds):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
-    tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-          "-lpthread"])
-    os.replace(tmp, LIB)
+    for lib, srcs in ((LIB, SOURCES), (PROBE_LIB, PROBE_SOURCES)):
+        tmp = lib + ".tmp"
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
+              *[objs[s] for s in srcs], "-lpthread"])
+        os.replace(tmp, lib)
     return LIB
 
 

@@ -155,6 +155,7 @@ struct petstrip_handle {
   petstrip::WorkItem* d_items = nullptr;
   size_t cap_items = 0;
   int n_items = 0;
+  int64_t window_px = 0;  // sum over work items of window pixels
   int max_rows = 1;
   size_t lds_bytes = 0;
   int col_stride = 0;  // LDS column stride of the fused windows (0: none fits)
@@ -450,6 +451,8 @@ int prepare_fused(petstrip_handle* h) {
       HIP_TRY(hipStreamSynchronize(h->stream));
     }
   }
+  h->window_px = 0;
+  for (const auto& it : items) h->window_px += (int64_t)it.ww * it.wh;
   h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
                 h->opts.update == PETSTRIP_UPDATE_CPU &&
                 h->opts.kernel == PETSTRIP_KERNEL_GAUSSIAN &&
@@ -971,6 +974,7 @@ int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
   out->fused = h->fused_ok ? (h->banded ? (h->ncb > 1 ? 3 : 2) : 1) : 0;
   out->n_work_items = h->n_items;
   out->lds_bytes = (int64_t)h->lds_bytes;
+  out->n_window_pixels = h->fused_ok ? h->window_px : 0;
   return PETSTRIP_OK;
 }
 

@@ -69,6 +69,7 @@ class CStats(C.Structure):
         ("flop_alg", C.c_double), ("last_iteration_ms", C.c_double),
         ("last_kernel_ms", C.c_double), ("fused", C.c_int32),
         ("n_work_items", C.c_int32), ("lds_bytes", C.c_int64),
+        ("n_window_pixels", C.c_int64),
     ]
 
 
@@ -180,6 +181,39 @@ def device_count():
     c = C.c_int(0)
     _check(lib.petstrip_device_count(C.byref(c)))
     return c.value
+
+
+PROBE_LIB_PATH = os.path.join(HERE, "libpetstrip_probe.so")
+PROBE_GLOBAL_F32, PROBE_GLOBAL_F64, PROBE_GLOBAL_U64, PROBE_LDS_U64 = 0, 1, 2, 3
+_probe_lib = None
+
+
+def load_probe_library(path=PROBE_LIB_PATH):
+    """libpetstrip_probe.so (include/petstrip_probe.h)."""
+    global _probe_lib
+    if _probe_lib is not None:
+        return _probe_lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(path)
+    lib.petstrip_probe_atomics.argtypes = [C.c_int, C.c_int, C.c_int32, C.c_int32,
+                                           C.c_int32, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double)]
+    lib.petstrip_probe_atomics.restype = C.c_int
+    _probe_lib = lib
+    return lib
+
+
+def probe_atomics(kind, width=512, height=512, reps=1024, device=0):
+    """Device-wide rate of one kind of atomic update (PROBE_*), in atomics
+    per second, and the timed launch's milliseconds."""
+    lib = load_probe_library()
+    rate, ms = C.c_double(0.0), C.c_double(0.0)
+    rc = lib.petstrip_probe_atomics(int(device), int(kind), int(width), int(height),
+                                    int(reps), C.byref(rate), C.byref(ms))
+    if rc != 0:
+        raise PetstripError(rc, f"atomic probe failed with HIP error {rc}")
+    return rate.value, ms.value
 
 
 def phantom_events(scanner, regions, n, seed=1, n_threads=None):

@@ -12,8 +12,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "petstrip.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+PROBE_HEADER = os.path.join(ROOT, "include", "petstrip_probe.h")
+
+
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(petstrip_\w+)\s*\(", text, flags=re.M)
     return sorted(set(names))
@@ -32,6 +35,20 @@ def test_library_exports_every_declared_symbol(pkg):
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
     assert lib.petstrip_abi_version() == 1
+
+
+def test_probe_library_exports_every_declared_symbol(pkg):
+    """libpetstrip_probe.so exports what include/petstrip_probe.h declares and
+    rejects bad arguments before touching a device."""
+    import ctypes
+    lib = pkg.petstrip.load_probe_library()
+    names = declared_functions(PROBE_HEADER)
+    assert names == ["petstrip_probe_atomics"]
+    assert all(hasattr(lib, n) for n in names)
+    rate = ctypes.c_double(0.0)
+    assert lib.petstrip_probe_atomics(0, 7, 512, 512, 16, ctypes.byref(rate), None) != 0
+    assert lib.petstrip_probe_atomics(0, 0, 8, 512, 16, ctypes.byref(rate), None) != 0
+    assert lib.petstrip_probe_atomics(0, 0, 512, 512, 0, ctypes.byref(rate), None) != 0
 
 
 def test_errors_are_returned_not_aborted(pkg):
