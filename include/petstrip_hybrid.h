@@ -1,0 +1,132 @@
+/* 3D hybrid reconstruction on the GPU: the strip kernel walked per LOR of a
+ * barrel scanner over a voxel grid (libpetstrip_hip.so).
+ *
+ * Replaces PET3D::Hybrid::GPU::Reconstruction::run
+ * (grey277/j-pet-mlem src/3d/hybrid/cuda/reconstruction.h:31-58,
+ * cuda/reconstruction.cu:31-160), the --gpu path of 3d_hybrid_reconstruction
+ * (src/3d/hybrid/reconstruction_cmd.cpp:596-640).  The iteration is the
+ * reference's CPU one (src/3d/hybrid/reconstruction.h:267-445): per event,
+ * the voxels of its pixel-info range along the LOR times its plane range,
+ * weight = strip kernel / frame sensitivity x LOR-pixel weight x rho,
+ * denominator = sum of weight x sensitivity, rho' = sum over events of
+ * weight / denominator.  The host-side preparation the reference does before
+ * its GPU call is available here too: the LOR-pixel weights and sensitivity
+ * (calculate_weight, calculate_sensitivity, normalize_geometry_weights,
+ * reconstruction.h:212-265) on the host, the frame events (translate_to_frame
+ * and the grid filter, reconstruction.h:97-161) on the device.
+ *
+ * Errors: every call returns 0 or a negative PETSTRIP_E* code with the
+ * message in petstrip_last_error() (include/petstrip.h). */
+#ifndef PETSTRIP_HYBRID_H
+#define PETSTRIP_HYBRID_H
+
+#include <stdint.h>
+
+#include "petstrip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The reference's GeometrySOA (src/2d/barrel/geometry_soa.h:22-48,178-199)
+ * and voxel grid (src/3d/geometry/voxel_grid.h).  LOR (first, second) with
+ * first >= second has index first (first + 1) / 2 + second; its pixel infos
+ * are [info_begin, info_end) with positions ascending.  The arrays are copied
+ * at create. */
+typedef struct petstrip_hybrid_geometry {
+  int32_t n_detectors;
+  int32_t n_columns, n_rows, n_planes;
+  double pixel_size;
+  double lower_left_x, lower_left_y; /* the pixel grid's lower-left corner */
+  double z_left;                     /* z of the first plane's lower face   */
+  int64_t n_pixel_infos;
+  const float* segments;    /* n_lors x 4: start x, y, end x, y             */
+  const float* widths;      /* n_lors                                        */
+  const int64_t* info_begin;/* n_lors                                        */
+  const int64_t* info_end;  /* n_lors                                        */
+  const int16_t* pixels;    /* n_pixel_infos x 2: x, y                        */
+  const float* positions;   /* n_pixel_infos: t along the LOR (0 at start)   */
+  const float* weights;     /* n_pixel_infos, read with
+                               PETSTRIP_HYBRID_GEOMETRY_WEIGHTS (may be NULL
+                               otherwise)                                    */
+} petstrip_hybrid_geometry;
+
+enum {
+  /* keep the geometry's weights (the reference's --system matrices) instead
+   * of the Gaussian in the distance to the LOR (calculate_weight) */
+  PETSTRIP_HYBRID_GEOMETRY_WEIGHTS = 1,
+  /* sensitivity one (--sens-to-one) instead of the sum of the weights */
+  PETSTRIP_HYBRID_SENSITIVITY_ONE = 2
+};
+
+/* A frame event (reconstruction.h:53-63).  end may exceed the LOR's last
+ * pixel info by one, as the reference's binary search leaves it
+ * (reconstruction.h:133-138); it is clamped to n_pixel_infos. */
+typedef struct petstrip_hybrid_event {
+  int16_t first, second;
+  float up, right, tan, sec;
+  int64_t begin, end;
+  int16_t plane_begin, plane_end;
+} petstrip_hybrid_event;
+
+typedef struct petstrip_hybrid_stats {
+  int64_t n_events;          /* frame events kept                          */
+  int64_t n_voxels;          /* sum over events of pixel range x planes    */
+  double last_iteration_ms;  /* device time of the last iteration          */
+  double last_kernel_ms;     /* device time of its event kernel            */
+} petstrip_hybrid_stats;
+
+typedef struct petstrip_hybrid petstrip_hybrid;
+
+/* Copies the geometry, computes the LOR-pixel weights and the W x H
+ * sensitivity map as flags say (with the reference's float expressions, on
+ * the host), allocates the voxel image (rho = 1).  sigma_z, sigma_dl: the
+ * strip kernel's widths; length: the scintillator length of the frame
+ * sensitivity. */
+int petstrip_hybrid_create(const petstrip_hybrid_geometry* geometry, double sigma_z,
+                           double sigma_dl, double length, int32_t flags, int32_t device,
+                           petstrip_hybrid** out);
+void petstrip_hybrid_destroy(petstrip_hybrid* h);
+
+/* Responses (LOR, z_up, z_dn, dl): lors n x 2 (first, second, first >
+ * second), zzd n x 3.  Frame events are formed and filtered on the device
+ * and kept in input order; previous events are replaced. */
+int petstrip_hybrid_set_responses(petstrip_hybrid* h, const int16_t* lors, const float* zzd,
+                                  int64_t n);
+/* Frame events formed by the caller (the reference's host code). */
+int petstrip_hybrid_set_events(petstrip_hybrid* h, const petstrip_hybrid_event* events,
+                               int64_t n);
+int petstrip_hybrid_get_events(petstrip_hybrid* h, petstrip_hybrid_event* events);
+
+/* Replaces the weights (n_pixel_infos) and/or the sensitivity map (rows x
+ * columns), e.g. normalised by the caller; either may be NULL. */
+int petstrip_hybrid_set_weights(petstrip_hybrid* h, const float* weights,
+                                const float* sensitivity);
+int petstrip_hybrid_get_weights(petstrip_hybrid* h, float* weights, float* sensitivity);
+
+/* The voxel image, planes x rows x columns (voxel index x + W (y + H z)). */
+int petstrip_hybrid_set_rho(petstrip_hybrid* h, const float* rho);
+int petstrip_hybrid_get_rho(petstrip_hybrid* h, float* rho);
+
+/* n MLEM iterations. */
+int petstrip_hybrid_iterate(petstrip_hybrid* h, int32_t n);
+
+/* The reference's run(): n_blocks x n_iterations_in_block iterations from
+ * rho_inout, skipping those below start_iteration; output(iteration,
+ * image) after every block (image: planes x rows x columns, valid during the
+ * call; output(-1, sensitivity) first with the W x H map), progress after
+ * every iteration (cuda/reconstruction.cu:92-156). */
+typedef int (*petstrip_hybrid_output_cb)(void* user, int iteration, const float* image,
+                                         int32_t width, int32_t height, int32_t depth);
+int petstrip_hybrid_run(petstrip_hybrid* h, float* rho_inout, int32_t n_blocks,
+                        int32_t n_iterations_in_block, int32_t start_iteration,
+                        petstrip_hybrid_output_cb output, petstrip_progress_cb progress,
+                        void* user);
+
+int petstrip_hybrid_stats_get(petstrip_hybrid* h, petstrip_hybrid_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PETSTRIP_HYBRID_H */

@@ -1,0 +1,195 @@
+"""Python host side of the 3D hybrid reconstruction (include/petstrip_hybrid.h).
+
+HybridReconstruction mirrors the reference's PET3D::Hybrid::Reconstruction
+as 3d_hybrid_reconstruction drives it (src/3d/hybrid/reconstruction_cmd.cpp:
+456-640): construct from the geometry and the scanner's sigmas and length
+(weights and sensitivity are formed then), add responses (frame events), call
+it for n iterations, read rho (planes x rows x columns).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import petstrip as _ps
+from .hybrid_geometry import HybridGeometry, cylinder_responses, ring_geometry  # noqa: F401
+
+GEOMETRY_WEIGHTS = 1
+SENSITIVITY_ONE = 2
+
+EVENT = np.dtype([("first", np.int16), ("second", np.int16), ("up", np.float32),
+                  ("right", np.float32), ("tan", np.float32), ("sec", np.float32),
+                  ("begin", np.int64), ("end", np.int64),
+                  ("plane_begin", np.int16), ("plane_end", np.int16)], align=True)
+
+
+class CHybridGeometry(C.Structure):
+    _fields_ = [
+        ("n_detectors", C.c_int32), ("n_columns", C.c_int32), ("n_rows", C.c_int32),
+        ("n_planes", C.c_int32), ("pixel_size", C.c_double), ("lower_left_x", C.c_double),
+        ("lower_left_y", C.c_double), ("z_left", C.c_double), ("n_pixel_infos", C.c_int64),
+        ("segments", C.c_void_p), ("widths", C.c_void_p), ("info_begin", C.c_void_p),
+        ("info_end", C.c_void_p), ("pixels", C.c_void_p), ("positions", C.c_void_p),
+        ("weights", C.c_void_p),
+    ]
+
+
+class CHybridStats(C.Structure):
+    _fields_ = [("n_events", C.c_int64), ("n_voxels", C.c_int64),
+                ("last_iteration_ms", C.c_double), ("last_kernel_ms", C.c_double)]
+
+
+OUTPUT_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int32,
+                        C.c_int32, C.c_int32)
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    lib = _ps.load_library()
+    if not _bound:
+        vp, i64, i32, d = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+        sig = {
+            "petstrip_hybrid_create": ([C.POINTER(CHybridGeometry), d, d, d, i32, i32,
+                                        C.POINTER(vp)], C.c_int),
+            "petstrip_hybrid_destroy": ([vp], None),
+            "petstrip_hybrid_set_responses": ([vp, vp, vp, i64], C.c_int),
+            "petstrip_hybrid_set_events": ([vp, vp, i64], C.c_int),
+            "petstrip_hybrid_get_events": ([vp, vp], C.c_int),
+            "petstrip_hybrid_set_weights": ([vp, vp, vp], C.c_int),
+            "petstrip_hybrid_get_weights": ([vp, vp, vp], C.c_int),
+            "petstrip_hybrid_set_rho": ([vp, vp], C.c_int),
+            "petstrip_hybrid_get_rho": ([vp, vp], C.c_int),
+            "petstrip_hybrid_iterate": ([vp, i32], C.c_int),
+            "petstrip_hybrid_run": ([vp, vp, i32, i32, i32, OUTPUT_CB, _ps.PROGRESS_CB, vp],
+                                    C.c_int),
+            "petstrip_hybrid_stats_get": ([vp, C.POINTER(CHybridStats)], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _bound = True
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise _ps.PetstripError(rc, _ps.load_library().petstrip_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class HybridReconstruction:
+    """The 3D hybrid reconstruction on one GPU (libpetstrip_hip.so).
+
+    geometry: a HybridGeometry (GeometrySOA arrays and the voxel grid).
+    flags: GEOMETRY_WEIGHTS keeps the geometry's weights, SENSITIVITY_ONE sets
+    the sensitivity to one (the reference's --sens-to-one).
+    """
+
+    def __init__(self, geometry, sigma_z, sigma_dl, length, flags=0, device=0):
+        lib = _lib()
+        g = geometry
+        self.geometry = g
+        self._keep = [g.segments, g.widths, g.info_begin, g.info_end, g.pixels, g.positions,
+                      g.weights]
+        cg = CHybridGeometry(g.n_detectors, g.n_columns, g.n_rows, g.n_planes, g.pixel_size,
+                             g.lower_left[0], g.lower_left[1], g.z_left, g.n_pixel_infos,
+                             _p(g.segments), _p(g.widths), _p(g.info_begin), _p(g.info_end),
+                             _p(g.pixels), _p(g.positions), _p(g.weights))
+        self._h = C.c_void_p()
+        _check(lib.petstrip_hybrid_create(C.byref(cg), sigma_z, sigma_dl, length, int(flags),
+                                          int(device), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            _lib().petstrip_hybrid_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        g = self.geometry
+        return (g.n_planes, g.n_rows, g.n_columns)
+
+    def set_responses(self, lors, zzd):
+        lors = np.ascontiguousarray(lors, np.int16).reshape(-1, 2)
+        zzd = np.ascontiguousarray(zzd, np.float32).reshape(-1, 3)
+        if len(lors) != len(zzd):
+            raise ValueError("lors and zzd differ in length")
+        _check(_lib().petstrip_hybrid_set_responses(self._h, _p(lors), _p(zzd), len(lors)))
+
+    def set_events(self, events):
+        events = np.ascontiguousarray(events, EVENT)
+        _check(_lib().petstrip_hybrid_set_events(self._h, _p(events), len(events)))
+
+    def events(self):
+        out = np.zeros(self.stats()["n_events"], EVENT)
+        _check(_lib().petstrip_hybrid_get_events(self._h, _p(out)))
+        return out
+
+    def weights(self):
+        w = np.zeros(self.geometry.n_pixel_infos, np.float32)
+        s = np.zeros(self.shape[1:], np.float32)
+        _check(_lib().petstrip_hybrid_get_weights(self._h, _p(w), _p(s)))
+        return w, s
+
+    def set_weights(self, weights=None, sensitivity=None):
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        s = None if sensitivity is None else np.ascontiguousarray(sensitivity, np.float32)
+        _check(_lib().petstrip_hybrid_set_weights(self._h, _p(w), _p(s)))
+
+    @property
+    def rho(self):
+        out = np.zeros(self.shape, np.float32)
+        _check(_lib().petstrip_hybrid_get_rho(self._h, _p(out)))
+        return out
+
+    @rho.setter
+    def rho(self, value):
+        v = np.ascontiguousarray(value, np.float32).reshape(self.shape)
+        _check(_lib().petstrip_hybrid_set_rho(self._h, _p(v)))
+
+    def __call__(self, n=1):
+        _check(_lib().petstrip_hybrid_iterate(self._h, int(n)))
+        return self
+
+    def run(self, n_blocks, n_iterations_in_block, rho=None, start_iteration=0,
+            output=None, progress=None):
+        """The reference's block schedule: output(iteration, image) after
+        every block (and output(-1, sensitivity) first)."""
+        img = np.ones(self.shape, np.float32) if rho is None else \
+            np.ascontiguousarray(rho, np.float32).reshape(self.shape).copy()
+
+        def on_out(_u, it, ptr, w, h, d):
+            if output is not None:
+                output(it, np.ctypeslib.as_array(ptr, (d, h, w)).copy())
+            return 0
+
+        def on_prog(_u, done, fin):
+            if progress is not None:
+                progress(done, bool(fin))
+
+        cb_o, cb_p = OUTPUT_CB(on_out), _ps.PROGRESS_CB(on_prog)
+        _check(_lib().petstrip_hybrid_run(self._h, _p(img), n_blocks, n_iterations_in_block,
+                                          start_iteration, cb_o, cb_p, None))
+        return img
+
+    def stats(self):
+        s = CHybridStats()
+        _check(_lib().petstrip_hybrid_stats_get(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in CHybridStats._fields_}

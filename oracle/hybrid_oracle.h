@@ -1,0 +1,76 @@
+/* CPU oracle of the 3D hybrid reconstruction (test infrastructure only:
+ * tests/, smoke() and bench.py's CPU leg call it as the checker; nothing in
+ * the product links it).  A restatement of PET3D::Hybrid::Reconstruction
+ * (grey277/j-pet-mlem src/3d/hybrid/reconstruction.h) in its default mode:
+ * LOR-pixel weights from the Gaussian in the distance to the LOR, the 2D
+ * sensitivity map from those weights (or one), normalised weights, frame
+ * events from (LOR, z_up, z_dn, dl) responses, and the list-mode MLEM
+ * iteration over the voxels of each event's pixel range and plane range.
+ * The FP32 functions follow the reference's float expressions without
+ * contraction (the reference harness is built with -ffp-contract=off) and
+ * are bitwise equal to it; the FP64 twin computes the same in doubles. */
+#ifndef HYBRID_ORACLE_H
+#define HYBRID_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* GeometrySOA arrays (src/2d/barrel/geometry_soa.h): per LOR index
+ * first (first + 1) / 2 + second: segment (start x, y, end x, y), width,
+ * pixel-info range; per pixel info: pixel (x, y), position t along the LOR.
+ * The voxel grid: n_columns x n_rows pixels of pixel_size from the lower
+ * left corner, n_planes planes from z_left. */
+typedef struct {
+  int32_t n_detectors, n_columns, n_rows, n_planes;
+  double pixel_size, lower_left_x, lower_left_y, z_left;
+  int64_t n_pixel_infos;
+  const float* segments;
+  const float* widths;
+  const int64_t* info_begin;
+  const int64_t* info_end;
+  const int16_t* pixels;
+  const float* positions;
+} oracle_hybrid_geometry;
+
+/* A frame event (reconstruction.h:53-63). */
+typedef struct {
+  int16_t first, second;
+  float up, right, tan, sec;
+  int64_t begin, end;
+  int16_t plane_begin, plane_end;
+} oracle_hybrid_event;
+
+/* calculate_weight, calculate_sensitivity (or set_sensitivity_to_one) and
+ * normalize_geometry_weights (reconstruction.h:212-265): weights
+ * (n_pixel_infos) and the W x H sensitivity map. */
+void oracle_hybrid_weights_f32(const oracle_hybrid_geometry* g, int sensitivity_one,
+                               float* weights, float* sensitivity);
+void oracle_hybrid_weights_f64(const oracle_hybrid_geometry* g, int sensitivity_one,
+                               double* weights, double* sensitivity);
+
+/* translate_to_frame and the grid filter (reconstruction.h:97-161) for n
+ * responses (lors: first, second; zzd: z_up, z_dn, dl); returns the number of
+ * events kept, written to out in order. */
+int64_t oracle_hybrid_frames_f32(const oracle_hybrid_geometry* g, double sigma_z,
+                                 double sigma_dl, const float* weights, const int16_t* lors,
+                                 const float* zzd, int64_t n, oracle_hybrid_event* out);
+
+/* n_iter iterations (reconstruction.h:267-445) from rho (W x H x planes);
+ * the image after every iteration goes to out (n_iter x voxels). */
+void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
+                            double sigma_dl, double length, const float* weights,
+                            const float* sensitivity, const oracle_hybrid_event* ev,
+                            int64_t n, int n_iter, const float* rho, float* out);
+void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
+                            double sigma_dl, double length, const double* weights,
+                            const double* sensitivity, const oracle_hybrid_event* ev,
+                            int64_t n, int n_iter, const double* rho, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYBRID_ORACLE_H */

@@ -1,0 +1,135 @@
+// Harness around the reference's own 3D hybrid reconstruction
+// (PET3D::Hybrid::Reconstruction, src/3d/hybrid/reconstruction.h), built
+// from the reference tree by oracle/Makefile into _ref/libref_hybrid.so.
+// Test infrastructure: the tests call it to pin the C restatement in
+// hybrid_oracle.c; nothing in the product links it.
+//
+// The geometry is handed over as the arrays of a GeometrySOA (one LOR
+// segment, width and pixel-info range per LOR; pixel, position and weight
+// per pixel info); the harness builds the reference's Geometry from them,
+// runs calculate_weight / calculate_sensitivity (or set_sensitivity_to_one)
+// and normalize_geometry_weights as 3d_hybrid_reconstruction does
+// (reconstruction_cmd.cpp:500-521), loads the responses through the
+// reference's text stream reader (translate_to_frame and the grid filter) and
+// runs the iterations with one OpenMP thread, so the accumulation order is
+// the reference's sequential event order.
+#include <cstdint>
+#include <cstring>
+#include <sstream>
+#include <vector>
+
+#if _OPENMP
+#include <omp.h>
+#endif
+
+#include "2d/barrel/generic_scanner.h"
+#include "2d/barrel/geometry.h"
+#include "2d/barrel/scanner_builder.h"
+#include "2d/barrel/square_detector.h"
+#include "2d/strip/gaussian_kernel.h"
+#include "3d/hybrid/reconstruction.h"
+#include "3d/hybrid/scanner.h"
+
+namespace {
+using F = float;
+using S = short;
+using Scanner2D = PET2D::Barrel::GenericScanner<PET2D::Barrel::SquareDetector<F>, S>;
+using Scanner = PET3D::Hybrid::Scanner<Scanner2D>;
+using Kernel = PET2D::Strip::GaussianKernel<F>;
+using Reconstruction = PET3D::Hybrid::Reconstruction<Scanner, Kernel>;
+using Geometry = PET2D::Barrel::Geometry<F, S>;
+using LOR = PET2D::Barrel::LOR<S>;
+using Point2D = PET2D::Point<F>;
+}  // namespace
+
+extern "C" {
+
+// Frame event fields out: lor first, second, up, right, tan, sec, pixel
+// info begin, end, plane begin, end (10 doubles per kept event).
+int ref_hybrid_run(int n_detectors, int n_columns, int n_rows, float pixel_size,
+                   float lower_left_x, float lower_left_y, float z_left, int n_planes,
+                   const float* segments, const float* widths, const int64_t* info_begin,
+                   const int64_t* info_end, const int16_t* pixels, const float* positions,
+                   const float* weights, float sigma_z, float sigma_dl, float length,
+                   int sensitivity_one, const int16_t* lors, const float* zzd, int64_t n_events,
+                   int n_iterations, float* out_weights, float* out_sensitivity,
+                   int64_t* out_n_kept, double* out_frames, float* out_rho) {
+  try {
+#if _OPENMP
+    omp_set_num_threads(1);
+#endif
+    Geometry::Grid grid(n_columns, n_rows, pixel_size, Point2D(lower_left_x, lower_left_y));
+    Geometry geometry(n_detectors, grid);
+    for (S first = 0; first < n_detectors; ++first) {
+      for (S second = 0; second <= first; ++second) {
+        LOR lor(first, second);
+        const auto i = lor.index();
+        auto& lg = geometry[lor];
+        lg.lor = lor;
+        lg.segment = PET2D::LineSegment<F>(Point2D(segments[4 * i], segments[4 * i + 1]),
+                                           Point2D(segments[4 * i + 2], segments[4 * i + 3]));
+        lg.width = widths[i];
+        lg.pixel_infos.clear();
+        for (int64_t k = info_begin[i]; k < info_end[i]; ++k) {
+          Geometry::PixelInfo pi;
+          pi.pixel = PET2D::Pixel<S>(pixels[2 * k], pixels[2 * k + 1]);
+          pi.t = positions[k];
+          pi.distance = 0;
+          pi.fill = 0;
+          pi.weight = weights[k];
+          lg.pixel_infos.push_back(pi);
+        }
+      }
+    }
+    Reconstruction::Geometry soa(geometry, 1);
+    Scanner2D barrel = PET2D::Barrel::ScannerBuilder<Scanner2D>::build_single_ring(
+        F(1), 4, F(0.1), F(0.1));
+    Scanner scanner(barrel, length);
+    scanner.set_sigmas(sigma_z, sigma_dl);
+    Reconstruction::Grid grid3d(grid, z_left, (S)n_planes);
+    Reconstruction rec(scanner, grid3d, soa, false);
+    rec.calculate_weight();
+    if (sensitivity_one)
+      rec.set_sensitivity_to_one();
+    else
+      rec.calculate_sensitivity();
+    rec.normalize_geometry_weights();
+    std::memcpy(out_weights, soa.pixel_weights, sizeof(F) * soa.n_pixel_infos);
+    std::memcpy(out_sensitivity, rec.sensitivity.data,
+                sizeof(F) * (size_t)n_columns * n_rows);
+    // responses through the reference's stream reader (text: "first second
+    // z_up z_dn dl", nine significant digits, which read back to the same
+    // floats)
+    std::stringstream text;
+    text.precision(9);
+    for (int64_t e = 0; e < n_events; ++e)
+      text << lors[2 * e] << ' ' << lors[2 * e + 1] << ' ' << zzd[3 * e] << ' '
+           << zzd[3 * e + 1] << ' ' << zzd[3 * e + 2] << '\n';
+    rec << text;
+    *out_n_kept = rec.n_events();
+    for (int i = 0; i < rec.n_events(); ++i) {
+      const auto ev = rec.frame_event(i);
+      double* o = out_frames + 10 * (size_t)i;
+      o[0] = ev.lor.first;
+      o[1] = ev.lor.second;
+      o[2] = ev.up;
+      o[3] = ev.right;
+      o[4] = ev.tan;
+      o[5] = ev.sec;
+      o[6] = (double)ev.pixel_info_begin;
+      o[7] = (double)ev.pixel_info_end;
+      o[8] = ev.plane_begin;
+      o[9] = ev.plane_end;
+    }
+    const size_t n_vox = (size_t)n_columns * n_rows * n_planes;
+    for (int it = 0; it < n_iterations; ++it) {
+      rec();
+      std::memcpy(out_rho + (size_t)it * n_vox, rec.rho.data, sizeof(F) * n_vox);
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+}  // extern "C"

@@ -215,3 +215,120 @@ def ref_mlem(sc, events, n_iter, n_threads=1):
                      sc.pixel_width, sc.sigma_z, sc.sigma_dl, P(ev), ev.shape[0],
                      P(rho), n_iter, n_threads, P(iters))
     return iters.reshape(n_iter, sc.n_y_pixels, sc.n_z_pixels)
+
+
+# ---------------------------------------------------------------------------
+# 3D hybrid reconstruction: oracle/hybrid_oracle.c and the reference harness
+# oracle/_ref/libref_hybrid.so (PET3D::Hybrid::Reconstruction).
+REF_HYBRID_SO = os.path.join(ROOT, "oracle", "_ref", "libref_hybrid.so")
+
+HYBRID_EVENT = np.dtype([("first", np.int16), ("second", np.int16), ("up", np.float32),
+                         ("right", np.float32), ("tan", np.float32), ("sec", np.float32),
+                         ("begin", np.int64), ("end", np.int64),
+                         ("plane_begin", np.int16), ("plane_end", np.int16)], align=True)
+
+
+class OHybridGeometry(C.Structure):
+    _fields_ = [
+        ("n_detectors", C.c_int32), ("n_columns", C.c_int32), ("n_rows", C.c_int32),
+        ("n_planes", C.c_int32), ("pixel_size", C.c_double), ("lower_left_x", C.c_double),
+        ("lower_left_y", C.c_double), ("z_left", C.c_double), ("n_pixel_infos", C.c_int64),
+        ("segments", C.c_void_p), ("widths", C.c_void_p), ("info_begin", C.c_void_p),
+        ("info_end", C.c_void_p), ("pixels", C.c_void_p), ("positions", C.c_void_p),
+    ]
+
+
+def ohybrid(g):
+    return OHybridGeometry(g.n_detectors, g.n_columns, g.n_rows, g.n_planes, g.pixel_size,
+                           g.lower_left[0], g.lower_left[1], g.z_left, g.n_pixel_infos,
+                           P(g.segments), P(g.widths), P(g.info_begin), P(g.info_end),
+                           P(g.pixels), P(g.positions))
+
+
+_h = None
+
+
+def _hyb():
+    global _h
+    if _h is None:
+        o = C.CDLL(ORACLE_SO)
+        vp, i64, d = C.c_void_p, C.c_int64, C.c_double
+        G = C.POINTER(OHybridGeometry)
+        o.oracle_hybrid_weights_f32.argtypes = [G, C.c_int, vp, vp]
+        o.oracle_hybrid_weights_f64.argtypes = [G, C.c_int, vp, vp]
+        o.oracle_hybrid_frames_f32.argtypes = [G, d, d, vp, vp, vp, i64, vp]
+        o.oracle_hybrid_frames_f32.restype = i64
+        o.oracle_hybrid_mlem_f32.argtypes = [G, d, d, d, vp, vp, vp, i64, C.c_int, vp, vp]
+        o.oracle_hybrid_mlem_f64.argtypes = [G, d, d, d, vp, vp, vp, i64, C.c_int, vp, vp]
+        _h = o
+    return _h
+
+
+def hybrid_weights(g, sensitivity_one=False, f64=False):
+    """(weights, W x H sensitivity) after normalisation."""
+    t = np.float64 if f64 else np.float32
+    w = np.zeros(g.n_pixel_infos, t)
+    s = np.zeros((g.n_rows, g.n_columns), t)
+    og = ohybrid(g)
+    fn = _hyb().oracle_hybrid_weights_f64 if f64 else _hyb().oracle_hybrid_weights_f32
+    fn(C.byref(og), int(sensitivity_one), P(w), P(s))
+    return w, s
+
+
+def hybrid_frames(g, sigma_z, sigma_dl, weights, lors, zzd):
+    out = np.zeros(len(lors), HYBRID_EVENT)
+    og = ohybrid(g)
+    lors = np.ascontiguousarray(lors, np.int16)
+    zzd = np.ascontiguousarray(zzd, np.float32)
+    k = _hyb().oracle_hybrid_frames_f32(C.byref(og), sigma_z, sigma_dl, P(weights), P(lors),
+                                        P(zzd), len(lors), P(out))
+    return out[:k].copy()
+
+
+def hybrid_mlem(g, sigma_z, sigma_dl, length, weights, sensitivity, events, n_iter,
+                rho0=None, f64=False):
+    """Images after every iteration, (n_iter, planes, rows, columns)."""
+    t = np.float64 if f64 else np.float32
+    shape = (g.n_planes, g.n_rows, g.n_columns)
+    rho = np.ones(shape, t) if rho0 is None else np.ascontiguousarray(rho0, t)
+    out = np.zeros((n_iter,) + shape, t)
+    og = ohybrid(g)
+    events = np.ascontiguousarray(events, HYBRID_EVENT)
+    fn = _hyb().oracle_hybrid_mlem_f64 if f64 else _hyb().oracle_hybrid_mlem_f32
+    fn(C.byref(og), sigma_z, sigma_dl, length, P(np.ascontiguousarray(weights, t)),
+       P(np.ascontiguousarray(sensitivity, t)), P(events), len(events), n_iter, P(rho),
+       P(out))
+    return out
+
+
+def have_ref_hybrid():
+    return os.path.exists(REF_HYBRID_SO)
+
+
+def ref_hybrid(g, sigma_z, sigma_dl, length, lors, zzd, n_iter, sensitivity_one=False):
+    """The reference's own reconstruction on the same geometry and responses:
+    dict of weights, sensitivity, frames (kept events: first, second, up,
+    right, tan, sec, begin, end, plane begin, end) and images per iteration."""
+    r = C.CDLL(REF_HYBRID_SO)
+    vp, i64, f = C.c_void_p, C.c_int64, C.c_float
+    ci = C.c_int
+    r.ref_hybrid_run.argtypes = ([ci, ci, ci, f, f, f, f, ci] + [vp] * 7 +
+                                 [f, f, f, ci, vp, vp, i64, ci, vp, vp, vp, vp, vp])
+    r.ref_hybrid_run.restype = ci
+    lors = np.ascontiguousarray(lors, np.int16)
+    zzd = np.ascontiguousarray(zzd, np.float32)
+    n = len(lors)
+    w = np.zeros(g.n_pixel_infos, np.float32)
+    s = np.zeros((g.n_rows, g.n_columns), np.float32)
+    kept = C.c_int64(0)
+    frames = np.zeros((n, 10), np.float64)
+    rho = np.zeros((n_iter, g.n_planes, g.n_rows, g.n_columns), np.float32)
+    rc = r.ref_hybrid_run(g.n_detectors, g.n_columns, g.n_rows, g.pixel_size,
+                          g.lower_left[0], g.lower_left[1], g.z_left, g.n_planes,
+                          P(g.segments), P(g.widths), P(g.info_begin), P(g.info_end),
+                          P(g.pixels), P(g.positions), P(g.weights), sigma_z, sigma_dl,
+                          length, int(sensitivity_one), P(lors), P(zzd), n, n_iter, P(w),
+                          P(s), C.byref(kept), P(frames), P(rho))
+    if rc != 0:
+        raise RuntimeError("reference hybrid harness failed")
+    return {"weights": w, "sensitivity": s, "frames": frames[:kept.value], "rho": rho}
