@@ -61,6 +61,81 @@ def atomic_report(pkg, st, n_pix, kernel_ms):
     }
 
 
+def hybrid_report(pkg, name, steps, no_cpu, cpu_threads):
+    """The 3D hybrid reconstruction (include/petstrip_hybrid.h) on a named
+    workload: iteration time with the events resident on the device, parity
+    of a 2e4-event prefix against the FP64 oracle after two iterations, and
+    the reference's own iteration (oracle/_ref/libref_hybrid.so, OpenMP) on
+    the same prefix as the CPU baseline."""
+    import numpy as np
+    g, lors, zzd, sz, sdl, length = pkg.hybrid_workload(name)
+    t0 = time.perf_counter()
+    with pkg.HybridReconstruction(g, sz, sdl, length) as rec:
+        t_create = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rec.set_responses(lors, zzd)
+        t_events = time.perf_counter() - t0
+        st = rec.stats()
+        rec(1)
+        wall, kern = [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            rec(1)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            kern.append(rec.stats()["last_kernel_ms"])
+    n_ev = st["n_events"]
+    ms = float(np.median(wall))
+    out = {
+        "workload": name, "voxels": g.n_voxels, "pixel_infos": g.n_pixel_infos,
+        "responses": len(lors), "events": n_ev,
+        "voxels_per_event": st["n_voxels"] / max(n_ev, 1),
+        "ms_per_iteration": ms, "kernel_ms": float(np.mean(kern)),
+        "events_per_s_iter": n_ev / (ms * 1e-3),
+        "voxel_evaluations_per_s": 2 * st["n_voxels"] / (float(np.mean(kern)) * 1e-3),
+        "setup_s": {"weights_and_upload": t_create, "frame_events": t_events},
+    }
+    if no_cpu:
+        return out
+    # the CPU leg: the oracle as the checker, the reference as the baseline
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    try:
+        import oracle_lib as ol
+    except ImportError as e:
+        out["parity"] = {"error": str(e)}
+        return out
+    m = 20000
+    with pkg.HybridReconstruction(g, sz, sdl, length) as rec:
+        rec.set_responses(lors[:m], zzd[:m])
+        w, sens = rec.weights()
+        fr = rec.events()
+        gpu = []
+        for _ in range(2):
+            rec(1)
+            gpu.append(rec.rho)
+    w64, s64 = ol.hybrid_weights(g, f64=True)
+    o64 = ol.hybrid_mlem(g, sz, sdl, length, w64, s64, fr, 2, f64=True)
+    out["parity"] = {
+        "sample_events": len(fr),
+        "rel_l2_vs_fp64": [float(np.linalg.norm(gpu[i] - o64[i]) / np.linalg.norm(o64[i]))
+                           for i in range(2)]}
+    if ol.have_ref_hybrid():
+        r = ol.ref_hybrid(g, sz, sdl, length, lors[:m], zzd[:m], 1, n_threads=cpu_threads,
+                          keep_images=False)
+        base = {"value": len(r["frames"]) / r["seconds"], "unit": "events/s/iter",
+                "cores": cpu_threads, "kind": "reference",
+                "sample": f"first {m} responses, one iteration"}
+    else:
+        w32, s32 = ol.hybrid_weights(g)
+        fr32 = ol.hybrid_frames(g, sz, sdl, w32, lors[:m], zzd[:m])
+        t0 = time.perf_counter()
+        ol.hybrid_mlem(g, sz, sdl, length, w32, s32, fr32, 1)
+        base = {"value": len(fr32) / (time.perf_counter() - t0), "unit": "events/s/iter",
+                "cores": 1, "kind": "port", "sample": f"first {m} responses, one iteration"}
+    out["cpu_baseline"] = base
+    out["speedup_vs_cpu"] = out["events_per_s_iter"] / base["value"]
+    return out
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -74,6 +149,9 @@ def parse():
                         "3.4e10 box pixels of work, 6e6 events at 512 px / 1 mm)")
     p.add_argument("--cpu-threads", type=int, default=16)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--hybrid", default="hybrid-ring192-160px-4mm-128planes-1e6ev",
+                   help="3D hybrid workload measured beside the headline line "
+                        "('none' skips it)")
     p.add_argument("--no-grad", action="store_true")
     p.add_argument("--calibrate", type=int, default=0, metavar="ROUNDS",
                    help="also run ROUNDS rounds of sigma calibration (5 MLEM "
@@ -310,6 +388,10 @@ def main():
                 pkg, name, ev, cal_state, args.cpu_threads,
                 st["n_box_pixels"] / max(n_ev, 1))
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    if args.hybrid != "none":
+        print(f"hybrid workload {args.hybrid}", file=sys.stderr, flush=True)
+        out["hybrid"] = hybrid_report(pkg, args.hybrid, args.steps, args.no_cpu,
+                                      args.cpu_threads)
     print(json.dumps(out), flush=True)
 
 

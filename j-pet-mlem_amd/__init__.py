@@ -11,7 +11,8 @@ from .petstrip import (  # noqa: F401
     phantom_tan_kernel_map, REGION_ELLIPSE, REGION_POINT, REGION_RECTANGLE,
     PROBE_GLOBAL_F32, PROBE_GLOBAL_F64, PROBE_GLOBAL_U64, PROBE_LDS_U64, probe_atomics,
 )
-from .workloads import WORKLOADS, workload_scanner, workload_events  # noqa: F401
+from .workloads import (WORKLOADS, workload_scanner, workload_events,  # noqa: F401
+                        HYBRID_WORKLOADS, hybrid_workload)
 from . import hybrid_geometry  # noqa: F401
 from .hybrid import HybridReconstruction  # noqa: F401
 from .calibration import calibrate_sigma  # noqa: F401

@@ -11,10 +11,13 @@
 //     order-keeping compaction (wave ballots, block offsets scanned on the
 //     host);
 //   - k_hybrid_mlem: the iteration (reconstruction.h:267-445), one wave per
-//     event: its lanes walk the event's (pixel info, plane) pairs, pixel-major
-//     as the reference's loops, sum the denominator in a fixed shuffle tree,
-//     then add weight / denominator to a double accumulator with one global
+//     event: its lanes first put the per-pixel terms of the event's pixel
+//     infos into an LDS table, then walk the event's (pixel info, plane)
+//     pairs, pixel-major as the reference's loops, with one exponential and
+//     two arctangents per voxel, sum the denominator in a fixed shuffle tree,
+//     and add weight / denominator to a double accumulator with one global
 //     atomic per voxel; k_hybrid_finalize converts it to the float image.
+//     Image and accumulator are plane-fastest on the device.
 // The reference's backprojection reads each voxel's weight from a
 // voxel-indexed cache, so when an event's range holds one pixel twice (the
 // extra pixel info one past the LOR's end, reconstruction.h:133-138, can
@@ -223,15 +226,53 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return v;
 }
 
+// The per-pixel part of an event's voxel terms, one entry per pixel info of
+// its range, kept in LDS for the event's (pixel, plane) walk:
+//   y     the pixel centre's position along the LOR from its middle
+//   dyt   (y - up) tan
+//   kpix  c / sqrt(a S a) exp(-isdl br^2 / 2): the kernel's factors that do
+//         not depend on z (gaussian_kernel.h:51-79 splits into these and
+//         exp(-isz pq^2), pq = dz - dy tan)
+//   irm, irp  1 / (R - y), 1 / (R + y) of the frame sensitivity
+//   ws    weight x sensitivity (the denominator's factor)
+//   w2    the backprojection's weight (the extra pixel info's for a pixel it
+//         repeats)
+//   vox0  the pixel's first voxel in the plane-fastest layout
+struct PixTerm {
+  float y, dyt, kpix, irm, irp, ws, w2;
+  int32_t vox0;
+};
+constexpr int kMaxPix = 192;  // table entries per wave; longer ranges walk directly
+
+// z-dependent part of a voxel's weight for pixel term T at plane centre zc:
+// exp(-isz pq^2) kpix over the frame sensitivity (gaussian_kernel.h:140-149)
+// as products with the pixel's reciprocals.  kernel2d <= 0 (the reference's
+// skip) is returned as is.
+__device__ __forceinline__ float voxel_kernel(const HKernel& k, const PixTerm& T, float zc,
+                                              float right) {
+  const float pq = (zc - right) - T.dyt;
+  const float kz = T.kpix * __expf(-k.isz * pq * pq);
+  const float L2 = k.length / 2;
+  const float a = (L2 - zc) * T.irm, b = (L2 + zc) * T.irp;
+  const float c = -(L2 + zc) * T.irm, d = (zc - L2) * T.irp;
+  const float span = atanf(b < a ? b : a) - atanf(c < d ? d : c);
+  return kz * (float)M_PI * __builtin_amdgcn_rcpf(span);
+}
+
+// One wave per event.  The image and the accumulator are plane-fastest
+// (voxel (x, y, z) at z + planes (x + W y)), so the lanes of a wave, which
+// walk consecutive planes of a pixel, read and add contiguous words.
 __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
     const SegD* __restrict__ seg, const int32_t* __restrict__ pixels,
     const float* __restrict__ weights, const float* __restrict__ sens, int32_t n_infos,
     HGrid grid, HKernel k, const HEvent* __restrict__ events, int64_t n,
     const float* __restrict__ rho, double* __restrict__ acc) {
+  __shared__ PixTerm table_all[kWavesPerBlock][kMaxPix];
   const int lane = threadIdx.x & 63;
+  PixTerm* table = table_all[threadIdx.x >> 6];
   const int64_t wave0 = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const int64_t n_waves = (int64_t)gridDim.x * kWavesPerBlock;
-  const int64_t np = (int64_t)grid.w * grid.h;
+  const int planes = grid.planes;
   for (int64_t e = wave0; e < n; e += n_waves) {
     const HEvent ev = events[e];
     const SegD s = seg[ev.lor];
@@ -239,29 +280,55 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
     const int n_pix = end - ev.begin;
     const int n_pl = ev.plane_end - ev.plane_begin;
     if (n_pix <= 0 || n_pl <= 0) continue;
-    const int64_t total = (int64_t)n_pix * n_pl;
-    // (pixel, plane) of pair j = lane + 64 m, advanced by (64 / n_pl,
-    // 64 % n_pl) per step
-    const int step_i = 64 / n_pl, step_z = 64 - step_i * n_pl;
     const int32_t extra_pixel = ev.extra >= 0 && ev.extra < n_infos ? pixels[ev.extra] : -1;
     const float extra_w = extra_pixel != -1 ? weights[ev.extra] : 0.0f;
+    const float sec_sq = ev.sec * ev.sec;
+    // per-pixel terms (the pixel's position along the LOR as the reference
+    // forms it, line_segment.h:80-84)
+    auto pixel_term = [&](int i) {
+      PixTerm T;
+      const int32_t info = ev.begin + i;
+      const int32_t p = pixels[info];
+      const int px = (int16_t)(p & 0xffff), py = (int16_t)(p >> 16);
+      const float cx = grid.llcx + (float)px * grid.ps, cy = grid.llcy + (float)py * grid.ps;
+      T.y = (cx - s.mx) * s.dx + (cy - s.my) * s.dy;
+      const float dy = T.y - ev.up;
+      const float yp = dy + ev.up;
+      const float ap = -(yp - s.R) * sec_sq, aq = -(yp + s.R) * sec_sq;
+      const float ar = -2 * yp * ev.sec * ev.tan;
+      const float br = -2 * dy * ev.sec;
+      const float a_a = ap * k.isz * ap + aq * k.isz * aq + ar * k.isdl * ar;
+      T.kpix = k.c * __builtin_amdgcn_rsqf(a_a) * __expf(-0.5f * (br * k.isdl * br));
+      T.dyt = dy * ev.tan;
+      T.irm = __builtin_amdgcn_rcpf(s.R - T.y);
+      T.irp = __builtin_amdgcn_rcpf(s.R + T.y);
+      const int32_t pix = px + grid.w * py;
+      const float w = weights[info];
+      T.ws = w * sens[pix];
+      T.w2 = p == extra_pixel ? extra_w : w;
+      T.vox0 = pix * planes;
+      return T;
+    };
+    const bool tabled = n_pix <= kMaxPix;
+    if (tabled) {
+      for (int i = lane; i < n_pix; i += 64) table[i] = pixel_term(i);
+      // the wave's own writes, read back by its other lanes
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const int64_t total = (int64_t)n_pix * n_pl;
+    const int step_i = 64 / n_pl, step_z = 64 - step_i * n_pl;
+    const int i0 = lane / n_pl, z0 = lane - i0 * n_pl;
     float den = 0.0f;
     {
-      int i = lane / n_pl, z = lane - (lane / n_pl) * n_pl;
+      int i = i0, z = z0;
       for (int64_t j = lane; j < total; j += 64) {
-        const int32_t info = ev.begin + i;
-        const int32_t p = pixels[info];
-        const int px = (int16_t)(p & 0xffff), py = (int16_t)(p >> 16);
-        const float cx = grid.llcx + (float)px * grid.ps, cy = grid.llcy + (float)py * grid.ps;
-        const float y = (cx - s.mx) * s.dx + (cy - s.my) * s.dy;
+        const PixTerm T = tabled ? table[i] : pixel_term(i);
         const int iz = ev.plane_begin + z;
         const float zc = ((float)iz + 0.5f) * grid.ps + grid.z_left;
-        const float k2 = kernel2d(k, ev.up, ev.tan, ev.sec, s.R, zc, y, ev.right);
-        if (!(k2 <= 0)) {
-          const int64_t pix = px + (int64_t)grid.w * py;
-          const float wt = k2 * weights[info] * rho[pix + iz * np];
-          den += wt * sens[pix];
-        }
+        const float k2 = voxel_kernel(k, T, zc, ev.right);
+        if (!(k2 <= 0)) den += k2 * T.ws * rho[T.vox0 + iz];
         i += step_i;
         z += step_z;
         if (z >= n_pl) {
@@ -271,32 +338,29 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
       }
     }
     den = wave_sum_f(den);
-    if (!(den > 0)) continue;
-    const float inv = 1 / den;
-    int i = lane / n_pl, z = lane - (lane / n_pl) * n_pl;
-    for (int64_t j = lane; j < total; j += 64) {
-      const int32_t info = ev.begin + i;
-      const int32_t p = pixels[info];
-      const int px = (int16_t)(p & 0xffff), py = (int16_t)(p >> 16);
-      const float cx = grid.llcx + (float)px * grid.ps, cy = grid.llcy + (float)py * grid.ps;
-      const float y = (cx - s.mx) * s.dx + (cy - s.my) * s.dy;
-      const int iz = ev.plane_begin + z;
-      const float zc = ((float)iz + 0.5f) * grid.ps + grid.z_left;
-      const float k2 = kernel2d(k, ev.up, ev.tan, ev.sec, s.R, zc, y, ev.right);
-      if (!(k2 <= 0)) {
-        const int64_t vox = px + (int64_t)grid.w * py + iz * np;
-        // a pixel repeated by the extra pixel info adds the extra's weight
-        const float w = p == extra_pixel ? extra_w : weights[info];
-        const float wt = k2 * w * rho[vox];
-        unsafeAtomicAdd(acc + vox, (double)(wt * inv));
-      }
-      i += step_i;
-      z += step_z;
-      if (z >= n_pl) {
-        z -= n_pl;
-        ++i;
+    if (den > 0) {
+      const float inv = 1 / den;
+      int i = i0, z = z0;
+      for (int64_t j = lane; j < total; j += 64) {
+        const PixTerm T = tabled ? table[i] : pixel_term(i);
+        const int iz = ev.plane_begin + z;
+        const float zc = ((float)iz + 0.5f) * grid.ps + grid.z_left;
+        const float k2 = voxel_kernel(k, T, zc, ev.right);
+        if (!(k2 <= 0)) {
+          const int32_t vox = T.vox0 + iz;
+          unsafeAtomicAdd(acc + vox, (double)(k2 * T.w2 * rho[vox] * inv));
+        }
+        i += step_i;
+        z += step_z;
+        if (z >= n_pl) {
+          z -= n_pl;
+          ++i;
+        }
       }
     }
+    // the table is rewritten for the next event only after every lane's
+    // reads of this one
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -732,17 +796,27 @@ int petstrip_hybrid_get_weights(petstrip_hybrid* h, float* weights, float* sensi
   return PETSTRIP_OK;
 }
 
+// The image crosses the boundary as planes x rows x columns (the reference's
+// VoxelMap, voxel.h:27-29) and lives on the device plane-fastest.
 int petstrip_hybrid_set_rho(petstrip_hybrid* h, const float* rho) {
   if (!h || !rho) return set_error(PETSTRIP_EINVAL, "bad argument");
+  const int64_t np = (int64_t)h->grid.w * h->grid.h, P = h->grid.planes;
+  std::vector<float> t(h->n_voxels_grid);
+  for (int64_t z = 0; z < P; ++z)
+    for (int64_t p = 0; p < np; ++p) t[p * P + z] = rho[z * np + p];
   HTRY(hipSetDevice(h->device));
-  HTRY(hipMemcpy(h->d_rho, rho, h->n_voxels_grid * sizeof(float), hipMemcpyHostToDevice));
+  HTRY(hipMemcpy(h->d_rho, t.data(), h->n_voxels_grid * sizeof(float), hipMemcpyHostToDevice));
   return PETSTRIP_OK;
 }
 
 int petstrip_hybrid_get_rho(petstrip_hybrid* h, float* rho) {
   if (!h || !rho) return set_error(PETSTRIP_EINVAL, "bad argument");
+  const int64_t np = (int64_t)h->grid.w * h->grid.h, P = h->grid.planes;
+  std::vector<float> t(h->n_voxels_grid);
   HTRY(hipSetDevice(h->device));
-  HTRY(hipMemcpy(rho, h->d_rho, h->n_voxels_grid * sizeof(float), hipMemcpyDeviceToHost));
+  HTRY(hipMemcpy(t.data(), h->d_rho, h->n_voxels_grid * sizeof(float), hipMemcpyDeviceToHost));
+  for (int64_t p = 0; p < np; ++p)
+    for (int64_t z = 0; z < P; ++z) rho[z * np + p] = t[p * P + z];
   return PETSTRIP_OK;
 }
 

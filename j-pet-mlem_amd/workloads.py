@@ -65,3 +65,27 @@ def workload_events(name, n=None, seed=None):
     sc = workload_scanner(name)
     return phantom_events(sc, phantom, n_ev if n is None else n,
                           seed=s if seed is None else seed)
+
+
+# 3D hybrid workloads: one ring of detectors (hybrid_geometry.ring_geometry),
+# a voxel grid centred on the axis, cylinder responses
+# (hybrid_geometry.cylinder_responses).
+# name: (n_detectors, ring radius, detector width, n_pixels, pixel size,
+#        n_planes, n_events, sigma_z, sigma_dl, length, seed)
+HYBRID_WORKLOADS = {
+    "hybrid-ring192-160px-4mm-128planes-1e6ev":
+        (192, 0.43, 0.007, 160, 0.004, 128, 1_000_000, SIGMA_Z, SIGMA_DL, L, 1),
+}
+
+
+def hybrid_workload(name, n=None):
+    """(geometry, lors, zzd, sigma_z, sigma_dl, length) of a named hybrid
+    workload; n overrides the event count (a prefix of the same stream)."""
+    from . import hybrid_geometry as hg
+    n_det, radius, width, n_pix, ps, n_planes, n_ev, sz, sdl, length, seed = \
+        HYBRID_WORKLOADS[name]
+    g = hg.ring_geometry(n_detectors=n_det, ring_radius=radius, detector_width=width,
+                         n_pixels=n_pix, pixel_size=ps, n_planes=n_planes)
+    lors, zzd = hg.cylinder_responses(g, n_ev if n is None else n, sigma_z=sz,
+                                      sigma_dl=sdl, length=length, seed=seed)
+    return g, lors, zzd, sz, sdl, length

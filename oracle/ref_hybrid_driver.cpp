@@ -11,8 +11,10 @@
 // and normalize_geometry_weights as 3d_hybrid_reconstruction does
 // (reconstruction_cmd.cpp:500-521), loads the responses through the
 // reference's text stream reader (translate_to_frame and the grid filter) and
-// runs the iterations with one OpenMP thread, so the accumulation order is
-// the reference's sequential event order.
+// runs the iterations (one OpenMP thread unless asked for more: with one,
+// the accumulation order is the reference's sequential event order; the
+// iterations' wall time is returned for the CPU baseline).
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <sstream>
@@ -53,10 +55,11 @@ int ref_hybrid_run(int n_detectors, int n_columns, int n_rows, float pixel_size,
                    const float* weights, float sigma_z, float sigma_dl, float length,
                    int sensitivity_one, const int16_t* lors, const float* zzd, int64_t n_events,
                    int n_iterations, float* out_weights, float* out_sensitivity,
-                   int64_t* out_n_kept, double* out_frames, float* out_rho) {
+                   int64_t* out_n_kept, double* out_frames, float* out_rho, int n_threads,
+                   double* iteration_seconds) {
   try {
 #if _OPENMP
-    omp_set_num_threads(1);
+    omp_set_num_threads(n_threads > 0 ? n_threads : 1);
 #endif
     Geometry::Grid grid(n_columns, n_rows, pixel_size, Point2D(lower_left_x, lower_left_y));
     Geometry geometry(n_detectors, grid);
@@ -122,10 +125,14 @@ int ref_hybrid_run(int n_detectors, int n_columns, int n_rows, float pixel_size,
       o[9] = ev.plane_end;
     }
     const size_t n_vox = (size_t)n_columns * n_rows * n_planes;
+    double seconds = 0;
     for (int it = 0; it < n_iterations; ++it) {
+      const auto t0 = std::chrono::steady_clock::now();
       rec();
-      std::memcpy(out_rho + (size_t)it * n_vox, rec.rho.data, sizeof(F) * n_vox);
+      seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (out_rho) std::memcpy(out_rho + (size_t)it * n_vox, rec.rho.data, sizeof(F) * n_vox);
     }
+    if (iteration_seconds) *iteration_seconds = seconds;
     return 0;
   } catch (...) {
     return -1;

@@ -305,15 +305,17 @@ def have_ref_hybrid():
     return os.path.exists(REF_HYBRID_SO)
 
 
-def ref_hybrid(g, sigma_z, sigma_dl, length, lors, zzd, n_iter, sensitivity_one=False):
+def ref_hybrid(g, sigma_z, sigma_dl, length, lors, zzd, n_iter, sensitivity_one=False,
+               n_threads=1, keep_images=True):
     """The reference's own reconstruction on the same geometry and responses:
     dict of weights, sensitivity, frames (kept events: first, second, up,
-    right, tan, sec, begin, end, plane begin, end) and images per iteration."""
+    right, tan, sec, begin, end, plane begin, end), images per iteration and
+    the iterations' wall time (n_threads OpenMP threads)."""
     r = C.CDLL(REF_HYBRID_SO)
     vp, i64, f = C.c_void_p, C.c_int64, C.c_float
     ci = C.c_int
     r.ref_hybrid_run.argtypes = ([ci, ci, ci, f, f, f, f, ci] + [vp] * 7 +
-                                 [f, f, f, ci, vp, vp, i64, ci, vp, vp, vp, vp, vp])
+                                 [f, f, f, ci, vp, vp, i64, ci, vp, vp, vp, vp, vp, ci, vp])
     r.ref_hybrid_run.restype = ci
     lors = np.ascontiguousarray(lors, np.int16)
     zzd = np.ascontiguousarray(zzd, np.float32)
@@ -322,13 +324,17 @@ def ref_hybrid(g, sigma_z, sigma_dl, length, lors, zzd, n_iter, sensitivity_one=
     s = np.zeros((g.n_rows, g.n_columns), np.float32)
     kept = C.c_int64(0)
     frames = np.zeros((n, 10), np.float64)
-    rho = np.zeros((n_iter, g.n_planes, g.n_rows, g.n_columns), np.float32)
+    rho = (np.zeros((n_iter, g.n_planes, g.n_rows, g.n_columns), np.float32)
+           if keep_images else None)
+    seconds = C.c_double(0.0)
     rc = r.ref_hybrid_run(g.n_detectors, g.n_columns, g.n_rows, g.pixel_size,
                           g.lower_left[0], g.lower_left[1], g.z_left, g.n_planes,
                           P(g.segments), P(g.widths), P(g.info_begin), P(g.info_end),
                           P(g.pixels), P(g.positions), P(g.weights), sigma_z, sigma_dl,
                           length, int(sensitivity_one), P(lors), P(zzd), n, n_iter, P(w),
-                          P(s), C.byref(kept), P(frames), P(rho))
+                          P(s), C.byref(kept), P(frames), P(rho), int(n_threads),
+                          C.byref(seconds))
     if rc != 0:
         raise RuntimeError("reference hybrid harness failed")
-    return {"weights": w, "sensitivity": s, "frames": frames[:kept.value], "rho": rho}
+    return {"weights": w, "sensitivity": s, "frames": frames[:kept.value], "rho": rho,
+            "seconds": seconds.value}
