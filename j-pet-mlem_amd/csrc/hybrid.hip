@@ -14,10 +14,12 @@
 //     event: its lanes first put the per-pixel terms of the event's pixel
 //     infos into an LDS table, then walk the event's (pixel info, plane)
 //     pairs, pixel-major as the reference's loops, with one exponential and
-//     two arctangents per voxel, sum the denominator in a fixed shuffle tree,
-//     and add weight / denominator to a double accumulator with one global
-//     atomic per voxel; k_hybrid_finalize converts it to the float image.
-//     Image and accumulator are plane-fastest on the device.
+//     one arctangent per voxel, keep each voxel's kernel x rho in LDS (the
+//     reference's voxel cache) while they sum the denominator in a fixed
+//     shuffle tree, and add weight / denominator to the next image with one
+//     float atomic per voxel (as the reference sums float thread images);
+//     k_hybrid_finalize makes it the image.  Image and accumulator are
+//     plane-fastest on the device.
 // The reference's backprojection reads each voxel's weight from a
 // voxel-indexed cache, so when an event's range holds one pixel twice (the
 // extra pixel info one past the LOR's end, reconstruction.h:133-138, can
@@ -243,6 +245,7 @@ struct PixTerm {
   int32_t vox0;
 };
 constexpr int kMaxPix = 192;  // table entries per wave; longer ranges walk directly
+constexpr int kCacheVox = 1024;  // voxels per wave whose kernel x rho pass 2 reuses
 
 // z-dependent part of a voxel's weight for pixel term T at plane centre zc:
 // exp(-isz pq^2) kpix over the frame sensitivity (gaussian_kernel.h:140-149)
@@ -255,7 +258,12 @@ __device__ __forceinline__ float voxel_kernel(const HKernel& k, const PixTerm& T
   const float L2 = k.length / 2;
   const float a = (L2 - zc) * T.irm, b = (L2 + zc) * T.irp;
   const float c = -(L2 + zc) * T.irm, d = (zc - L2) * T.irp;
-  const float span = atanf(b < a ? b : a) - atanf(c < d ? d : c);
+  // atan u - atan v = atan((u - v) / (1 + u v)), plus pi (or minus pi)
+  // when 1 + u v < 0 and u > 0 (u < 0): one arctangent instead of two
+  const float u = b < a ? b : a, v = c < d ? d : c;
+  const float q = 1 + u * v;
+  float span = atanf((u - v) * __builtin_amdgcn_rcpf(q));
+  if (q < 0) span += u > 0 ? (float)M_PI : -(float)M_PI;
   return kz * (float)M_PI * __builtin_amdgcn_rcpf(span);
 }
 
@@ -266,10 +274,12 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
     const SegD* __restrict__ seg, const int32_t* __restrict__ pixels,
     const float* __restrict__ weights, const float* __restrict__ sens, int32_t n_infos,
     HGrid grid, HKernel k, const HEvent* __restrict__ events, int64_t n,
-    const float* __restrict__ rho, double* __restrict__ acc) {
+    const float* __restrict__ rho, float* __restrict__ acc) {
   __shared__ PixTerm table_all[kWavesPerBlock][kMaxPix];
+  __shared__ float cache_all[kWavesPerBlock][kCacheVox];
   const int lane = threadIdx.x & 63;
   PixTerm* table = table_all[threadIdx.x >> 6];
+  float* cache = cache_all[threadIdx.x >> 6];
   const int64_t wave0 = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const int64_t n_waves = (int64_t)gridDim.x * kWavesPerBlock;
   const int planes = grid.planes;
@@ -328,7 +338,13 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
         const int iz = ev.plane_begin + z;
         const float zc = ((float)iz + 0.5f) * grid.ps + grid.z_left;
         const float k2 = voxel_kernel(k, T, zc, ev.right);
-        if (!(k2 <= 0)) den += k2 * T.ws * rho[T.vox0 + iz];
+        // kernel x rho for pass 2 (0 for the reference's skipped voxels)
+        float kr = 0.0f;
+        if (!(k2 <= 0)) {
+          kr = k2 * rho[T.vox0 + iz];
+          den += kr * T.ws;
+        }
+        if (j < kCacheVox) cache[j] = kr;
         i += step_i;
         z += step_z;
         if (z >= n_pl) {
@@ -344,12 +360,16 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
       for (int64_t j = lane; j < total; j += 64) {
         const PixTerm T = tabled ? table[i] : pixel_term(i);
         const int iz = ev.plane_begin + z;
-        const float zc = ((float)iz + 0.5f) * grid.ps + grid.z_left;
-        const float k2 = voxel_kernel(k, T, zc, ev.right);
-        if (!(k2 <= 0)) {
-          const int32_t vox = T.vox0 + iz;
-          unsafeAtomicAdd(acc + vox, (double)(k2 * T.w2 * rho[vox] * inv));
+        const int32_t vox = T.vox0 + iz;
+        float kr;
+        if (j < kCacheVox) {
+          kr = cache[j];
+        } else {
+          const float zc = ((float)iz + 0.5f) * grid.ps + grid.z_left;
+          const float k2 = voxel_kernel(k, T, zc, ev.right);
+          kr = !(k2 <= 0) ? k2 * rho[vox] : 0.0f;
         }
+        if (kr != 0.0f) unsafeAtomicAdd(acc + vox, kr * T.w2 * inv);
         i += step_i;
         z += step_z;
         if (z >= n_pl) {
@@ -364,12 +384,12 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_hybrid_finalize(double* __restrict__ acc,
+__global__ __launch_bounds__(kBlock) void k_hybrid_finalize(float* __restrict__ acc,
                                                             float* __restrict__ rho, int64_t n) {
   const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (v >= n) return;
-  rho[v] = (float)acc[v];
-  acc[v] = 0.0;
+  rho[v] = acc[v];
+  acc[v] = 0.0f;
 }
 
 __global__ __launch_bounds__(kBlock) void k_count_voxels(const HEvent* __restrict__ events,
@@ -434,7 +454,7 @@ struct petstrip_hybrid {
   float* d_w = nullptr;
   float* d_sens = nullptr;
   float* d_rho = nullptr;
-  double* d_acc = nullptr;
+  float* d_acc = nullptr;
   HEvent* d_events = nullptr;
   int64_t n_events = 0, cap_events = 0;
   int64_t n_voxels = 0;
@@ -620,7 +640,7 @@ int petstrip_hybrid_create(const petstrip_hybrid_geometry* g, double sigma_z, do
   if (e == hipSuccess)
     e = hipMemcpy(h->d_rho, ones.data(), h->n_voxels_grid * sizeof(float),
                   hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(h->d_acc, 0, h->n_voxels_grid * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(h->d_acc, 0, h->n_voxels_grid * sizeof(float));
   if (e != hipSuccess) return fail_free(hip_fail(e, "hybrid geometry upload"));
   if ((rc = upload_weights(h))) return fail_free(rc);
   *out = h;
