@@ -112,10 +112,11 @@ int petstrip_hybrid_get_rho(petstrip_hybrid* h, float* rho);
 int petstrip_hybrid_iterate(petstrip_hybrid* h, int32_t n);
 
 /* The reference's run(): n_blocks x n_iterations_in_block iterations from
- * rho_inout, skipping those below start_iteration; output(iteration,
- * image) after every block (image: planes x rows x columns, valid during the
- * call; output(-1, sensitivity) first with the W x H map), progress after
- * every iteration (cuda/reconstruction.cu:92-156). */
+ * rho_inout, skipping those below start_iteration; output(iteration, image)
+ * after iterations 1-5, 10, 15, 20, 30, 50 and 100 of the first block and
+ * after every block (image: planes x rows x columns, also left in
+ * rho_inout); progress before and after every iteration and once at the end
+ * (cuda/reconstruction.cu:103-181). */
 typedef int (*petstrip_hybrid_output_cb)(void* user, int iteration, const float* image,
                                          int32_t width, int32_t height, int32_t depth);
 int petstrip_hybrid_run(petstrip_hybrid* h, float* rho_inout, int32_t n_blocks,

@@ -858,9 +858,12 @@ int petstrip_hybrid_run(petstrip_hybrid* h, float* rho_inout, int32_t n_blocks,
     return set_error(PETSTRIP_EINVAL, "bad argument");
   int rc;
   if ((rc = petstrip_hybrid_set_rho(h, rho_inout))) return rc;
-  if (output && output(user, -1, h->sens.data(), h->grid.w, h->grid.h, 1))
-    return set_error(PETSTRIP_ECALLBACK, "output callback stopped the run");
-  const int n_iterations = n_blocks * n_iterations_in_block;
+  auto emit = [&](int iteration) -> int {
+    if ((rc = petstrip_hybrid_get_rho(h, rho_inout))) return rc;
+    if (output && output(user, iteration, rho_inout, h->grid.w, h->grid.h, h->grid.planes))
+      return set_error(PETSTRIP_ECALLBACK, "output callback stopped the run");
+    return PETSTRIP_OK;
+  };
   for (int b = 0; b < n_blocks; ++b) {
     for (int i = 0; i < n_iterations_in_block; ++i) {
       const int it = b * n_iterations_in_block + i;
@@ -868,14 +871,16 @@ int petstrip_hybrid_run(petstrip_hybrid* h, float* rho_inout, int32_t n_blocks,
       if (progress) progress(user, it, 0);
       if ((rc = petstrip_hybrid_iterate(h, 1))) return rc;
       if (progress) progress(user, it, 1);
+      // the first block also outputs iterations 1-5, 10, 15, 20, 30, 50, 100
+      if (!b && i < n_iterations_in_block - 1 &&
+          (i < 5 || i == 9 || i == 14 || i == 19 || i == 29 || i == 49 || i == 99))
+        if ((rc = emit(i + 1))) return rc;
     }
     const int it = (b + 1) * n_iterations_in_block;
     if (it <= start_iteration) continue;
-    if ((rc = petstrip_hybrid_get_rho(h, rho_inout))) return rc;
-    if (output && output(user, it, rho_inout, h->grid.w, h->grid.h, h->grid.planes))
-      return set_error(PETSTRIP_ECALLBACK, "output callback stopped the run");
+    if ((rc = emit(it))) return rc;
   }
-  (void)n_iterations;
+  if (progress) progress(user, n_blocks * n_iterations_in_block, 0);
   return PETSTRIP_OK;
 }
 

@@ -170,8 +170,9 @@ class HybridReconstruction:
 
     def run(self, n_blocks, n_iterations_in_block, rho=None, start_iteration=0,
             output=None, progress=None):
-        """The reference's block schedule: output(iteration, image) after
-        every block (and output(-1, sensitivity) first)."""
+        """The reference's block schedule (cuda/reconstruction.cu:103-181):
+        output(iteration, image) after iterations 1-5, 10, 15, 20, 30, 50, 100
+        of the first block and after every block."""
         img = np.ones(self.shape, np.float32) if rho is None else \
             np.ascontiguousarray(rho, np.float32).reshape(self.shape).copy()
 

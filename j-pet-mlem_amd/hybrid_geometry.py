@@ -188,3 +188,97 @@ def cylinder_responses(geometry, n, sigma_z=0.01, sigma_dl=0.04, length=0.3, rad
         out_l = np.concatenate([out_l, lors])
         out_z = np.concatenate([out_z, zzd])
     return np.ascontiguousarray(out_l[:n]), np.ascontiguousarray(out_z[:n])
+
+
+# ---------------------------------------------------------------------------
+# The reference's binary geometry file (2d_barrel_geometry's output, read by
+# 3d_hybrid_reconstruction --geometry): native little-endian fields,
+#   uint32 magic "PETg" (float geometry), int16 n_detectors,
+#   grid: int16 n_columns, int16 n_rows, float pixel_size, float lower-left x, y
+#   then per LOR, in index order: int16 first, int16 second, float start x, y,
+#   end x, y, float width, uint64 n, n x (int16 x, y, float t, distance,
+#   fill, weight)
+# (src/2d/barrel/geometry.h:55-104, lor_geometry.h:46-88,
+#  2d/geometry/pixel_grid.h:63-77, line_segment.h:33-56, util/bstream.h).
+GEOMETRY_MAGIC = b"PETg"
+_PIXEL_INFO = np.dtype([("x", "<i2"), ("y", "<i2"), ("t", "<f4"), ("distance", "<f4"),
+                        ("fill", "<f4"), ("weight", "<f4")])
+
+
+def read_geometry(path, n_planes=None, z_left=None):
+    """A HybridGeometry from the reference's geometry file; the voxel grid's
+    planes default as 3d_hybrid_reconstruction's do (n_planes = the larger
+    pixel count, z_left = -n_planes pixel_size / 2; reconstruction_cmd.cpp:
+    287-315)."""
+    data = open(path, "rb").read()
+    if data[:4] != GEOMETRY_MAGIC:
+        raise ValueError("unknown binary geometry file")
+    n_det = int(np.frombuffer(data, "<i2", 1, 4)[0])
+    n_col, n_row = (int(v) for v in np.frombuffer(data, "<i2", 2, 6))
+    ps, llx, lly = (float(v) for v in np.frombuffer(data, "<f4", 3, 10))
+    n_lors = n_det * (n_det + 1) // 2
+    segments = np.zeros((n_lors, 4), np.float32)
+    widths = np.zeros(n_lors, np.float32)
+    begin = np.zeros(n_lors, np.int64)
+    end = np.zeros(n_lors, np.int64)
+    infos = []
+    off, total = 22, 0
+    head = np.dtype([("first", "<i2"), ("second", "<i2"), ("seg", "<f4", 4),
+                     ("width", "<f4"), ("n", "<u8")])
+    while off + head.itemsize <= len(data):
+        h = np.frombuffer(data, head, 1, off)[0]
+        off += head.itemsize
+        n = int(h["n"])
+        first, second = int(h["first"]), int(h["second"])
+        li = first * (first + 1) // 2 + second
+        if not (0 <= second <= first < n_det):
+            raise ValueError("geometry file: LOR out of range")
+        segments[li] = h["seg"]
+        widths[li] = h["width"]
+        pi = np.frombuffer(data, _PIXEL_INFO, n, off) if n else np.zeros(0, _PIXEL_INFO)
+        off += n * _PIXEL_INFO.itemsize
+        infos.append((li, pi))
+    # pixel infos laid out in LOR index order (the GeometrySOA layout)
+    infos.sort(key=lambda x: x[0])
+    pix, pos, wts = [], [], []
+    for li, pi in infos:
+        begin[li] = total
+        total += len(pi)
+        end[li] = total
+        pix.append(np.stack([pi["x"], pi["y"]], axis=1))
+        pos.append(pi["t"])
+        wts.append(pi["weight"])
+    # LORs absent from the file keep an empty range at the running offset
+    last = 0
+    for li in range(n_lors):
+        if end[li] == 0 and begin[li] == 0:
+            begin[li] = end[li] = last
+        last = end[li]
+    n_planes = max(n_col, n_row) if n_planes is None else int(n_planes)
+    z_left = -n_planes * ps / 2 if z_left is None else float(z_left)
+    cat = (lambda xs, t, shape: np.concatenate(xs).astype(t) if xs else np.zeros(shape, t))
+    return HybridGeometry(n_det, n_col, n_row, ps, (llx, lly), z_left, n_planes, segments,
+                          widths, begin, end, cat(pix, np.int16, (0, 2)),
+                          cat(pos, np.float32, (0,)), cat(wts, np.float32, (0,)), 0.0)
+
+
+def write_geometry(g, path):
+    """The geometry in the reference's binary format (every LOR, in index
+    order; distance and fill written as 0)."""
+    with open(path, "wb") as f:
+        f.write(GEOMETRY_MAGIC)
+        f.write(np.array([g.n_detectors, g.n_columns, g.n_rows], "<i2").tobytes())
+        f.write(np.array([g.pixel_size, g.lower_left[0], g.lower_left[1]], "<f4").tobytes())
+        for first in range(g.n_detectors):
+            for second in range(first + 1):
+                li = first * (first + 1) // 2 + second
+                b, e = int(g.info_begin[li]), int(g.info_end[li])
+                f.write(np.array([first, second], "<i2").tobytes())
+                f.write(np.asarray(g.segments[li], "<f4").tobytes())
+                f.write(np.array([g.widths[li]], "<f4").tobytes())
+                f.write(np.array([e - b], "<u8").tobytes())
+                pi = np.zeros(e - b, _PIXEL_INFO)
+                pi["x"], pi["y"] = g.pixels[b:e, 0], g.pixels[b:e, 1]
+                pi["t"] = g.positions[b:e]
+                pi["weight"] = g.weights[b:e]
+                f.write(pi.tobytes())

@@ -139,4 +139,46 @@ int ref_hybrid_run(int n_detectors, int n_columns, int n_rows, float pixel_size,
   }
 }
 
+// The reference's own reader of a binary geometry file (Geometry(ibstream),
+// geometry.h:55-83) into GeometrySOA arrays (geometry_soa.h:178-199):
+// sizes first (call with null arrays), then the arrays.
+int ref_hybrid_read_geometry(const char* path, int* n_detectors, int* n_columns,
+                             int* n_rows, float* grid3, int64_t* n_pixel_infos,
+                             float* segments, float* widths, int64_t* info_begin,
+                             int64_t* info_end, int16_t* pixels, float* positions,
+                             float* weights) {
+  try {
+    util::ibstream in(path);
+    if (!in.is_open()) return -1;
+    Geometry geometry(in);
+    Reconstruction::Geometry soa(geometry, 1);
+    *n_detectors = geometry.n_detectors;
+    *n_columns = geometry.grid.n_columns;
+    *n_rows = geometry.grid.n_rows;
+    grid3[0] = geometry.grid.pixel_size;
+    grid3[1] = geometry.grid.lower_left.x;
+    grid3[2] = geometry.grid.lower_left.y;
+    *n_pixel_infos = (int64_t)soa.n_pixel_infos;
+    if (!segments) return 0;
+    for (size_t l = 0; l < soa.n_lors; ++l) {
+      segments[4 * l] = soa.lor_line_segments[l].start.x;
+      segments[4 * l + 1] = soa.lor_line_segments[l].start.y;
+      segments[4 * l + 2] = soa.lor_line_segments[l].end.x;
+      segments[4 * l + 3] = soa.lor_line_segments[l].end.y;
+      widths[l] = soa.lor_widths[l];
+      info_begin[l] = (int64_t)soa.lor_pixel_info_begin[l];
+      info_end[l] = (int64_t)soa.lor_pixel_info_end[l];
+    }
+    for (size_t i = 0; i < soa.n_pixel_infos; ++i) {
+      pixels[2 * i] = soa.pixels[i].x;
+      pixels[2 * i + 1] = soa.pixels[i].y;
+      positions[i] = soa.pixel_positions[i];
+      weights[i] = soa.pixel_weights[i];
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
 }  // extern "C"

@@ -338,3 +338,28 @@ def ref_hybrid(g, sigma_z, sigma_dl, length, lors, zzd, n_iter, sensitivity_one=
         raise RuntimeError("reference hybrid harness failed")
     return {"weights": w, "sensitivity": s, "frames": frames[:kept.value], "rho": rho,
             "seconds": seconds.value}
+
+
+def ref_hybrid_read_geometry(path):
+    """The reference's reader of a binary geometry file, as GeometrySOA
+    arrays (dict)."""
+    r = C.CDLL(REF_HYBRID_SO)
+    vp = C.c_void_p
+    r.ref_hybrid_read_geometry.argtypes = [C.c_char_p, vp, vp, vp, vp, vp] + [vp] * 7
+    r.ref_hybrid_read_geometry.restype = C.c_int
+    nd, nc, nr = C.c_int(), C.c_int(), C.c_int()
+    g3 = np.zeros(3, np.float32)
+    ni = C.c_int64()
+    args = [path.encode(), C.byref(nd), C.byref(nc), C.byref(nr), P(g3), C.byref(ni)]
+    if r.ref_hybrid_read_geometry(*args, *([None] * 7)) != 0:
+        raise RuntimeError("reference geometry reader failed")
+    n_lors = nd.value * (nd.value + 1) // 2
+    out = {"n_detectors": nd.value, "n_columns": nc.value, "n_rows": nr.value, "grid": g3,
+           "segments": np.zeros((n_lors, 4), np.float32), "widths": np.zeros(n_lors, np.float32),
+           "info_begin": np.zeros(n_lors, np.int64), "info_end": np.zeros(n_lors, np.int64),
+           "pixels": np.zeros((ni.value, 2), np.int16),
+           "positions": np.zeros(ni.value, np.float32), "weights": np.zeros(ni.value, np.float32)}
+    keys = ["segments", "widths", "info_begin", "info_end", "pixels", "positions", "weights"]
+    if r.ref_hybrid_read_geometry(*args, *[P(out[k]) for k in keys]) != 0:
+        raise RuntimeError("reference geometry reader failed")
+    return out

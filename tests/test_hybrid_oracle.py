@@ -118,3 +118,29 @@ def test_planted_mistakes_fail():
     fr = ol.hybrid_frames(g, c[5], c[6], w, lors, zzd)
     rho = ol.hybrid_mlem(g, c[5] * 1.001, c[6], c[7], w, s, fr, 1)
     assert not same_bits(rho[0], f["rho"][0])
+
+
+def test_geometry_file_round_trip_and_reference_reader(pkg, tmp_path):
+    """write_geometry writes the reference's binary geometry format:
+    read_geometry reads it back unchanged, and the reference's own reader
+    (Geometry(ibstream) into a GeometrySOA) sees the same arrays."""
+    hg = pkg.hybrid_geometry
+    g = hg.ring_geometry(n_detectors=24, ring_radius=0.2, n_pixels=20, pixel_size=0.012,
+                         n_planes=14)
+    g.weights = np.random.default_rng(3).random(g.n_pixel_infos).astype(np.float32)
+    path = str(tmp_path / "ring24.bin")
+    hg.write_geometry(g, path)
+    h = hg.read_geometry(path, n_planes=14, z_left=g.z_left)
+    assert (h.n_detectors, h.n_columns, h.n_rows, h.n_planes) == (24, 20, 20, 14)
+    assert h.pixel_size == np.float32(g.pixel_size)
+    for k in ["segments", "widths", "info_begin", "info_end", "pixels", "positions",
+              "weights"]:
+        assert np.array_equal(getattr(h, k), getattr(g, k)), k
+    if not ol.have_ref_hybrid():
+        pytest.skip("oracle/_ref/libref_hybrid.so not built")
+    r = ol.ref_hybrid_read_geometry(path)
+    assert (r["n_detectors"], r["n_columns"], r["n_rows"]) == (24, 20, 20)
+    assert np.array_equal(r["grid"], np.array([g.pixel_size, *g.lower_left], np.float32))
+    for k in ["segments", "widths", "info_begin", "info_end", "pixels", "positions",
+              "weights"]:
+        assert np.array_equal(r[k], getattr(g, k)), k
