@@ -91,9 +91,19 @@ def hybrid_report(pkg, name, steps, no_cpu, cpu_threads):
         "voxels_per_event": st["n_voxels"] / max(n_ev, 1),
         "ms_per_iteration": ms, "kernel_ms": float(np.mean(kern)),
         "events_per_s_iter": n_ev / (ms * 1e-3),
-        "voxel_evaluations_per_s": 2 * st["n_voxels"] / (float(np.mean(kern)) * 1e-3),
+        "voxel_evaluations_per_s": st["n_voxels"] / (float(np.mean(kern)) * 1e-3),
         "setup_s": {"weights_and_upload": t_create, "frame_events": t_events},
     }
+    # the launch's bound: one float atomic per voxel (at most) against the
+    # device's measured rate for the same kind of update
+    try:
+        peak, _ = pkg.probe_atomics(pkg.PROBE_GLOBAL_F32, g.n_columns, g.n_rows, reps=1024)
+        achieved = st["n_voxels"] / (float(np.mean(kern)) * 1e-3)
+        out["roofline"] = {"bound": "global float atomics", "achieved": achieved,
+                           "peak": peak, "unit": "atomics/s", "frac": achieved / peak,
+                           "kernel": "k_hybrid_mlem"}
+    except (ImportError, OSError, pkg.PetstripError) as e:
+        out["roofline"] = {"error": str(e)}
     if no_cpu:
         return out
     # the CPU leg: the oracle as the checker, the reference as the baseline

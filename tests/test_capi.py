@@ -30,11 +30,28 @@ def test_header_declares_entry_points():
         assert n in names
 
 
+HYBRID_HEADER = os.path.join(ROOT, "include", "petstrip_hybrid.h")
+
+
 def test_library_exports_every_declared_symbol(pkg):
     lib = pkg.load_library()
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
     assert lib.petstrip_abi_version() == 1
+    hybrid = declared_functions(HYBRID_HEADER)
+    assert "petstrip_hybrid_create" in hybrid and "petstrip_hybrid_run" in hybrid
+    missing = [n for n in hybrid if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_hybrid_errors_are_returned_without_a_device(pkg):
+    lib = pkg.load_library()
+    lib.petstrip_hybrid_create.restype = C.c_int
+    h = C.c_void_p()
+    assert lib.petstrip_hybrid_create(None, C.c_double(0.01), C.c_double(0.04),
+                                      C.c_double(0.3), 0, 0, C.byref(h)) == -1
+    assert b"bad argument" in lib.petstrip_last_error()
+    assert lib.petstrip_hybrid_iterate(None, 1) == -1
 
 
 def test_probe_library_exports_every_declared_symbol(pkg):
