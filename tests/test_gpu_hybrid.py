@@ -85,6 +85,37 @@ def test_hybrid_vs_oracle_f64(gpu_pkg):
         assert r32 < 1e-5 and r64 < 1e-4
 
 
+def test_hybrid_long_ranges(gpu_pkg):
+    """Wide detectors: ranges of more than 192 pixel infos (the kernel then
+    forms the per-pixel terms at every voxel instead of in its LDS table) and
+    events of more than 1024 voxels (whose later voxels the backprojection
+    recomputes instead of reading them from the LDS cache)."""
+    import oracle_lib as ol
+    pkg = gpu_pkg
+    hg = pkg.hybrid_geometry
+    g = hg.ring_geometry(n_detectors=32, ring_radius=0.2, detector_width=0.018, n_pixels=48,
+                         pixel_size=0.004, n_planes=40)
+    lors, zzd = hg.cylinder_responses(g, 3000, sigma_z=0.012, sigma_dl=0.05, length=0.3,
+                                      seed=21)
+    print("case: 32 detectors of 18 mm, 48x48x40 voxels of 4 mm, 3000 responses", flush=True)
+    with pkg.HybridReconstruction(g, 0.012, 0.05, 0.3) as rec:
+        rec.set_responses(lors, zzd)
+        fr = rec.events()
+        w, s = rec.weights()
+        gpu = []
+        for _ in range(2):
+            rec(1)
+            gpu.append(rec.rho)
+    n_pix = np.minimum(fr["end"], g.n_pixel_infos) - fr["begin"]
+    n_pl = fr["plane_end"] - fr["plane_begin"]
+    assert 0.2 < np.mean(n_pix > 192) < 0.9 and np.mean(n_pix * n_pl > 1024) > 0.9
+    o32 = ol.hybrid_mlem(g, 0.012, 0.05, 0.3, w, s, fr, 2)
+    for it in range(2):
+        r = rel(gpu[it], o32[it])
+        print(f"  iteration {it + 1}: rel L2 vs oracle f32 {r:.2e}", flush=True)
+        assert r < 1e-5
+
+
 def test_hybrid_events_and_run_schedule(gpu_pkg):
     """Frame events handed over by the caller give the image of the
     responses; run() follows the reference's block schedule; with sensitivity
