@@ -95,9 +95,11 @@ def hybrid_report(pkg, name, steps, no_cpu, cpu_threads):
         "setup_s": {"weights_and_upload": t_create, "frame_events": t_events},
     }
     # the launch's bound: one float atomic per voxel (at most) against the
-    # device's measured rate for the same kind of update
+    # device's measured rate for the same kind of update over a buffer of the
+    # accumulator's size (columns x rows x planes as a columns-wide image)
     try:
-        peak, _ = pkg.probe_atomics(pkg.PROBE_GLOBAL_F32, g.n_columns, g.n_rows, reps=1024)
+        peak, _ = pkg.probe_atomics(pkg.PROBE_GLOBAL_F32, g.n_columns,
+                                    min(g.n_rows * g.n_planes, 32768), reps=256)
         achieved = st["n_voxels"] / (float(np.mean(kern)) * 1e-3)
         out["roofline"] = {"bound": "global float atomics", "achieved": achieved,
                            "peak": peak, "unit": "atomics/s", "frac": achieved / peak,
