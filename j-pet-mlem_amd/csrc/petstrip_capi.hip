@@ -144,6 +144,8 @@ struct petstrip_handle {
   float* d_out = nullptr;
   float* d_sens = nullptr;  // 5-point sensitivity for the --gpu variant
   double* d_scalars = nullptr;
+  double* d_scal_part = nullptr;  // per-workgroup partials of the scalars
+  size_t cap_scal_part = 0;
   bool lambda_valid = false;
   petstrip_stats stats{};
   // spatially binned events for the fused iteration
@@ -453,6 +455,16 @@ int prepare_fused(petstrip_handle* h) {
   }
   h->window_px = 0;
   for (const auto& it : items) h->window_px += (int64_t)it.ww * it.wh;
+  {
+    const size_t need = (size_t)std::max<int64_t>(h->n_items, petstrip::kCombineBlocks) *
+                        petstrip::kScalStride;
+    if (need > h->cap_scal_part) {
+      dfree(h->d_scal_part);
+      h->cap_scal_part = 0;
+      if ((rc = dalloc(&h->d_scal_part, need))) return rc;
+      h->cap_scal_part = need;
+    }
+  }
   h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
                 h->opts.update == PETSTRIP_UPDATE_CPU &&
                 h->opts.kernel == PETSTRIP_KERNEL_GAUSSIAN &&
@@ -517,6 +529,7 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.split = 0;
   a.inv_fixed_scale = std::ldexp(1.0, -kMlemFixedBits);
   a.scalars = h->d_scalars;
+  a.scal_part = h->d_scal_part;
   a.row_scan = h->opts.row_scan;
   a.s_pix = h->s_pix;
   a.du = h->du;
@@ -737,6 +750,7 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_out);
   dfree(h->d_sens);
   dfree(h->d_scalars);
+  dfree(h->d_scal_part);
   dfree(h->d_bin);
   dfree(h->d_count);
   dfree(h->d_sgeom);

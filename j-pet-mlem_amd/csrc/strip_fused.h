@@ -49,7 +49,13 @@ struct FusedArgs {
   double* scalars;           // FORWARD: {sum log lambda, dl/dsz, dl/dsdl,
                              //           max_e |g_e| K_e, sum_e |g_e| K_e}
                              // with K_e the event's largest kernel value
+  double* scal_part;         // FORWARD, GRAD: the same five per workgroup,
+                             // kScalStride doubles apart, added to scalars
+                             // by k_scalars_finish in a fixed order
 };
+
+constexpr int kScalStride = 8;       // doubles per workgroup in scal_part
+constexpr int kCombineBlocks = 1024; // workgroups of k_band_combine (grid-stride)
 
 hipError_t launch_bin_count(hipStream_t st, const EventGeom* geom, int64_t n,
                             int bin_w, int bin_h, int nbx, int32_t* bin,
@@ -78,6 +84,8 @@ size_t fused_lds_bytes(int max_ww, int col_stride, int mode);
 // Column stride for windows of at most max_wh rows: the next multiple of
 // 64, or 0 when above kMaxColStride.
 int fused_col_stride(int max_wh);
+// MODE_FORWARD and MODE_GRAD also add their five scalars to args.scalars,
+// through args.scal_part (n_items * kScalStride doubles).
 hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int compensated,
                               const FusedArgs& args, int n_items, size_t lds_bytes);
 // Banded form (column stride kBandRows): band = 1 runs pass 1 into the band
@@ -87,7 +95,8 @@ hipError_t launch_strip_banded(hipStream_t st, int mode, int band, const FusedAr
                                int n_items, size_t lds_bytes);
 // Band partials -> lambda (sorted and original order); sigma = 1 also adds
 // the log-likelihood terms, weighted sigma derivatives, max and sum of |g| K
-// to the scalars (as the FORWARD mode does).
+// to the scalars (as the FORWARD mode does), through args.scal_part
+// (kCombineBlocks * kScalStride doubles).
 hipError_t launch_band_combine(hipStream_t st, const FusedArgs& args, int64_t n_sorted,
                                int sigma);
 

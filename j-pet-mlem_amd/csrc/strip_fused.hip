@@ -752,18 +752,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
     }
   }
 
-  if constexpr (kSigma) {
-    // lane 0 of each wave holds the wave's scalars
-    if (lane == 0) {
-      unsafeAtomicAdd(&a.scalars[0], ll);
-      unsafeAtomicAdd(&a.scalars[1], gsz);
-      unsafeAtomicAdd(&a.scalars[2], gsdl);
-      // non-negative doubles order like their bit patterns
-      atomicMax(reinterpret_cast<U*>(&a.scalars[3]), (U)__double_as_longlong(mmax));
-      unsafeAtomicAdd(&a.scalars[4], msum);
-    }
-  }
-
   if constexpr (kPass2) {
     __syncthreads();
     const int ntc = (ww + 15) >> 4, ntr = (wh + 3) >> 2;
@@ -783,6 +771,65 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + p,
                         (float)((double)(long long)v * a.inv_fixed_scale));
     }
+  }
+
+  if constexpr (kSigma && BAND == 0) {
+    // Lane 0 of each wave holds the wave's scalars.  The workgroup's sums go
+    // to its own slot in a fixed wave order: five atomics per wave onto the
+    // same five addresses queue at one L2 channel, and each wave's wait for
+    // them held its workgroup for milliseconds on small footprints.
+    __syncthreads();  // every wave is done with the LDS windows
+    double* red = reinterpret_cast<double*>(smem);
+    if (lane == 0) {
+      red[wave * 5 + 0] = ll;
+      red[wave * 5 + 1] = gsz;
+      red[wave * 5 + 2] = gsdl;
+      red[wave * 5 + 3] = mmax;
+      red[wave * 5 + 4] = msum;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+      const int i = threadIdx.x;
+      double s = red[i];
+      for (int w = 1; w < kWaves; ++w) s = i == 3 ? fmax(s, red[w * 5 + i]) : s + red[w * 5 + i];
+      a.scal_part[(size_t)blockIdx.x * kScalStride + i] = s;
+    }
+  }
+}
+
+// Adds the per-workgroup partials of the scalars (sum log lambda, the two
+// sigma sums, max and sum of |g| K; kScalStride doubles apart) to scalars,
+// in a fixed order: thread t sums blocks t, t + 256, ..., then a fixed tree.
+__global__ __launch_bounds__(256) void k_scalars_finish(const double* part, int64_t nblocks,
+                                                        double* scalars) {
+  __shared__ double red[5][256];
+  const int t = threadIdx.x;
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int64_t b = t; b < nblocks; b += 256) {
+    const double* p = part + b * kScalStride;
+    v[0] += p[0];
+    v[1] += p[1];
+    v[2] += p[2];
+    v[3] = fmax(v[3], p[3]);
+    v[4] += p[4];
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) red[i][t] = v[i];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        red[i][t] = i == 3 ? fmax(red[i][t], red[i][t + s]) : red[i][t] + red[i][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    scalars[0] += red[0][0];
+    scalars[1] += red[1][0];
+    scalars[2] += red[2][0];
+    scalars[3] = fmax(scalars[3], red[3][0]);
+    scalars[4] += red[4][0];
   }
 }
 
@@ -855,9 +902,15 @@ hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds)
   for (int i0 = 0; i0 < n_items; i0 += kSlice) {
     FusedArgs b = a;
     b.items = a.items + i0;
+    b.scal_part = a.scal_part + (size_t)i0 * kScalStride;
     hipLaunchKernelGGL(kern, dim3((unsigned)std::min(kSlice, n_items - i0)),
                        dim3(64 * kWaves), lds, st, b);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  if constexpr (BAND == 0 && (MODE == MODE_FORWARD || MODE == MODE_GRAD)) {
+    hipLaunchKernelGGL(k_scalars_finish, dim3(1), dim3(256), 0, st, a.scal_part,
+                       (int64_t)n_items, a.scalars);
+    return hipGetLastError();
   }
   return hipSuccess;
 }
@@ -876,17 +929,18 @@ hipError_t launch_h(int col_stride, hipStream_t st, const FusedArgs& a, int n_it
 
 // Sums each event's band partials in band order: lambda (sorted and original
 // order) and, with sigma, the log-likelihood terms, the sigma derivative
-// weighted by g_e (or 1/lambda_e) and the largest |g_e| K; block sums go to
-// the scalars in doubles.
+// weighted by g_e (or 1/lambda_e) and the largest |g_e| K.  A fixed grid of
+// kCombineBlocks blocks strides over the events; each block's sums go to its
+// slot of scal_part (k_scalars_finish adds them up in a fixed order).
 __global__ __launch_bounds__(256) void k_band_combine(const float* part, int64_t n,
                                                       int nbands, const int32_t* sidx,
                                                       const float* g, int sigma,
                                                       float* lam_sorted, float* lambda,
-                                                      double* scalars) {
+                                                      double* scal_part) {
   __shared__ double red[4][5];
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0, msum = 0.0;
-  if (e < n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
     float lam = 0.0f, sz = 0.0f, sdl = 0.0f, lmx = -1e30f;
     const float* p = part + (size_t)e * nbands * 4;
     for (int j = 0; j < nbands; ++j) {
@@ -901,11 +955,12 @@ __global__ __launch_bounds__(256) void k_band_combine(const float* part, int64_t
     lambda[orig] = lam;
     if (sigma) {
       const float ge = g ? g[orig] : (lam > 0.0f ? 1.0f / lam : 0.0f);
-      if (lam > 0.0f) ll = log((double)lam);
-      gsz = (double)ge * sz;
-      gsdl = (double)ge * sdl;
-      mmax = (double)fabsf(ge) * (double)exp2f(lmx);
-      msum = mmax;
+      if (lam > 0.0f) ll += log((double)lam);
+      gsz += (double)ge * sz;
+      gsdl += (double)ge * sdl;
+      const double m = (double)fabsf(ge) * (double)exp2f(lmx);
+      mmax = fmax(mmax, m);
+      msum += m;
     }
   }
   if (sigma) {
@@ -933,12 +988,8 @@ __global__ __launch_bounds__(256) void k_band_combine(const float* part, int64_t
         red[0][3] = fmax(red[0][3], red[i][3]);
         red[0][4] += red[i][4];
       }
-      unsafeAtomicAdd(&scalars[0], red[0][0]);
-      unsafeAtomicAdd(&scalars[1], red[0][1]);
-      unsafeAtomicAdd(&scalars[2], red[0][2]);
-      atomicMax(reinterpret_cast<unsigned long long*>(&scalars[3]),
-                (unsigned long long)__double_as_longlong(red[0][3]));
-      unsafeAtomicAdd(&scalars[4], red[0][4]);
+      double* p = scal_part + (size_t)blockIdx.x * kScalStride;
+      for (int i = 0; i < 5; ++i) p[i] = red[0][i];
     }
   }
 }
@@ -980,9 +1031,13 @@ hipError_t launch_strip_banded(hipStream_t st, int mode, int band, const FusedAr
 
 hipError_t launch_band_combine(hipStream_t st, const FusedArgs& a, int64_t n, int sigma) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_band_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     a.part, n, a.nbands, a.sidx, a.g, sigma, a.lam_sorted, a.lambda,
-                     a.scalars);
+  const int blocks = (int)std::min<int64_t>(kCombineBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(k_band_combine, dim3((unsigned)blocks), dim3(256), 0, st, a.part, n,
+                     a.nbands, a.sidx, a.g, sigma, a.lam_sorted, a.lambda, a.scal_part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !sigma) return e;
+  hipLaunchKernelGGL(k_scalars_finish, dim3(1), dim3(256), 0, st, a.scal_part,
+                     (int64_t)blocks, a.scalars);
   return hipGetLastError();
 }
 

@@ -169,6 +169,34 @@ def test_workload_256px_gradient(gpu_pkg):
     assert ll == pytest.approx(ll_ref, rel=1e-6)
 
 
+@pytest.mark.parametrize("bands", [0, 1])
+def test_sigma_gradient_reproducible(gpu_pkg, bands):
+    """The sigma gradient and the log-likelihood terms are summed per
+    workgroup (per block of the band combine) and then in a fixed order, so
+    two calls give the same bits; the fused form and the banded form agree
+    with each other and with the oracle."""
+    pkg = gpu_pkg
+    sc, ev, rho, _ = setup(pkg, n=50000, seed=5)
+    with pkg.Reconstruction(sc) as rec:
+        rec.set_options(force_bands=bands)
+        rec.set_events(ev)
+        rec.rho = rho
+        out = [rec.backward(want_loglik=True) for _ in range(3)]
+        fw = [rec.backward(g=np.full(len(ev), 0.5, np.float32), want_rho=False)
+              for _ in range(2)]
+    ll_ref, _, g_ref, gs_ref = ol.loglik_f64(sc, ev, rho)
+    print(f"bands {bands}: sigma {out[0][1]} vs {gs_ref}, loglik {out[0][2]} vs {ll_ref}",
+          flush=True)
+    for grad, gsig, ll in out[1:]:
+        assert gsig == out[0][1]
+        assert ll == out[0][2]
+        assert np.array_equal(grad, out[0][0])
+    assert fw[1][1] == fw[0][1]
+    assert out[0][1][0] == pytest.approx(gs_ref[0], rel=2e-4)
+    assert out[0][1][1] == pytest.approx(gs_ref[1], rel=2e-4)
+    assert out[0][2] == pytest.approx(ll_ref, rel=1e-6)
+
+
 class _EventEllipses:
     """FP64 centres and ellipse coefficients of every event (response.h:22-37,
     gaussian_kernel.h:108-137), to find the events whose ellipse may hold a
