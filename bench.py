@@ -148,6 +148,44 @@ def hybrid_report(pkg, name, steps, no_cpu, cpu_threads):
     return out
 
 
+def time_gradient(rec, reps=3):
+    """Median wall time (ms) of the full image + (sigma_z, sigma_dl) gradient
+    of the Poisson list-mode log-likelihood at the current image: one launch
+    computing lambda, the sigma derivative and the adjoint, kernels
+    recomputed, plus the copy of the W*H gradient to the host."""
+    import numpy as np
+    import torch
+    rec.backward()
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        g0 = time.perf_counter()
+        rec.backward()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - g0) * 1e3)
+    return float(np.median(times))
+
+
+def gradient_report(pkg, name):
+    """The gradient against one MLEM iteration on the differentiation
+    workload (256 x 256 pixels of 2 mm, 5e6 events of the three-ellipse
+    phantom), events generated on the device, after two MLEM iterations."""
+    import numpy as np
+    n_pix, size, n_ev, _, phantom, seed = pkg.WORKLOADS[name]
+    sc = pkg.workload_scanner(name)
+    with pkg.Reconstruction(sc, device=0) as rec:
+        rec.set_phantom_events(phantom, n_ev, seed=seed)
+        rec(2)
+        it_ms = []
+        for _ in range(5):
+            rec(1)
+            it_ms.append(rec.stats()["last_iteration_ms"])
+        g_ms = time_gradient(rec)
+    mlem = float(np.median(it_ms))
+    return {"workload": name, "events": n_ev, "mlem_iteration_ms": mlem,
+            "grad_ms": g_ms, "grad_over_mlem": g_ms / mlem}
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -165,6 +203,9 @@ def parse():
                    help="3D hybrid workload measured beside the headline line "
                         "('none' skips it)")
     p.add_argument("--no-grad", action="store_true")
+    p.add_argument("--grad-workload", default="strip-256px-2mm-5e6ev",
+                   help="workload of the gradient-vs-MLEM-iteration line "
+                        "('none' skips it)")
     p.add_argument("--calibrate", type=int, default=0, metavar="ROUNDS",
                    help="also run ROUNDS rounds of sigma calibration (5 MLEM "
                         "iterations + one gradient step on (sigma_z, sigma_dl) "
@@ -336,20 +377,11 @@ def main():
         torch.cuda.synchronize()
         out["setup_s"]["generate_on_device_and_bin"] = time.perf_counter() - t0
     if not args.no_grad:
-        # full image + (sigma_z, sigma_dl) gradient of the Poisson list-mode
-        # log-likelihood at the current image: one forward pass (lambda, sigma
-        # derivative) and one adjoint pass, kernels recomputed; includes the
-        # copy of the W*H gradient to the host.  Median of 3.
-        rec.backward()
-        times = []
-        for _ in range(3):
-            torch.cuda.synchronize()
-            g0 = time.perf_counter()
-            rec.backward()
-            torch.cuda.synchronize()
-            times.append((time.perf_counter() - g0) * 1e3)
-        out["grad_ms"] = float(np.median(times))
+        out["grad_ms"] = time_gradient(rec)
+        out["grad_over_mlem"] = out["grad_ms"] / float(np.median(iter_ms))
     rec.close()
+    if not args.no_grad and args.grad_workload != "none":
+        out["gradient"] = gradient_report(pkg, args.grad_workload)
     cal_state = None
     if args.calibrate > 0:
         with pkg.Reconstruction(sc, device=0, accumulation=acc) as cal:
