@@ -99,10 +99,19 @@ int petstrip_hybrid_set_events(petstrip_hybrid* h, const petstrip_hybrid_event* 
 int petstrip_hybrid_get_events(petstrip_hybrid* h, petstrip_hybrid_event* events);
 
 /* Replaces the weights (n_pixel_infos) and/or the sensitivity map (rows x
- * columns), e.g. normalised by the caller; either may be NULL. */
+ * columns), e.g. normalised by the caller; either may be NULL.  A map
+ * replaces an external volume set below. */
 int petstrip_hybrid_set_weights(petstrip_hybrid* h, const float* weights,
                                 const float* sensitivity);
 int petstrip_hybrid_get_weights(petstrip_hybrid* h, float* weights, float* sensitivity);
+/* External 3D sensitivity, the command's --sensitivity
+ * (src/3d/hybrid/reconstruction_cmd.cpp:162,505-519): depth = n_planes / 2
+ * planes of rows x columns, voxel (x, y, p) at x + W (y + H p).  The
+ * geometry weights are normalised by plane 0 (reconstruction.h:255-271) and
+ * the denominator takes plane |iz - depth| for plane iz (reconstruction.h:
+ * 357-362; at iz = 0, one plane past the reference's volume, plane depth - 1). */
+int petstrip_hybrid_set_sensitivity_3d(petstrip_hybrid* h, const float* sensitivity,
+                                       int32_t depth);
 
 /* The voxel image, planes x rows x columns (voxel index x + W (y + H z)). */
 int petstrip_hybrid_set_rho(petstrip_hybrid* h, const float* rho);

@@ -11,8 +11,12 @@
 // counts) and each output iteration (base_NNN + .nrrd and raw or text
 // floats, planes x rows x columns, optionally cropped) are written as the
 // reference's --gpu path writes them.  -G is accepted (the GPU always
-// runs); the reference's per-plane system matrices (--system), external 3D
-// sensitivity (--sensitivity) and inactive detectors are not supported.
+// runs).  --sensitivity reads an external volume of n_planes / 2 planes of
+// raw floats (the reference's binary VoxelMap, voxel_map.h:175-178); the
+// reference's text reader assigns to copies of the voxels and leaves the
+// volume unset (voxel_map.h:153-158), so a .txt volume is refused.  The
+// reference's per-plane system matrices (--system) and inactive detectors are
+// not supported.
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -43,7 +47,7 @@ const std::vector<OptSpec> kOptions = {
     {"s-z", 0, true, "0.015", true, kAlwaysSave, "TOF sigma along z axis"},
     {"s-dl", 0, true, "0.060", true, kAlwaysSave, "TOF sigma axis"},
     {"sens-to-one", 0, false, "", true, kSave, "sets sensitivity to one"},
-    {"sensitivity", 0, true, "", true, kSave, "external 3D sensitivity (not supported)"},
+    {"sensitivity", 0, true, "", true, kSave, "external 3D sensitivity"},
     {"inactive", 0, true, "", true, kSave, "list of inactive detectors (not supported)"},
     {"crop", 0, true, "0", true, kSave, "numer of pixel to crop the output image"},
     {"crop-x", 0, true, "0", true, kSave, "crop origin pixel x"},
@@ -255,7 +259,7 @@ int run(int argc, char** argv) {
           "note: All length options below should be expressed in meters.");
     return 0;
   }
-  for (const char* o : {"system", "sensitivity", "inactive"})
+  for (const char* o : {"system", "inactive"})
     if (a.exist(o))
       throw std::runtime_error(std::string("--") + o + " is not supported by this build");
   if (!a.exist("geometry"))
@@ -282,6 +286,26 @@ int run(int argc, char** argv) {
     std::cerr << "3D hybrid reconstruction w/geometry:\n    detectors = " << g.n_detectors
               << "\n  planes = " << n_planes << ", z-left = " << z_left << std::endl;
 
+  // reconstruction_cmd.cpp:505-519: an external volume takes precedence over
+  // --sens-to-one and normalises the geometry weights; read before any
+  // device work
+  std::vector<float> vol;
+  if (a.exist("sensitivity")) {
+    const std::string sn = a.get("sensitivity");
+    if (strip_io::ext(sn) == ".txt")
+      throw std::runtime_error("a text sensitivity volume is not read (give raw floats): " + sn);
+    const int depth = n_planes / 2;
+    if (depth < 1) throw std::runtime_error("--sensitivity needs at least two planes");
+    vol.resize((size_t)g.n_columns * g.n_rows * depth);
+    std::ifstream in(sn, std::ios::binary);
+    if (!in.is_open()) throw std::runtime_error("cannot open sensitivity file: " + sn);
+    in.read(reinterpret_cast<char*>(vol.data()), (std::streamsize)(vol.size() * sizeof(float)));
+    if ((size_t)in.gcount() != vol.size() * sizeof(float))
+      throw std::runtime_error("sensitivity file shorter than " + std::to_string(depth) +
+                               " planes of " + std::to_string(g.n_columns) + " x " +
+                               std::to_string(g.n_rows) + " floats: " + sn);
+  }
+
   petstrip_hybrid_geometry hg{g.n_detectors, g.n_columns, g.n_rows, n_planes, g.pixel_size,
                               g.llx, g.lly, z_left, (int64_t)g.positions.size(),
                               g.segments.data(), g.widths.data(), g.begin.data(),
@@ -296,6 +320,8 @@ int run(int argc, char** argv) {
     petstrip_hybrid* h;
     ~Guard() { petstrip_hybrid_destroy(h); }
   } guard{h};
+
+  if (!vol.empty()) check(petstrip_hybrid_set_sensitivity_3d(h, vol.data(), n_planes / 2));
 
   const size_t nv = (size_t)g.n_columns * g.n_rows * n_planes;
   std::vector<float> rho(nv, 1.0f);

@@ -278,7 +278,8 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
     const SegD* __restrict__ seg, const int32_t* __restrict__ pixels,
     const float* __restrict__ weights, const float* __restrict__ sens, int32_t n_infos,
     HGrid grid, HKernel k, const HEvent* __restrict__ events, int64_t n,
-    const float* __restrict__ rho, float* __restrict__ acc) {
+    const float* __restrict__ rho, float* __restrict__ acc, const float* __restrict__ sens3,
+    int sens_depth) {
   __shared__ PixTerm table_all[kWavesPerBlock][kMaxPix];
   __shared__ float cache_all[kWavesPerBlock][kCacheVox];
   const int lane = threadIdx.x & 63;
@@ -347,7 +348,16 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
         float kr = 0.0f;
         if (!(k2 <= 0)) {
           kr = k2 * rho[T.vox0 + iz];
-          den += kr * T.ws;
+          float ws = T.ws;
+          if (sens_depth > 0) {
+            // an external volume of D planes, read at plane |iz - D|
+            // (reconstruction.h:357-362); |iz - D| = D is one plane past the
+            // volume there and reads plane D - 1 here
+            int p = iz < sens_depth ? sens_depth - iz : iz - sens_depth;
+            p = p < sens_depth - 1 ? p : sens_depth - 1;
+            ws *= sens3[T.vox0 + p];
+          }
+          den += kr * ws;
         }
         if (j < kCacheVox) cache[j] = kr;
         i += step_i;
@@ -456,6 +466,8 @@ struct petstrip_hybrid {
   int64_t n_lors = 0, n_infos = 0, n_voxels_grid = 0;
   std::vector<int64_t> info_begin, info_end;
   std::vector<float> weights, sens;
+  std::vector<float> raw_weights;  // before normalisation
+  std::vector<int64_t> pix_index;  // pixel (x + W y) of every pixel info
   SegD* d_seg = nullptr;
   int64_t* d_lbeg = nullptr;
   int64_t* d_lend = nullptr;
@@ -463,6 +475,10 @@ struct petstrip_hybrid {
   int32_t* d_pix = nullptr;
   float* d_w = nullptr;
   float* d_sens = nullptr;
+  // external sensitivity volume (sens_depth planes, 0: none), plane-fastest
+  // with the image's stride: plane p of pixel pix at pix * planes + p
+  float* d_sens3 = nullptr;
+  int sens_depth = 0;
   float* d_rho = nullptr;
   float* d_acc = nullptr;
   HEvent* d_events = nullptr;
@@ -522,7 +538,8 @@ int one_iteration(petstrip_hybrid* h) {
                                              256 * 64);
     hipLaunchKernelGGL(k_hybrid_mlem, dim3((unsigned)blocks), dim3(kBlock), 0, h->stream,
                        h->d_seg, h->d_pix, h->d_w, h->d_sens, (int32_t)h->n_infos, h->grid,
-                       h->kern, h->d_events, h->n_events, h->d_rho, h->d_acc);
+                       h->kern, h->d_events, h->n_events, h->d_rho, h->d_acc, h->d_sens3,
+                       h->sens_depth);
     HTRY(hipGetLastError());
   }
   HTRY(hipEventRecord(h->k1, h->stream));
@@ -611,6 +628,10 @@ int petstrip_hybrid_create(const petstrip_hybrid_geometry* g, double sigma_z, do
   }
   // sensitivity and the normalised weights (reconstruction.h:236-265)
   const size_t np = (size_t)g->n_columns * g->n_rows;
+  h->raw_weights = h->weights;
+  h->pix_index.resize(h->n_infos);
+  for (int64_t i = 0; i < h->n_infos; ++i)
+    h->pix_index[i] = g->pixels[2 * i] + (int64_t)g->n_columns * g->pixels[2 * i + 1];
   h->sens.assign(np, (flags & PETSTRIP_HYBRID_SENSITIVITY_ONE) ? 1.0f : 0.0f);
   if (!(flags & PETSTRIP_HYBRID_SENSITIVITY_ONE))
     for (int64_t i = 0; i < h->n_infos; ++i)
@@ -669,6 +690,7 @@ void petstrip_hybrid_destroy(petstrip_hybrid* h) {
   dfree(h->d_pix);
   dfree(h->d_w);
   dfree(h->d_sens);
+  dfree(h->d_sens3);
   dfree(h->d_rho);
   dfree(h->d_acc);
   dfree(h->d_events);
@@ -815,8 +837,36 @@ int petstrip_hybrid_set_weights(petstrip_hybrid* h, const float* weights,
                                 const float* sensitivity) {
   if (!h) return set_error(PETSTRIP_EINVAL, "bad argument");
   if (weights) std::memcpy(h->weights.data(), weights, h->n_infos * sizeof(float));
-  if (sensitivity) std::memcpy(h->sens.data(), sensitivity, h->sens.size() * sizeof(float));
+  if (sensitivity) {
+    std::memcpy(h->sens.data(), sensitivity, h->sens.size() * sizeof(float));
+    h->sens_depth = 0;  // the map replaces an external volume
+  }
   HTRY(hipSetDevice(h->device));
+  return upload_weights(h);
+}
+
+// reconstruction_cmd.cpp:499-519 with --sensitivity: the geometry weights
+// normalised by the volume's plane 0 (reconstruction.h:255-271) and the
+// volume in the denominator (reconstruction.h:357-362).
+int petstrip_hybrid_set_sensitivity_3d(petstrip_hybrid* h, const float* sensitivity,
+                                       int32_t depth) {
+  if (!h || !sensitivity) return set_error(PETSTRIP_EINVAL, "bad argument");
+  const int planes = h->grid.planes;
+  if (depth < 1 || depth != planes / 2)
+    return set_error(PETSTRIP_EINVAL, "the sensitivity volume must have n_planes / 2 planes");
+  const size_t np = (size_t)h->grid.w * h->grid.h;
+  std::vector<float> t((size_t)h->n_voxels_grid, 0.0f);
+  for (int p = 0; p < depth; ++p)
+    for (size_t pix = 0; pix < np; ++pix) t[pix * planes + p] = sensitivity[pix + np * p];
+  for (int64_t i = 0; i < h->n_infos; ++i)
+    h->weights[i] = h->raw_weights[i] / sensitivity[h->pix_index[i]];
+  h->sens.assign(np, 1.0f);
+  HTRY(hipSetDevice(h->device));
+  int rc;
+  if (!h->d_sens3 && (rc = dalloc(&h->d_sens3, (size_t)h->n_voxels_grid))) return rc;
+  HTRY(hipMemcpyAsync(h->d_sens3, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice,
+                      h->stream));
+  h->sens_depth = depth;
   return upload_weights(h);
 }
 

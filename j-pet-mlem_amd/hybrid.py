@@ -58,6 +58,7 @@ def _lib():
             "petstrip_hybrid_get_events": ([vp, vp], C.c_int),
             "petstrip_hybrid_set_weights": ([vp, vp, vp], C.c_int),
             "petstrip_hybrid_get_weights": ([vp, vp, vp], C.c_int),
+            "petstrip_hybrid_set_sensitivity_3d": ([vp, vp, i32], C.c_int),
             "petstrip_hybrid_set_rho": ([vp, vp], C.c_int),
             "petstrip_hybrid_get_rho": ([vp, vp], C.c_int),
             "petstrip_hybrid_iterate": ([vp, i32], C.c_int),
@@ -152,6 +153,17 @@ class HybridReconstruction:
         w = None if weights is None else np.ascontiguousarray(weights, np.float32)
         s = None if sensitivity is None else np.ascontiguousarray(sensitivity, np.float32)
         _check(_lib().petstrip_hybrid_set_weights(self._h, _p(w), _p(s)))
+
+    def set_sensitivity_3d(self, volume):
+        """External 3D sensitivity (the command's --sensitivity,
+        reconstruction_cmd.cpp:505-519): n_planes / 2 planes of rows x
+        columns; the geometry weights are normalised by plane 0 and the
+        denominator of plane iz takes plane |iz - n_planes / 2|."""
+        v = np.ascontiguousarray(volume, np.float32)
+        if v.ndim != 3 or v.shape[1:] != self.shape[1:]:
+            raise ValueError(f"sensitivity volume of shape (D, rows, columns) expected, "
+                             f"got {v.shape}")
+        _check(_lib().petstrip_hybrid_set_sensitivity_3d(self._h, _p(v), v.shape[0]))
 
     @property
     def rho(self):

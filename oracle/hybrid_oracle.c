@@ -103,7 +103,10 @@ void oracle_hybrid_weights_f32(const oracle_hybrid_geometry* g, int sensitivity_
     }
   }
   const size_t np = (size_t)g->n_columns * g->n_rows;
-  for (size_t p = 0; p < np; ++p) sensitivity[p] = sensitivity_one ? 1.0f : 0.0f;
+  /* sensitivity_one = 2: the caller's map (an external volume's plane 0,
+   * reconstruction_cmd.cpp:505-519) normalises the weights as it is */
+  if (sensitivity_one != 2)
+    for (size_t p = 0; p < np; ++p) sensitivity[p] = sensitivity_one ? 1.0f : 0.0f;
   if (!sensitivity_one)
     for (int64_t i = 0; i < g->n_pixel_infos; ++i)
       sensitivity[g->pixels[2 * i] + (size_t)g->n_columns * g->pixels[2 * i + 1]] +=
@@ -128,7 +131,10 @@ void oracle_hybrid_weights_f64(const oracle_hybrid_geometry* g, int sensitivity_
     }
   }
   const size_t np = (size_t)g->n_columns * g->n_rows;
-  for (size_t p = 0; p < np; ++p) sensitivity[p] = sensitivity_one ? 1.0 : 0.0;
+  /* sensitivity_one = 2: the caller's map (an external volume's plane 0,
+   * reconstruction_cmd.cpp:505-519) normalises the weights as it is */
+  if (sensitivity_one != 2)
+    for (size_t p = 0; p < np; ++p) sensitivity[p] = sensitivity_one ? 1.0 : 0.0;
   if (!sensitivity_one)
     for (int64_t i = 0; i < g->n_pixel_infos; ++i)
       sensitivity[g->pixels[2 * i] + (size_t)g->n_columns * g->pixels[2 * i + 1]] +=
@@ -273,14 +279,25 @@ int64_t oracle_hybrid_frames_f32(const oracle_hybrid_geometry* g, double sigma_z
 }
 
 /* ------------------------------------------------------------------------ */
+/* Index of the sensitivity factor of voxel (pix, iz): the pixel's entry of
+ * the W x H map, or plane |iz - D| of an external volume of D planes
+ * (reconstruction.h:357-362); |iz - D| = D, past the volume, reads D - 1. */
+static inline size_t sens_index(size_t pix, size_t np, int iz, int depth) {
+  if (depth <= 0) return pix;
+  int p = iz < depth ? depth - iz : iz - depth;
+  if (p > depth - 1) p = depth - 1;
+  return pix + np * (size_t)p;
+}
+
 /* The iteration (reconstruction.h:267-445) with one thread: per event the
  * denominator over (pixel info, plane) in that order, each voxel's weight
  * kept in a voxel-indexed cache, then every (pixel info, plane) adds its
  * voxel's cached weight over the denominator; rho becomes the sum. */
 void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
                             double sigma_dl, double length, const float* weights,
-                            const float* sensitivity, const oracle_hybrid_event* ev,
-                            int64_t n, int n_iter, const float* rho0, float* out) {
+                            const float* sensitivity, int sens_depth,
+                            const oracle_hybrid_event* ev, int64_t n, int n_iter,
+                            const float* rho0, float* out) {
   const k32 k = kernel_consts_f(sigma_z, sigma_dl);
   const grid32 r = grid_f(g);
   const float L = (float)length, z_left = (float)g->z_left;
@@ -314,7 +331,7 @@ void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
             continue;
           }
           const float wt = k2 * w * rho[vox];
-          denom += wt * sensitivity[pix];
+          denom += wt * sensitivity[sens_index(pix, np, iz, sens_depth)];
           cache[vox] = wt;
         }
       }
@@ -339,8 +356,9 @@ void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
 
 void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
                             double sigma_dl, double length, const double* weights,
-                            const double* sensitivity, const oracle_hybrid_event* ev,
-                            int64_t n, int n_iter, const double* rho0, double* out) {
+                            const double* sensitivity, int sens_depth,
+                            const oracle_hybrid_event* ev, int64_t n, int n_iter,
+                            const double* rho0, double* out) {
   const double isz = 1 / (sigma_z * sigma_z), isdl = 1 / (sigma_dl * sigma_dl);
   const double c = 1 / (2 * M_PI * M_PI) * sqrt(isz * isz * isdl);
   const size_t np = (size_t)g->n_columns * g->n_rows;
@@ -373,7 +391,7 @@ void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
             continue;
           }
           const double wt = k2 * weights[i] * rho[vox];
-          denom += wt * sensitivity[pix];
+          denom += wt * sensitivity[sens_index(pix, np, iz, sens_depth)];
           cache[vox] = wt;
         }
       }

@@ -45,7 +45,9 @@ typedef struct {
 
 /* calculate_weight, calculate_sensitivity (or set_sensitivity_to_one) and
  * normalize_geometry_weights (reconstruction.h:212-265): weights
- * (n_pixel_infos) and the W x H sensitivity map. */
+ * (n_pixel_infos) and the W x H sensitivity map.  sensitivity_one = 2 takes
+ * the map as given (plane 0 of an external sensitivity volume) and only
+ * normalises the weights by it. */
 void oracle_hybrid_weights_f32(const oracle_hybrid_geometry* g, int sensitivity_one,
                                float* weights, float* sensitivity);
 void oracle_hybrid_weights_f64(const oracle_hybrid_geometry* g, int sensitivity_one,
@@ -59,15 +61,21 @@ int64_t oracle_hybrid_frames_f32(const oracle_hybrid_geometry* g, double sigma_z
                                  const float* zzd, int64_t n, oracle_hybrid_event* out);
 
 /* n_iter iterations (reconstruction.h:267-445) from rho (W x H x planes);
- * the image after every iteration goes to out (n_iter x voxels). */
+ * the image after every iteration goes to out (n_iter x voxels).
+ * sens_depth = 0: sensitivity is the W x H map of the pixel (reconstruction.h
+ * :364-367); sens_depth = D > 0: it is an external volume of D planes of
+ * W x H (D = planes / 2) read at plane |iz - D| (reconstruction.h:357-362),
+ * where plane D (iz = 0, one plane past the volume) reads plane D - 1. */
 void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
                             double sigma_dl, double length, const float* weights,
-                            const float* sensitivity, const oracle_hybrid_event* ev,
-                            int64_t n, int n_iter, const float* rho, float* out);
+                            const float* sensitivity, int sens_depth,
+                            const oracle_hybrid_event* ev, int64_t n, int n_iter,
+                            const float* rho, float* out);
 void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
                             double sigma_dl, double length, const double* weights,
-                            const double* sensitivity, const oracle_hybrid_event* ev,
-                            int64_t n, int n_iter, const double* rho, double* out);
+                            const double* sensitivity, int sens_depth,
+                            const oracle_hybrid_event* ev, int64_t n, int n_iter,
+                            const double* rho, double* out);
 
 #ifdef __cplusplus
 }

@@ -56,7 +56,7 @@ int ref_hybrid_run(int n_detectors, int n_columns, int n_rows, float pixel_size,
                    int sensitivity_one, const int16_t* lors, const float* zzd, int64_t n_events,
                    int n_iterations, float* out_weights, float* out_sensitivity,
                    int64_t* out_n_kept, double* out_frames, float* out_rho, int n_threads,
-                   double* iteration_seconds) {
+                   double* iteration_seconds, const float* sensitivity_3d) {
   try {
 #if _OPENMP
     omp_set_num_threads(n_threads > 0 ? n_threads : 1);
@@ -90,9 +90,14 @@ int ref_hybrid_run(int n_detectors, int n_columns, int n_rows, float pixel_size,
     Scanner scanner(barrel, length);
     scanner.set_sigmas(sigma_z, sigma_dl);
     Reconstruction::Grid grid3d(grid, z_left, (S)n_planes);
-    Reconstruction rec(scanner, grid3d, soa, false);
+    // an external volume: n_planes / 2 planes of W x H, as the command's
+    // --sensitivity reads it (reconstruction_cmd.cpp:499-519)
+    Reconstruction rec(scanner, grid3d, soa, sensitivity_3d != nullptr);
     rec.calculate_weight();
-    if (sensitivity_one)
+    if (sensitivity_3d)
+      std::memcpy(rec.sensitivity.data, sensitivity_3d,
+                  sizeof(F) * (size_t)n_columns * n_rows * (n_planes / 2));
+    else if (sensitivity_one)
       rec.set_sensitivity_to_one();
     else
       rec.calculate_sensitivity();

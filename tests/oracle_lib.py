@@ -258,20 +258,28 @@ def _hyb():
         o.oracle_hybrid_weights_f64.argtypes = [G, C.c_int, vp, vp]
         o.oracle_hybrid_frames_f32.argtypes = [G, d, d, vp, vp, vp, i64, vp]
         o.oracle_hybrid_frames_f32.restype = i64
-        o.oracle_hybrid_mlem_f32.argtypes = [G, d, d, d, vp, vp, vp, i64, C.c_int, vp, vp]
-        o.oracle_hybrid_mlem_f64.argtypes = [G, d, d, d, vp, vp, vp, i64, C.c_int, vp, vp]
+        o.oracle_hybrid_mlem_f32.argtypes = [G, d, d, d, vp, vp, C.c_int, vp, i64, C.c_int,
+                                             vp, vp]
+        o.oracle_hybrid_mlem_f64.argtypes = [G, d, d, d, vp, vp, C.c_int, vp, i64, C.c_int,
+                                             vp, vp]
         _h = o
     return _h
 
 
-def hybrid_weights(g, sensitivity_one=False, f64=False):
-    """(weights, W x H sensitivity) after normalisation."""
+def hybrid_weights(g, sensitivity_one=False, f64=False, external=None):
+    """(weights, W x H sensitivity) after normalisation; external: a W x H
+    map (an external volume's plane 0) that normalises the weights instead."""
     t = np.float64 if f64 else np.float32
     w = np.zeros(g.n_pixel_infos, t)
-    s = np.zeros((g.n_rows, g.n_columns), t)
+    if external is not None:
+        s = np.array(external, t).reshape(g.n_rows, g.n_columns)
+        mode = 2
+    else:
+        s = np.zeros((g.n_rows, g.n_columns), t)
+        mode = int(sensitivity_one)
     og = ohybrid(g)
     fn = _hyb().oracle_hybrid_weights_f64 if f64 else _hyb().oracle_hybrid_weights_f32
-    fn(C.byref(og), int(sensitivity_one), P(w), P(s))
+    fn(C.byref(og), mode, P(w), P(s))
     return w, s
 
 
@@ -287,8 +295,12 @@ def hybrid_frames(g, sigma_z, sigma_dl, weights, lors, zzd):
 
 def hybrid_mlem(g, sigma_z, sigma_dl, length, weights, sensitivity, events, n_iter,
                 rho0=None, f64=False):
-    """Images after every iteration, (n_iter, planes, rows, columns)."""
+    """Images after every iteration, (n_iter, planes, rows, columns).  A 2D
+    sensitivity (rows, columns) is the pixel's factor; a 3D one (D, rows,
+    columns), D = planes / 2, is an external volume read at plane |iz - D|."""
     t = np.float64 if f64 else np.float32
+    sensitivity = np.asarray(sensitivity)
+    depth = sensitivity.shape[0] if sensitivity.ndim == 3 else 0
     shape = (g.n_planes, g.n_rows, g.n_columns)
     rho = np.ones(shape, t) if rho0 is None else np.ascontiguousarray(rho0, t)
     out = np.zeros((n_iter,) + shape, t)
@@ -296,8 +308,8 @@ def hybrid_mlem(g, sigma_z, sigma_dl, length, weights, sensitivity, events, n_it
     events = np.ascontiguousarray(events, HYBRID_EVENT)
     fn = _hyb().oracle_hybrid_mlem_f64 if f64 else _hyb().oracle_hybrid_mlem_f32
     fn(C.byref(og), sigma_z, sigma_dl, length, P(np.ascontiguousarray(weights, t)),
-       P(np.ascontiguousarray(sensitivity, t)), P(events), len(events), n_iter, P(rho),
-       P(out))
+       P(np.ascontiguousarray(sensitivity, t)), depth, P(events), len(events), n_iter,
+       P(rho), P(out))
     return out
 
 
@@ -306,7 +318,7 @@ def have_ref_hybrid():
 
 
 def ref_hybrid(g, sigma_z, sigma_dl, length, lors, zzd, n_iter, sensitivity_one=False,
-               n_threads=1, keep_images=True):
+               n_threads=1, keep_images=True, sensitivity_3d=None):
     """The reference's own reconstruction on the same geometry and responses:
     dict of weights, sensitivity, frames (kept events: first, second, up,
     right, tan, sec, begin, end, plane begin, end), images per iteration and
@@ -315,7 +327,8 @@ def ref_hybrid(g, sigma_z, sigma_dl, length, lors, zzd, n_iter, sensitivity_one=
     vp, i64, f = C.c_void_p, C.c_int64, C.c_float
     ci = C.c_int
     r.ref_hybrid_run.argtypes = ([ci, ci, ci, f, f, f, f, ci] + [vp] * 7 +
-                                 [f, f, f, ci, vp, vp, i64, ci, vp, vp, vp, vp, vp, ci, vp])
+                                 [f, f, f, ci, vp, vp, i64, ci, vp, vp, vp, vp, vp, ci, vp,
+                                  vp])
     r.ref_hybrid_run.restype = ci
     lors = np.ascontiguousarray(lors, np.int16)
     zzd = np.ascontiguousarray(zzd, np.float32)
@@ -333,7 +346,9 @@ def ref_hybrid(g, sigma_z, sigma_dl, length, lors, zzd, n_iter, sensitivity_one=
                           P(g.pixels), P(g.positions), P(g.weights), sigma_z, sigma_dl,
                           length, int(sensitivity_one), P(lors), P(zzd), n, n_iter, P(w),
                           P(s), C.byref(kept), P(frames), P(rho), int(n_threads),
-                          C.byref(seconds))
+                          C.byref(seconds),
+                          None if sensitivity_3d is None else
+                          P(np.ascontiguousarray(sensitivity_3d, np.float32)))
     if rc != 0:
         raise RuntimeError("reference hybrid harness failed")
     return {"weights": w, "sensitivity": s, "frames": frames[:kept.value], "rho": rho,

@@ -56,6 +56,46 @@ def test_hybrid_vs_reference_fixture(gpu_pkg, name):
             assert r < 1e-5
 
 
+def test_hybrid_3d_sensitivity_vs_oracle(gpu_pkg):
+    """An external 3D sensitivity (the command's --sensitivity): weights,
+    frames and two iterations against the oracle, which matches the
+    reference bit for bit on this case (test_hybrid_oracle.py); a 2D map set
+    afterwards replaces the volume again."""
+    import oracle_lib as ol
+    pkg = gpu_pkg
+    hg = pkg.hybrid_geometry
+    g = hg.ring_geometry(n_detectors=40, ring_radius=0.21, detector_width=0.008, n_pixels=28,
+                         pixel_size=0.009, n_planes=48)
+    lors, zzd = hg.cylinder_responses(g, 1200, sigma_z=0.011, sigma_dl=0.045, length=0.28,
+                                      seed=17)
+    vol = np.random.default_rng(17).uniform(0.5, 1.5, (24, 28, 28)).astype(np.float32)
+    print("case: 28x28x48 voxels, 24-plane sensitivity volume, 1200 responses", flush=True)
+    w_ref, _ = ol.hybrid_weights(g, external=vol[0])
+    fr = ol.hybrid_frames(g, 0.011, 0.045, w_ref, lors, zzd)
+    want = ol.hybrid_mlem(g, 0.011, 0.045, 0.28, w_ref, vol, fr, 2)
+    with pkg.HybridReconstruction(g, 0.011, 0.045, 0.28) as rec:
+        w2d, s2d = rec.weights()
+        with pytest.raises(Exception):
+            rec.set_sensitivity_3d(vol[:23])
+        rec.set_sensitivity_3d(vol)
+        w, s = rec.weights()
+        assert np.array_equal(w.view(np.uint32), w_ref.view(np.uint32))
+        assert np.all(s == 1.0)
+        rec.set_responses(lors, zzd)
+        assert frames_equal(rec.events(), fr)
+        for it in range(2):
+            rec(1)
+            r = rel(rec.rho, want[it])
+            print(f"  iteration {it + 1}: rel L2 vs oracle {r:.2e}", flush=True)
+            assert r < 1e-5
+        # back to the plain 2D sensitivity: the oracle's 2D run
+        rec.set_weights(w2d, s2d)
+        rec.rho = np.ones(rec.shape, np.float32)
+        rec(1)
+        want2d = ol.hybrid_mlem(g, 0.011, 0.045, 0.28, w2d, s2d, fr, 1)
+        assert rel(rec.rho, want2d[0]) < 1e-5
+
+
 def test_hybrid_vs_oracle_f64(gpu_pkg):
     import oracle_lib as ol
     pkg = gpu_pkg

@@ -73,6 +73,39 @@ def test_oracle_bitwise_vs_reference_fresh_case(pkg):
     assert same_bits(rho, r["rho"])
 
 
+def _case_3d_sensitivity(pkg):
+    """40 detectors, 28 x 28 pixels of 9 mm x 48 planes, an external volume
+    of 24 planes; the planes reach past the cylinder's kernels on both sides,
+    so no event touches plane 0 (which would read one plane past the
+    reference's volume)."""
+    hg = pkg.hybrid_geometry
+    g = hg.ring_geometry(n_detectors=40, ring_radius=0.21, detector_width=0.008, n_pixels=28,
+                         pixel_size=0.009, n_planes=48)
+    lors, zzd = hg.cylinder_responses(g, 1200, sigma_z=0.011, sigma_dl=0.045, length=0.28,
+                                      seed=17)
+    vol = np.random.default_rng(17).uniform(0.5, 1.5, (24, 28, 28)).astype(np.float32)
+    return g, lors, zzd, vol
+
+
+@pytest.mark.skipif(not ol.have_ref_hybrid(), reason="oracle/_ref/libref_hybrid.so not built")
+def test_oracle_bitwise_vs_reference_3d_sensitivity(pkg):
+    """--sensitivity (reconstruction_cmd.cpp:505-519): weights normalised by
+    the volume's plane 0, denominators over plane |iz - 24|."""
+    g, lors, zzd, vol = _case_3d_sensitivity(pkg)
+    r = ol.ref_hybrid(g, 0.011, 0.045, 0.28, lors, zzd, 2, sensitivity_3d=vol)
+    w, s = ol.hybrid_weights(g, external=vol[0])
+    fr = ol.hybrid_frames(g, 0.011, 0.045, w, lors, zzd)
+    assert fr["plane_begin"].min() >= 1
+    rho = ol.hybrid_mlem(g, 0.011, 0.045, 0.28, w, vol, fr, 2)
+    assert same_bits(w, r["weights"]) and same_bits(s, vol[0])
+    assert same_bits(r["sensitivity"], vol[0])
+    assert np.array_equal(frames_matrix(fr), r["frames"])
+    assert same_bits(rho, r["rho"])
+    # the planes beyond 0 matter: the volume's plane 0 repeated gives another image
+    flat = ol.hybrid_mlem(g, 0.011, 0.045, 0.28, w, np.repeat(vol[:1], 24, axis=0), fr, 1)
+    assert not np.allclose(flat[0], rho[0], rtol=1e-3)
+
+
 def test_sensitivity_one_conserves_events():
     """With sensitivity one the denominator is the sum of the weights, so
     every event with a positive denominator adds exactly one to the image
