@@ -637,14 +637,15 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         // weight of this event: upstream g_e or 1/lambda_e (log-likelihood)
         const float ge =
             a.g ? a.g[orig] : (lam > 0.0f ? __builtin_amdgcn_rcpf(lam) : 0.0f);
-        const float wsz = wave_sum(sz), wsdl = wave_sum(sdl);
+        // every lane adds g_e times its rows' sigma sums in doubles; the
+        // lanes are summed once, at the end, instead of two wave sums here
+        gsz = fma((double)ge, (double)sz, gsz);
+        gsdl = fma((double)ge, (double)sdl, gsdl);
         float kmax = 0.0f;
         if constexpr (MODE == MODE_FORWARD) kmax = __builtin_amdgcn_exp2f(wave_max_f(lmax));
         if (lane == 0) {
           // ln lambda from the hardware log2; the sum over events is in doubles
           if (lam > 0.0f) ll += (double)__builtin_amdgcn_logf(lam) * 0.69314718055994531;
-          gsz += (double)ge * (double)wsz;
-          gsdl += (double)ge * (double)wsdl;
           if constexpr (MODE == MODE_FORWARD) {
             const double m = (double)fabsf(ge) * (double)kmax;
             mmax = fmax(mmax, m);
@@ -774,10 +775,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   }
 
   if constexpr (kSigma && BAND == 0) {
-    // Lane 0 of each wave holds the wave's scalars.  The workgroup's sums go
+    // Lane 0 of each wave holds the wave's scalars (the sigma sums once its
+    // lanes are added).  The workgroup's sums go
     // to its own slot in a fixed wave order: five atomics per wave onto the
     // same five addresses queue at one L2 channel, and each wave's wait for
     // them held its workgroup for milliseconds on small footprints.
+    gsz = wave_sum_d(gsz);
+    gsdl = wave_sum_d(gsdl);
     __syncthreads();  // every wave is done with the LDS windows
     double* red = reinterpret_cast<double*>(smem);
     if (lane == 0) {
