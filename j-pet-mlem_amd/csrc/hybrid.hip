@@ -274,6 +274,8 @@ __device__ __forceinline__ float voxel_kernel(const HKernel& k, const PixTerm& T
 // One wave per event.  The image and the accumulator are plane-fastest
 // (voxel (x, y, z) at z + planes (x + W y)), so the lanes of a wave, which
 // walk consecutive planes of a pixel, read and add contiguous words.
+// S3: with an external 3D sensitivity volume (sens3, sens_depth planes).
+template <bool S3>
 __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
     const SegD* __restrict__ seg, const int32_t* __restrict__ pixels,
     const float* __restrict__ weights, const float* __restrict__ sens, int32_t n_infos,
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
         if (!(k2 <= 0)) {
           kr = k2 * rho[T.vox0 + iz];
           float ws = T.ws;
-          if (sens_depth > 0) {
+          if constexpr (S3) {
             // an external volume of D planes, read at plane |iz - D|
             // (reconstruction.h:357-362); |iz - D| = D is one plane past the
             // volume there and reads plane D - 1 here
@@ -536,9 +538,10 @@ int one_iteration(petstrip_hybrid* h) {
     // one wave per event; enough waves to fill the device several times over
     const int64_t blocks = std::min<int64_t>((h->n_events + kWavesPerBlock - 1) / kWavesPerBlock,
                                              256 * 64);
-    hipLaunchKernelGGL(k_hybrid_mlem, dim3((unsigned)blocks), dim3(kBlock), 0, h->stream,
-                       h->d_seg, h->d_pix, h->d_w, h->d_sens, (int32_t)h->n_infos, h->grid,
-                       h->kern, h->d_events, h->n_events, h->d_rho, h->d_acc, h->d_sens3,
+    auto kern = h->sens_depth > 0 ? k_hybrid_mlem<true> : k_hybrid_mlem<false>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kBlock), 0, h->stream, h->d_seg,
+                       h->d_pix, h->d_w, h->d_sens, (int32_t)h->n_infos, h->grid, h->kern,
+                       h->d_events, h->n_events, h->d_rho, h->d_acc, h->d_sens3,
                        h->sens_depth);
     HTRY(hipGetLastError());
   }
