@@ -225,7 +225,9 @@ int petstrip_forward(petstrip_handle* h, float* lambda);
  *   grad_sigma[1] = sum_e g_e d lambda_e / d sigma_dl
  * with the support (ellipse and box) held at the current sigma.  Kernels
  * are recomputed, not stored.  loglik (nullable) receives l.  Uses the
- * lambda of the last petstrip_forward (computes it if absent). */
+ * lambda of the last petstrip_forward (computes it if absent).  On the
+ * fused path grad_sigma and loglik are summed per workgroup and then in a
+ * fixed order, so they are bit-reproducible run to run. */
 int petstrip_backward(petstrip_handle* h, const float* g, float* grad_rho,
                       double grad_sigma[2], double* loglik);
 
