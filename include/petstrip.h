@@ -314,6 +314,34 @@ int petstrip_phantom_tan_kernel_map(const petstrip_scanner* scanner,
                                     double kernel_point_x, double kernel_point_y,
                                     double scale, float* out, int device);
 
+/* --- response files (input side; host only, no device needed) ------------- */
+
+/* Reads a text response file as the reference commands read `.txt`
+ * responses, with its fast text reader: Strip::Reconstruction::
+ * fast_load_txt_events (src/2d/strip/reconstruction.h:169-196) and the 3D
+ * hybrid Reconstruction's (src/3d/hybrid/reconstruction.h:165-188), both
+ * over util::text_parser (src/util/text_parser.h:29-175).  Its floats are
+ * formed in FP32 as float(whole) + float(fraction) x 0.1f^digits (the power
+ * by repeated float multiplication, the sum in one fused multiply-add),
+ * then scaled by the exponent (e > 0 multiplies by 10 x 0.1f^(e-1), as
+ * there), then signed; they are not the correctly rounded values of the
+ * decimal strings.  Lines end at '\n' (a last line without one is not
+ * read); fields are separated by spaces only.
+ *   columns = 3: "z_u z_d dl" per line;
+ *   columns = 5: "i j z_u z_d dl" (--3d-response, the 3D hybrid command);
+ *                i, j go to lors (2 per line, int16) when it is not NULL.
+ * responses == NULL: *n_out = the number of lines (the reader's first
+ * pass, reconstruction.h:171-173), nothing is parsed.  Otherwise up to
+ * `capacity` responses (3 floats each) are written and *n_out is their
+ * number.  A line that does not parse fails with "error line: <line>"
+ * (the reference prints the same and throws). */
+int petstrip_read_text_responses(const char* path, int32_t columns, float* responses,
+                                 int16_t* lors, int64_t capacity, int64_t* n_out);
+/* The same float reader on n space-separated fields of one string
+ * (util::text_parser::operator>> <float>, text_parser.h:140-186); consumed
+ * (nullable) receives the characters read. */
+int petstrip_parse_text_floats(const char* text, int32_t n, float* out, int32_t* consumed);
+
 /* Copies the handle's n_events responses (z_u, z_d, dl triples) to the host. */
 int petstrip_get_events(petstrip_handle* h, float* responses);
 

@@ -156,28 +156,13 @@ Geometry read_geometry(const std::string& fn) {
   return g;
 }
 
-// Responses: text "d1 d2 z_up z_dn dl" per line (reconstruction.h:163-186) or
-// binary int16 d1, d2, float z_up, z_dn, dl records
+// Responses: text "d1 d2 z_up z_dn dl" per line, read as the reference's
+// fast text parser reads them (reconstruction.h:165-188, through the
+// library's reader), or binary int16 d1, d2, float z_up, z_dn, dl records
 // (3d/hybrid/scanner.h:80-88).
 void load_responses(const std::string& fn, std::vector<int16_t>& lors, std::vector<float>& zzd) {
   if (strip_io::ext(fn) == ".txt") {
-    std::ifstream in(fn);
-    if (!in.is_open()) throw std::runtime_error("cannot open input response file: " + fn);
-    std::string line;
-    while (std::getline(in, line)) {
-      std::istringstream ls(line);
-      int a, b;
-      float zu, zd, dl;
-      if (!(ls >> a >> b >> zu >> zd >> dl)) {
-        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
-        throw std::runtime_error("error line: " + line);
-      }
-      lors.push_back((int16_t)a);
-      lors.push_back((int16_t)b);
-      zzd.push_back(zu);
-      zzd.push_back(zd);
-      zzd.push_back(dl);
-    }
+    strip_io::load_text_responses(fn, true, zzd, &lors);
     return;
   }
   std::ifstream in(fn, std::ios::binary);

@@ -102,33 +102,6 @@ void load_binary_responses(const std::string& fn, std::vector<float>& out) {
     throw std::runtime_error("short read: " + fn);
 }
 
-void load_text_responses(const std::string& fn, bool is_3d, std::vector<float>& out) {
-  std::ifstream in(fn);
-  if (!in.is_open()) throw std::runtime_error("cannot open phantom responses file: " + fn);
-  std::string line;
-  while (std::getline(in, line)) {
-    const char* p = line.c_str();
-    char* end = nullptr;
-    double v[5];
-    int need = is_3d ? 5 : 3, got = 0;
-    while (got < need) {
-      double x = std::strtod(p, &end);
-      if (end == p) break;
-      v[got++] = x;
-      p = end;
-    }
-    if (got == 0) {
-      bool blank = std::all_of(line.begin(), line.end(), [](char c) { return std::isspace((unsigned char)c); });
-      if (blank) continue;
-    }
-    if (got < need) throw std::runtime_error("error line: " + line);
-    const double* r = is_3d ? v + 2 : v;
-    out.push_back((float)r[0]);
-    out.push_back((float)r[1]);
-    out.push_back((float)r[2]);
-  }
-}
-
 void write_raw(const std::string& fn, const float* img, size_t n) {
   std::ofstream out(fn, std::ios::binary);
   if (!out.is_open()) throw std::runtime_error("cannot write " + fn);

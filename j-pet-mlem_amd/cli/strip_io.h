@@ -26,7 +26,11 @@ std::string add_index(const std::string& base, int index, int max_value);
 
 // Responses (z_u, z_d, dl) appended to out.  Throws std::runtime_error.
 void load_binary_responses(const std::string& fn, std::vector<float>& out);
-void load_text_responses(const std::string& fn, bool is_3d, std::vector<float>& out);
+// Text responses through the library's reader (responses_io.cpp, linked
+// with libpetstrip_hip): 3 fields per line, or 5 with is_3d, whose first
+// two (the LOR's detectors) go to lors when it is not null.
+void load_text_responses(const std::string& fn, bool is_3d, std::vector<float>& out,
+                         std::vector<int16_t>* lors = nullptr);
 
 // Images: width columns x height rows, row-major.
 void write_raw(const std::string& fn, const float* img, size_t n);

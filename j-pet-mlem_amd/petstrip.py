@@ -126,6 +126,10 @@ def load_library(path=LIB_PATH):
                                        C.c_int], C.c_int),
         "petstrip_phantom_tan_kernel_map": ([C.POINTER(CScanner), C.POINTER(CPhantomGrid),
                                              dp, dp, dp, fp, C.c_int], C.c_int),
+        "petstrip_read_text_responses": ([C.c_char_p, i32, fp, fp, i64,
+                                          C.POINTER(C.c_int64)], C.c_int),
+        "petstrip_parse_text_floats": ([C.c_char_p, i32, fp, C.POINTER(C.c_int32)],
+                                       C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -142,6 +146,37 @@ def _check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def read_text_responses(path, columns=3, with_lors=False):
+    """Responses of a text file as the reference's fast text reader reads
+    them (src/util/text_parser.h:29-175, src/2d/strip/reconstruction.h:
+    169-196): float32 (n, 3) of (z_u, z_d, dl); columns = 5 for
+    "i j z_u z_d dl" lines, whose (i, j) come back as int16 (n, 2) with
+    with_lors.  Host only."""
+    lib = load_library()
+    n = C.c_int64(0)
+    _check(lib.petstrip_read_text_responses(os.fsencode(path), columns, None, None, 0,
+                                            C.byref(n)))
+    ev = np.zeros((n.value, 3), np.float32)
+    lors = np.zeros((n.value, 2), np.int16) if with_lors else None
+    got = C.c_int64(0)
+    _check(lib.petstrip_read_text_responses(os.fsencode(path), columns, _ptr(ev), _ptr(lors),
+                                            n.value, C.byref(got)))
+    ev, lors = ev[:got.value], (lors[:got.value] if with_lors else None)
+    return (ev, lors) if with_lors else ev
+
+
+def parse_text_floats(text, n=1):
+    """n space-separated floats of a string, read as the reference's fast
+    text reader reads them (text_parser.h:140-186); returns (values,
+    characters consumed)."""
+    lib = load_library()
+    out = np.zeros(n, np.float32)
+    used = C.c_int32(0)
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    _check(lib.petstrip_parse_text_floats(raw, n, _ptr(out), C.byref(used)))
+    return out, used.value
 
 
 class Scanner:

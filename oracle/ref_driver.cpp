@@ -8,6 +8,8 @@
 //   - Strip::Scanner                      src/2d/strip/scanner.h:26-291
 //   - Strip::Response::calculate_tan_y_z  src/2d/strip/response.h:22-37
 //   - Strip::AnalyticKernel               src/2d/strip/analytic_kernel.h:16-87
+//   - util::text_parser                   src/util/text_parser.h:19-176
+//   - Reconstruction::fast_load_txt_events src/2d/strip/reconstruction.h:169-196
 // Only tests and the golden-vector script load the built library.
 #include <cstdint>
 #include <cstring>
@@ -16,6 +18,7 @@
 #include "2d/strip/reconstruction.h"
 #include "2d/strip/gaussian_kernel.h"
 #include "2d/strip/analytic_kernel.h"
+#include "util/text_parser.h"
 
 using F = float;
 using S = short;
@@ -140,6 +143,59 @@ void ref_strip_pixel_at(double radius, double length, int n_y, int n_z,
                   0.04);
   auto p = scanner.pixel_at(PET2D::Point<F>(z, y));
   out[0] = p.x; out[1] = p.y;
+}
+
+// Scanner::pixel_center (scanner.h:136-140) in float: {z, y}.
+void ref_strip_pixel_center(double radius, double length, int n_y, int n_z,
+                            double pixel_size, int x, int y, float out[2]) {
+  Scanner scanner(radius, length, n_y, n_z, pixel_size, pixel_size, 0.01, 0.04);
+  auto p = scanner.pixel_center(PET2D::Pixel<S>((S)x, (S)y));
+  out[0] = p.x;
+  out[1] = p.y;
+}
+
+// util::text_parser's readers on one line: n fields, each an int
+// (kinds[k] = 0, operator>><int>) or a float (kinds[k] = 1,
+// operator>><float>).  Returns -1 if the parser throws.
+int ref_text_parse_fields(const char* text, int n, const int* kinds, float* out_f,
+                          int* out_i) {
+  util::text_parser p(text);
+  try {
+    for (int k = 0; k < n; ++k) {
+      if (kinds[k] == 0) {
+        int v;
+        p >> v;
+        out_i[k] = v;
+      } else {
+        float v;
+        p >> v;
+        out_f[k] = v;
+      }
+    }
+  } catch (const char*) {
+    return -1;
+  }
+  return 0;
+}
+
+// Strip::Reconstruction::fast_load_txt_events on a file: the responses it
+// reads (up to cap triples into out).  Returns their number, or -1 if the
+// reader throws.
+int64_t ref_fast_load_txt_events(const char* fn, int is_3d, float* out, int64_t cap) {
+  Scanner scanner(0.43, 0.5, 8, 8, 0.01, 0.01, 0.01, 0.04);
+  Recon recon(scanner);
+  try {
+    recon.fast_load_txt_events(fn, is_3d != 0);
+  } catch (...) {
+    return -1;
+  }
+  const int64_t n = (int64_t)recon.responses.size();
+  for (int64_t e = 0; e < n && e < cap; ++e) {
+    out[3 * e] = recon.responses[e].z_u;
+    out[3 * e + 1] = recon.responses[e].z_d;
+    out[3 * e + 2] = recon.responses[e].dl;
+  }
+  return n;
 }
 
 }  // extern "C"

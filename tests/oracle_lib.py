@@ -94,6 +94,8 @@ def ref():
         r.ref_strip_tan_y_z.argtypes = [d, vp, vp]
         r.ref_strip_pixel_at.argtypes = [d, d, C.c_int, C.c_int, d, C.c_float,
                                          C.c_float, vp]
+        r.ref_strip_pixel_center.argtypes = [d, d, C.c_int, C.c_int, d, C.c_int,
+                                             C.c_int, vp]
         _r = r
     return _r
 

@@ -52,7 +52,13 @@ def test_hybrid_cli_matches_library(gpu_pkg, tmp_path):
     hg.write_geometry(g, str(tmp_path / "geom.bin"))
     with open(tmp_path / "resp.txt", "w") as f:
         for (a, b), (u, d, l) in zip(lors, zzd):
-            f.write(f"{a} {b} {u:.9g} {d:.9g} {l:.9g}\n")
+            f.write(f"{a} {b} {u:.6g} {d:.6g} {l:.6g}\n")
+    # the responses as the reference's fast text reader reads them
+    # (reconstruction.h:165-188; the reader is pinned bit for bit against
+    # the reference in test_text_responses.py)
+    zzd_txt, lors_txt = pkg.read_text_responses(tmp_path / "resp.txt", 5, with_lors=True)
+    assert np.array_equal(lors_txt, lors)
+    zzd = zzd_txt
     print("case: 32x32x32 ring geometry file, 6000 text responses, 2 blocks of 2", flush=True)
     r = run(["--geometry", "geom.bin", "--s-z", "0.012", "--s-dl", "0.05", "-l", "0.3",
              "-i", "2", "-I", "2", "-o", "out.bin", "resp.txt"], tmp_path)

@@ -206,6 +206,88 @@ def test_point_values_vs_reference():
         assert _pixel_at(s, float(z), float(y), f64=False) == tuple(ref)
 
 
+FOOTPRINTS = np.load(os.path.join(GOLDEN, "footprints.npz"))
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_footprint_vs_reference(path):
+    """oracle_footprint, per event, against the reference's own footprints
+    (tests/golden/footprints.npz, from the reference compiled from its
+    headers): the ellipse pixels are those bb_update writes for the event
+    alone, the box the clamped box it scans; and against the MLEM oracle's
+    own counts (the path pinned bit for bit above), per event and in
+    total."""
+    f = np.load(path)
+    sc = sc_from(f["scanner"])
+    name = os.path.basename(path)[len("mlem_"):-len(".npz")]
+    want_box, want_ell = FOOTPRINTS[f"box_{name}"], FOOTPRINTS[f"ell_{name}"]
+    ev = f["events"][:len(want_box)]
+    nb, ne = ol.footprint(sc, ev)
+    assert np.array_equal(nb, want_box)
+    assert np.array_equal(ne, want_ell)
+    assert ne.sum() > 0 and np.all(ne <= nb)
+    _, cnt = ol.mlem_f32(sc, ev, 1, counts=True)
+    assert (int(cnt[0]), int(cnt[1])) == (int(nb.sum()), int(ne.sum()))
+    for k in range(0, len(ev), 41):
+        _, c1 = ol.mlem_f32(sc, ev[k:k + 1], 1, counts=True)
+        assert (int(c1[0]), int(c1[1])) == (int(nb[k]), int(ne[k])), k
+
+
+def test_footprint_brute_force_tiny():
+    """A tiny image where every box pixel is tested by hand in FP64 with
+    the reference's rules: the count of pixels with q = A dy^2 + B dy dz +
+    C dz^2 <= 9 matches oracle_footprint wherever no pixel's q lies within
+    1e-5 of the threshold (there the float test may round either way)."""
+    rng = np.random.default_rng(5)
+    sc = ol.SimpleScanner(0.43, 0.5, 12, 12, 0.02, 0.02, 0.01, 0.04)
+    o = ol.oracle()
+    n_checked = 0
+    for _ in range(200):
+        ev = np.array([rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1),
+                       rng.normal(0, 0.08)], np.float32)
+        tyz = np.zeros(3, np.float64)
+        o.oracle_tan_y_z_f64(0.43, ol.P(ev.astype(np.float64)), ol.P(tyz))
+        t, y, z = tyz
+        bb = np.zeros(6, np.float64)
+        o.oracle_ellipse_bb_f64(0.01, 0.04, t, ol.P(bb))
+        _, A, B, C, bby, bbz = bb
+        cx = int(np.trunc((z + 0.12) / 0.02))
+        cy = int(np.trunc((0.12 - y) / 0.02))
+        hw, hh = int(bbz / 0.02 + 0.5), int(bby / 0.02 + 0.5)
+        x0, x1 = min(11, max(0, cx - hw)), min(11, max(0, cx + hw))
+        y0, y1 = min(11, max(0, cy - hh)), min(11, max(0, cy + hh))
+        qs = [A * (0.12 - 0.01 - iy * 0.02 - y) ** 2 +
+              B * (0.12 - 0.01 - iy * 0.02 - y) * (-0.12 + 0.01 + iz * 0.02 - z) +
+              C * (-0.12 + 0.01 + iz * 0.02 - z) ** 2
+              for iy in range(y0, y1) for iz in range(x0, x1)]
+        if any(abs(q - 9) < 1e-5 for q in qs):
+            continue
+        nb, ne = ol.footprint(sc, ev[None, :])
+        assert nb[0] == len(qs) and ne[0] == sum(q <= 9 for q in qs)
+        n_checked += 1
+    assert n_checked > 150
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_pixel_center_f32_vs_reference(path):
+    """oracle_pixel_center_f32 for every pixel against the reference's
+    Scanner::pixel_center in float (tests/golden/footprints.npz)."""
+    import ctypes as C
+    f = np.load(path)
+    sc = sc_from(f["scanner"])
+    name = os.path.basename(path)[len("mlem_"):-len(".npz")]
+    want = FOOTPRINTS[f"centre_{name}"]
+    fn = ol.oracle().oracle_pixel_center_f32
+    s = ol.oscanner(sc)
+    got = np.zeros_like(want)
+    for y in range(sc.n_y_pixels):
+        for x in range(sc.n_z_pixels):
+            o = np.zeros(2, np.float32)
+            fn(C.byref(s), x, y, ol.P(o))
+            got[y, x] = o
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 @pytest.mark.skipif(not ol.have_ref(), reason="reference harness not built here")
 def test_random_cases_vs_live_reference():
     """Fresh random cases against the compiled reference (only where the
