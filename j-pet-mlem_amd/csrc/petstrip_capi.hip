@@ -183,6 +183,11 @@ struct petstrip_handle {
   float* d_lam_sorted = nullptr;
   size_t cap_part = 0;
   int* d_flag = nullptr;  // device overflow flag of the fixed-point image
+  unsigned long long* d_nzero = nullptr;  // events whose last denominator was not > 0
+  // sum of rho, exact after set_rho and an estimate after iterations
+  // (sum rho' = events with a positive denominator); it sets the
+  // fixed-point unit of the log-likelihood gradient
+  double rho_sum = 0.0;
   // G(k) = 2^(-(k du)^2) of the fused walks, for the current sk and pw
   float* d_gtab = nullptr;
   float* d_mtab = nullptr;  // (k G(k), k^2 G(k)) pairs
@@ -241,9 +246,13 @@ constexpr int kMinGroupItems = 2048;  // banded items per pipelined group, at le
 // accumulation mode (two conversions)
 constexpr int kMlemFixedBits = 31;
 constexpr int kFixed64Bits = 40;
-// log-likelihood gradient (K / lambda, unbounded): 2^-26 units, values up to
-// 2^31.5 units take one conversion, up to 2^52 the split one
-constexpr int kGradBits = 26;
+// log-likelihood gradient (K / lambda, unbounded): values up to 2^30.5 units
+// take one conversion, up to 2^52 the split one, beyond that a direct FP64
+// atomic.  The unit puts a typical value, 1 / (mean rho x mean ellipse
+// pixels), at 2^kGradTypicalBits units: each value is then rounded to about
+// 1e-6 of itself, and values up to 2^10 times the typical one (cold pixels,
+// small denominators) still take the single conversion.
+constexpr int kGradTypicalBits = 20;
 
 // Along a box row the reference's float ellipse test has no holes when the
 // quadratic's column-to-column curvature dominates the rounding of q.  With
@@ -713,6 +722,7 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   if ((rc = dalloc(&h->d_scalars, 8))) return bail(rc);
   if ((rc = dalloc((double**)&h->d_acc, h->n_pix))) return bail(rc);
   if ((rc = dalloc(&h->d_flag, 1))) return bail(rc);
+  if ((rc = dalloc(&h->d_nzero, 1))) return bail(rc);
   if (hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipMemset failed"));
   if (hipMemsetAsync(h->d_acc, 0, h->n_pix * sizeof(double), h->stream) != hipSuccess)
@@ -727,6 +737,7 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   }
   if ((rc = reserve_events(h, 1))) return bail(rc);
   // rho = 1 (reconstruction.h:59), 5-point sensitivity for the --gpu variant
+  h->rho_sum = (double)h->n_pix;
   if (petstrip::launch_fill(h->stream, h->d_rho, 1.0f, (int64_t)h->n_pix) ||
       petstrip::launch_sensitivity(h->stream, h->sg, 1, h->d_sens) ||
       hipStreamSynchronize(h->stream) != hipSuccess)
@@ -757,6 +768,7 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_sidx);
   dfree(h->d_items);
   dfree(h->d_flag);
+  dfree(h->d_nzero);
   dfree(h->d_gtab);
   dfree(h->d_mtab);
   dfree(h->d_part);
@@ -868,6 +880,9 @@ int petstrip_set_rho(petstrip_handle* h, const float* rho) {
   if ((rc = ensure_device(h))) return rc;
   HIP_TRY(hipMemcpyAsync(h->d_rho, rho, h->n_pix * sizeof(float),
                          hipMemcpyHostToDevice, h->stream));
+  double sum = 0.0;
+  for (size_t i = 0; i < h->n_pix; ++i) sum += rho[i];
+  h->rho_sum = sum;
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->lambda_valid = false;
   return PETSTRIP_OK;
@@ -879,7 +894,13 @@ int petstrip_set_rho_device(petstrip_handle* h, const float* d_rho) {
   if ((rc = ensure_device(h))) return rc;
   HIP_TRY(hipMemcpyAsync(h->d_rho, d_rho, h->n_pix * sizeof(float),
                          hipMemcpyDeviceToDevice, h->stream));
+  std::vector<float> host(h->n_pix);
+  HIP_TRY(hipMemcpyAsync(host.data(), d_rho, h->n_pix * sizeof(float),
+                         hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  double sum = 0.0;
+  for (float v : host) sum += v;
+  h->rho_sum = sum;
   h->lambda_valid = false;
   return PETSTRIP_OK;
 }
@@ -910,8 +931,17 @@ int petstrip_iterate(petstrip_handle* h, int n_iterations) {
     if ((rc = one_iteration(h))) return rc;
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
   }
+  if (n_iterations > 0)
+    HIP_TRY(petstrip::launch_count_nonpositive(h->stream, h->d_lambda, h->n_events,
+                                               h->d_nzero));
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (n_iterations > 0) {
+    unsigned long long nzero = 0;
+    HIP_TRY(hipMemcpy(&nzero, h->d_nzero, sizeof(nzero), hipMemcpyDeviceToHost));
+    h->stats.n_zero_denominator = (int64_t)nzero;
+    // sum rho' = events with a positive denominator (the CPU update; within a
+    // small factor for the sensitivity-weighted one)
+    h->rho_sum = (double)(h->n_events - (int64_t)nzero);
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.last_iteration_ms = ms;
@@ -1073,6 +1103,17 @@ int backward_per_event(petstrip_handle* h, const float* g, float* grad_rho,
   return PETSTRIP_OK;
 }
 
+// log2 of the fixed-point unit of the log-likelihood gradient's values K /
+// lambda_e: a typical value is 1 / (mean rho x mean ellipse pixels).
+int grad_unit_bits(const petstrip_handle* h) {
+  const double mean_rho = h->rho_sum / (double)std::max<size_t>(h->n_pix, 1);
+  const double mean_ell = (double)h->stats.n_ellipse_pixels /
+                          (double)std::max<int64_t>(h->stats.n_events, 1);
+  if (!(mean_rho > 0.0) || !(mean_ell > 0.0)) return 26;
+  const int k = (int)std::floor(kGradTypicalBits + std::log2(mean_rho * mean_ell));
+  return std::max(-60, std::min(60, k));
+}
+
 // The log-likelihood gradient (g = 1/lambda) in one GRAD launch: lambda,
 // sigma derivative and log-likelihood terms in pass 1, sum_e K / lambda_e in
 // pass 2 with the row setups kept from pass 1; grad = that - 1.
@@ -1081,9 +1122,10 @@ int gradient_fused(petstrip_handle* h, float* grad_rho, double grad_sigma[2],
   const int64_t n = h->n_events;
   petstrip::FusedArgs a = fused_args(h);
   a.acc_kind = petstrip::GLOBAL_F64;
-  a.log2_scale = (float)kGradBits;
+  const int k = grad_unit_bits(h);
+  a.log2_scale = (float)k;
   a.split = 0;
-  a.inv_fixed_scale = std::ldexp(1.0, -kGradBits);
+  a.inv_fixed_scale = std::ldexp(1.0, -k);
   HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
   HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
   HIP_TRY(hipEventRecord(h->evk0, h->stream));

@@ -315,6 +315,21 @@ __global__ void k_fill(float* __restrict__ p, float v, int64_t n) {
   if (i < n) p[i] = v;
 }
 
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// Adds to *count the entries of x that are not > 0 (zero, negative or NaN
+// denominators): a grid-stride count per lane, a wave sum, one atomic per
+// wave.
+__global__ void k_count_nonpositive(const float* __restrict__ x, int64_t n,
+                                    unsigned long long* count) {
+  long long c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    c += !(x[i] > 0.0f);
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+  if (__lane_id() == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
 __global__ void k_add_scalar(float* __restrict__ p, float v, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] += v;
@@ -462,6 +477,15 @@ hipError_t launch_tan_kernel_map(hipStream_t st, const StripConst& k, int w, int
 hipError_t launch_fill(hipStream_t st, float* p, float v, int64_t n) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_nonpositive(hipStream_t st, const float* x, int64_t n,
+                                    unsigned long long* count) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, count);
+  if (n > 0)
+    hipLaunchKernelGGL(k_count_nonpositive, dim3(std::min<unsigned>(blocks_for(n, 256), 2048)),
+                       dim3(256), 0, st, x, n, count);
   return hipGetLastError();
 }
 
