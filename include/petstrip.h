@@ -314,6 +314,31 @@ int petstrip_phantom_tan_kernel_map(const petstrip_scanner* scanner,
                                     double kernel_point_x, double kernel_point_y,
                                     double scale, float* out, int device);
 
+/* --- one iteration that allocates nothing ----------------------------------- */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one MLEM iteration of the CPU update (the one
+ * petstrip_iterate computes) on the per-event kernels
+ * (PETSTRIP_PATH_PER_EVENT), without a handle, calling no hipMalloc, hipFree
+ * or other allocating routine.  Its event geometry, per-event denominators
+ * and weights and its FP64 accumulator live in a static device array of
+ * PETSTRIP_NOALLOC_SCRATCH_BYTES, which each device that runs this call holds
+ * from its first call until the process ends; so 56 n + 8 W H bytes (plus
+ * alignment) must fit it: petstrip_noalloc_max_events gives the largest n
+ * for an image size, and larger n fails with PETSTRIP_EINVAL before any
+ * device work.  d_responses (n packed triples), d_rho and d_rho_out (W*H
+ * floats; may be the same buffer) are device memory on the current device;
+ * the work is enqueued on `stream` (a hipStream_t, NULL for the default
+ * stream) and the call returns without waiting.  Every call on one device
+ * shares the scratch array: calls that overlap on one device race. */
+#define PETSTRIP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int petstrip_noalloc_max_events(int32_t n_y_pixels, int32_t n_z_pixels,
+                                int64_t* max_events);
+int petstrip_iterate_noalloc(const petstrip_scanner* scanner, const float* d_responses,
+                             int64_t n, const float* d_rho, float* d_rho_out,
+                             void* stream);
+
 /* --- response files (input side; host only, no device needed) ------------- */
 
 /* Reads a text response file as the reference commands read `.txt`

@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PETSTRIP_ARCH", "gfx950")
 
 SOURCES = ["strip_kernels.hip", "strip_bins.hip", "strip_fused.hip", "petstrip_capi.hip",
-           "phantom.hip", "hybrid.hip"]
+           "phantom.hip", "hybrid.hip", "noalloc.hip"]
 PROBE_SOURCES = ["atomic_probe.hip"]
 HOST_SOURCES = ["text_responses.cpp"]
 CXX = os.environ.get("CXX", "g++")

@@ -116,6 +116,7 @@ int set_error(int code, const char* msg) { return fail(code, msg); }
 StripConst scanner_const(const petstrip_scanner& sc, double sigma_z, double sigma_dl) {
   return make_const(sc, sigma_z, sigma_dl);
 }
+int scanner_check(const petstrip_scanner* sc) { return check_scanner(sc); }
 int phantom_spec(const petstrip_scanner* sc, const double* regions, int32_t n_regions,
                  uint64_t seed, PhantomSpec& p);
 int phantom_device(const PhantomSpec& p, int64_t n, float* out, hipStream_t st,
