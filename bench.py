@@ -12,9 +12,12 @@ roofline: the fused iteration kernel is FP32-VALU bound (no contraction to
 put on MFMA, image and events are a few hundred MB against ~5e12 flop);
 achieved = algorithmic flop per launch (sum_e 15 N_box + 41 N_ell + 50) /
 mean launch time measured with HIP events on the library's stream.
-cpu_baseline: the reference CPU algorithm (the reference's own sources
-compiled by oracle/Makefile when present, else the oracle's restatement) on a
-bounded prefix of the same events, with 16 threads.
+cpu_baseline: the reference CPU algorithm, the reference's own sources
+compiled by oracle/Makefile (or, with --cpu-kind port, the oracle's
+restatement), on a bounded prefix of the same events: iterations 1-3 from
+rho = 1, each timed, events/s/iter from the median of iterations 2-3; with
+the OpenMP threads used, the CPUs in the process's affinity mask, nproc and
+the CPU model.
 parity: the GPU image over the same prefix against that CPU image, and over a
 shorter prefix against FP64 runs of the same algorithm (relative L2 and the
 largest per-pixel relative error, per iteration).
@@ -61,7 +64,7 @@ def atomic_report(pkg, st, n_pix, kernel_ms):
     }
 
 
-def hybrid_report(pkg, name, steps, no_cpu, cpu_threads):
+def hybrid_report(pkg, name, steps, no_cpu, cpu_threads, cpu_kind):
     """The 3D hybrid reconstruction (include/petstrip_hybrid.h) on a named
     workload: iteration time with the events resident on the device, parity
     of a 2e4-event prefix against the FP64 oracle after two iterations, and
@@ -130,12 +133,14 @@ def hybrid_report(pkg, name, steps, no_cpu, cpu_threads):
         "sample_events": len(fr),
         "rel_l2_vs_fp64": [float(np.linalg.norm(gpu[i] - o64[i]) / np.linalg.norm(o64[i]))
                            for i in range(2)]}
-    if ol.have_ref_hybrid():
-        r = ol.ref_hybrid(g, sz, sdl, length, lors[:m], zzd[:m], 1, n_threads=cpu_threads,
+    if cpu_kind == "reference":
+        mc = min(100000, len(lors))
+        r = ol.ref_hybrid(g, sz, sdl, length, lors[:mc], zzd[:mc], 3, n_threads=cpu_threads,
                           keep_images=False)
-        base = {"value": len(r["frames"]) / r["seconds"], "unit": "events/s/iter",
-                "cores": cpu_threads, "kind": "reference",
-                "sample": f"first {m} responses, one iteration"}
+        base = {"value": 3 * len(r["frames"]) / r["seconds"], "unit": "events/s/iter",
+                "cores": cpu_threads, "kind": "reference", **host_info(),
+                "sample": f"first {mc} responses, 3 iterations, mean iteration "
+                          f"{r['seconds'] / 3:.3f} s, {cpu_threads} OpenMP threads"}
     else:
         w32, s32 = ol.hybrid_weights(g)
         fr32 = ol.hybrid_frames(g, sz, sdl, w32, lors[:m], zzd[:m])
@@ -186,6 +191,70 @@ def gradient_report(pkg, name):
             "grad_ms": g_ms, "grad_over_mlem": g_ms / mlem}
 
 
+CFG5 = "strip-1024px-0.5mm-1e8ev"
+
+
+def cfg5_report(pkg, rounds, threads, no_cpu):
+    """The calibration workload (1024 x 1024 pixels of 0.5 mm, 1e8 events of
+    the ten-ellipse phantom simulated at sigma = (0.01, 0.04), generated on
+    the device): MLEM iteration and gradient times at the simulated sigma;
+    `rounds` rounds of 5 MLEM iterations and a sigma step from (0.015, 0.06)
+    with the support held at the start (calibration.py), its recovered
+    sigma, events/s and, as the checker, one more iteration from the final
+    image against FP64 on a prefix; and the same sigma ascent with the image
+    held at the phantom's expected image of detected emissions, for half as
+    many rounds (the estimator's check: it recovers the simulated widths)."""
+    import numpy as np
+    import torch
+    n_pix, size, n_ev, _, phantom, seed = pkg.WORKLOADS[CFG5]
+    sc = pkg.workload_scanner(CFG5)
+    out = {"workload": CFG5, "events": n_ev}
+    with pkg.Reconstruction(sc, device=0) as rec:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rec.set_phantom_events(phantom, n_ev, seed=seed)
+        torch.cuda.synchronize()
+        out["setup_s"] = time.perf_counter() - t0
+        rec(1)
+        it_ms = []
+        for _ in range(3):
+            rec(1)
+            it_ms.append(rec.stats()["last_iteration_ms"])
+        out["mlem_iteration_ms"] = float(np.median(it_ms))
+        out["mlem_events_per_s_iter"] = n_ev / (out["mlem_iteration_ms"] * 1e-3)
+        out["grad_ms"] = time_gradient(rec)
+        out["grad_over_mlem"] = out["grad_ms"] / out["mlem_iteration_ms"]
+        rec.rho = np.ones((n_pix, n_pix), np.float32)
+        res = pkg.calibrate_sigma(
+            rec, (0.015, 0.06), rounds=rounds,
+            log=lambda h: print(f"cfg5 calibration round {h['round']}: sigma "
+                                f"({h['sigma_z']:.6f}, {h['sigma_dl']:.6f}) loglik "
+                                f"{h['loglik']:.6e}", file=sys.stderr, flush=True))
+        state = (res["sigma_z"], res["sigma_dl"], rec.rho.copy())
+        out["calibration"] = {
+            "start": [0.015, 0.06], "simulated": [sc.sigma_z, sc.sigma_dl],
+            "recovered": [res["sigma_z"], res["sigma_dl"]], "rounds": rounds,
+            "mlem_iterations": res["mlem_iterations"], "support": "start",
+            "seconds": res["seconds"],
+            "mlem_events_per_s_per_iter": n_ev * res["mlem_iterations"] / res["seconds_mlem"],
+            "gradient_s_per_round": res["seconds_gradient"] / max(rounds, 1),
+            "trajectory": [[h["sigma_z"], h["sigma_dl"]] for h in res["history"]],
+        }
+        # the estimator on the true image: sigma steps only
+        rec.set_sigma(sc.sigma_z, sc.sigma_dl)
+        rec.rho = pkg.expected_image(sc, phantom, n_ev, rec.sensitivity(0))
+        n_exp = max(1, rounds // 2)
+        exp = pkg.calibrate_sigma(rec, (0.015, 0.06), rounds=n_exp, mlem_per_round=0)
+        out["calibration_expected_image"] = {
+            "recovered": [exp["sigma_z"], exp["sigma_dl"]], "rounds": n_exp,
+            "seconds": exp["seconds"]}
+    if not no_cpu:
+        ev = pkg.workload_events(CFG5, n=500_000)
+        box_px = 23126 * (0.015 / 0.01) * (0.06 / 0.04)
+        out["calibration"]["parity"] = calibration_parity(pkg, CFG5, ev, state, threads, box_px)
+    return out
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -196,8 +265,16 @@ def parse():
                    choices=["fp32", "compensated", "fixed64"])
     p.add_argument("--cpu-sample", type=float, default=None,
                    help="events in the CPU baseline sample (default: about "
-                        "3.4e10 box pixels of work, 6e6 events at 512 px / 1 mm)")
-    p.add_argument("--cpu-threads", type=int, default=16)
+                        "2.2e10 box pixels of work per iteration, 3.8e6 events at "
+                        "512 px / 1 mm)")
+    p.add_argument("--cpu-threads", type=int, default=None,
+                   help="OpenMP threads of the CPU baseline (default: the CPUs in "
+                        "this process's affinity mask, at most 16, the host share "
+                        "of one GPU)")
+    p.add_argument("--cpu-kind", default="reference", choices=["reference", "port"],
+                   help="reference: the reference's own CPU path compiled from its "
+                        "sources (oracle/_ref, built by __graft_entry__.build()); "
+                        "port: the oracle's restatement")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--hybrid", default="hybrid-ring192-160px-4mm-128planes-1e6ev",
                    help="3D hybrid workload measured beside the headline line "
@@ -210,6 +287,9 @@ def parse():
                    help="also run ROUNDS rounds of sigma calibration (5 MLEM "
                         "iterations + one gradient step on (sigma_z, sigma_dl) "
                         "each) from (0.015, 0.06) on the same events")
+    p.add_argument("--cfg5-rounds", type=int, default=20,
+                   help="rounds of the calibration block on the 1024 x 1024 / 0.5 mm "
+                        "workload with 1e8 events (0 skips the block)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="also run one device copy of MIB MiB (for HBM counter "
                         "calibration under rocprofv3 --pmc)")
@@ -241,26 +321,54 @@ def _errors(img, ref):
     return rel_l2, float(np.max(np.abs(a[m] - b[m]) / b[m]))
 
 
-def cpu_baseline(sc, ev, n_sample, threads, gpu_iterations, box_px):
-    """Reference CPU path on a prefix of the events, one iteration, timed;
-    then, as the checker, the GPU's images on the same prefixes against that
-    CPU image and against FP64 runs of the same algorithm."""
+def host_info():
+    """The CPUs this process may run on, nproc and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"affinity_cpus": len(os.sched_getaffinity(0)), "nproc": os.cpu_count(),
+            "cpu_model": model}
+
+
+def default_threads():
+    """The CPUs in the affinity mask, at most 16: the host share of one GPU
+    on the GPU boxes (their nproc counts the whole machine)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(sc, ev, n_sample, threads, kind, gpu_iterations, box_px):
+    """The reference CPU path on a prefix of the events: iterations 1-3
+    from rho = 1, each timed on its own, events/s/iter = events / the median
+    of iterations 2-3; then, as the checker, the GPU's images on the same
+    prefixes against the CPU image of iteration 1 and against FP64 runs of
+    the same algorithm."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
 
     import oracle_lib as ol
     sample = np.ascontiguousarray(ev[:int(n_sample)])
-    kind = "reference" if ol.have_ref() else "port"
     fn = ol.ref_mlem if kind == "reference" else ol.mlem_f32
     fn(sc, sample[:min(len(sample), 20000)], 1, n_threads=threads)  # warm
-    t0 = time.perf_counter()
-    cpu_img = fn(sc, sample, 1, n_threads=threads)[0]
-    dt = time.perf_counter() - t0
+    times, rho, cpu_img = [], None, None
+    for it in range(3):
+        t0 = time.perf_counter()
+        rho = fn(sc, sample, 1, n_threads=threads, rho0=rho)[0]
+        times.append(time.perf_counter() - t0)
+        if it == 0:
+            cpu_img = rho.copy()
+    dt = float(np.median(times[1:]))
     base = {"value": len(sample) / dt, "unit": "events/s/iter", "cores": threads,
-            "kind": kind,
-            "sample": f"first {len(sample)} events of the same event list, "
-                      f"1 iteration from rho=1, {threads} OpenMP threads, "
-                      f"{dt:.2f} s"}
+            "kind": kind, **host_info(),
+            "iteration_s": times,
+            "sample": f"first {len(sample)} events of the same event list, iterations "
+                      f"1-3 from rho=1 each timed, median of iterations 2-3 "
+                      f"({dt:.2f} s), {threads} OpenMP threads"}
     rel_cpu, max_cpu = _errors(gpu_iterations(sample, 1)[0], cpu_img)
     # FP64 runs: a prefix of about 3e9 box pixels, 2 iterations
     n64 = int(min(5e5, 3e9 / max(box_px, 1.0), len(sample)))
@@ -308,6 +416,19 @@ def main():
               "supported (the path would shard events across GPUs and all-reduce "
               "the image)", file=sys.stderr)
         sys.exit(2)
+    if args.cpu_threads is None:
+        args.cpu_threads = default_threads()
+    if not args.no_cpu and args.cpu_kind == "reference":
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        missing = [p for p, ok in ((ol.REF_SO, ol.have_ref()),
+                                   (ol.REF_HYBRID_SO, ol.have_ref_hybrid())) if not ok]
+        if missing:
+            print("bench.py: the reference CPU baseline needs " + ", ".join(missing) +
+                  " (built from the reference's sources by __graft_entry__.build() where "
+                  "the reference tree exists); pass --cpu-kind port to time the oracle's "
+                  "restatement instead", file=sys.stderr)
+            sys.exit(2)
     import numpy as np
     import torch
 
@@ -424,18 +545,21 @@ def main():
             return imgs
 
         box_px = st["n_box_pixels"] / max(n_ev, 1)
-        n_cpu = args.cpu_sample if args.cpu_sample else 3.4e10 / max(box_px, 1.0)
+        n_cpu = args.cpu_sample if args.cpu_sample else 2.2e10 / max(box_px, 1.0)
         out["cpu_baseline"], out["parity"] = cpu_baseline(
-            sc, ev, n_cpu, args.cpu_threads, gpu_iterations, box_px)
+            sc, ev, n_cpu, args.cpu_threads, args.cpu_kind, gpu_iterations, box_px)
         if cal_state is not None:
             out["calibration"]["parity"] = calibration_parity(
                 pkg, name, ev, cal_state, args.cpu_threads,
                 st["n_box_pixels"] / max(n_ev, 1))
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    if args.cfg5_rounds > 0 and name != CFG5:
+        print(f"calibration workload {CFG5}", file=sys.stderr, flush=True)
+        out["cfg5"] = cfg5_report(pkg, args.cfg5_rounds, args.cpu_threads, args.no_cpu)
     if args.hybrid != "none":
         print(f"hybrid workload {args.hybrid}", file=sys.stderr, flush=True)
         out["hybrid"] = hybrid_report(pkg, args.hybrid, args.steps, args.no_cpu,
-                                      args.cpu_threads)
+                                      args.cpu_threads, args.cpu_kind)
     print(json.dumps(out), flush=True)
 
 

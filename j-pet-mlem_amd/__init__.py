@@ -12,7 +12,7 @@ from .petstrip import (  # noqa: F401
     PROBE_GLOBAL_F32, PROBE_GLOBAL_F64, PROBE_GLOBAL_U64, PROBE_LDS_U64, probe_atomics,
 )
 from .workloads import (WORKLOADS, workload_scanner, workload_events,  # noqa: F401
-                        HYBRID_WORKLOADS, hybrid_workload)
+                        HYBRID_WORKLOADS, hybrid_workload, expected_image)
 from . import hybrid_geometry  # noqa: F401
 from .hybrid import HybridReconstruction  # noqa: F401
 from .calibration import calibrate_sigma  # noqa: F401

@@ -22,12 +22,18 @@ import time
 
 
 def calibrate_sigma(rec, sigma0, rounds=20, mlem_per_round=5, rate=0.5, max_step=0.1,
-                    support="current", log=None):
+                    support="start", grow=2.0, log=None):
     """Runs `rounds` rounds of `mlem_per_round` MLEM iterations followed by a
     step on (sigma_z, sigma_dl), starting from sigma0, on the events and image
-    already in `rec` (a Reconstruction).  Returns a dict with the final
+    already in `rec` (a Reconstruction).  The step is rate x sigma dL/dsigma /
+    N in log sigma, clipped to max_step; after an accepted full step the rate
+    grows by `grow` (the curvature of L in log sigma is much weaker than a
+    single Gaussian's, since the image spreads each event's likelihood), and
+    every halving of a rejected step halves it.  Returns a dict with the final
     sigmas, the per-round history and timings; rec holds the final image and
     sigma on return."""
+    if support not in ("start", "current"):
+        raise ValueError("support must be 'start' or 'current'")
     n = max(int(rec.n_events), 1)
     sz, sdl = float(sigma0[0]), float(sigma0[1])
     frozen = support == "start"
@@ -37,15 +43,16 @@ def calibrate_sigma(rec, sigma0, rounds=20, mlem_per_round=5, rate=0.5, max_step
     t_start = time.perf_counter()
     for r in range(rounds):
         t0 = time.perf_counter()
-        rec(mlem_per_round)
+        if mlem_per_round > 0:
+            rec(mlem_per_round)
         t1 = time.perf_counter()
         _, g, ll = rec.backward(want_rho=False, want_sigma=True, want_loglik=True)
         t_grad += time.perf_counter() - t1
         t_mlem += t1 - t0
         du = [max(-max_step, min(max_step, rate * s * gi / n))
               for s, gi in zip((sz, sdl), g)]
-        accepted = False
-        for _ in range(4):
+        accepted, halvings = False, 0
+        for halvings in range(4):
             nz, ndl = sz * math.exp(du[0]), sdl * math.exp(du[1])
             rec.set_sigma(nz, ndl, freeze_support=frozen)
             ll_new = rec.backward(want_rho=False, want_sigma=False, want_loglik=True)[2]
@@ -57,9 +64,10 @@ def calibrate_sigma(rec, sigma0, rounds=20, mlem_per_round=5, rate=0.5, max_step
             nz, ndl = sz, sdl
             rec.set_sigma(sz, sdl, freeze_support=frozen)
             ll_new = ll
+        rate = rate * grow if accepted and halvings == 0 else rate / 2 ** (halvings + 1 - accepted)
         entry = {"round": r + 1, "sigma_z": sz, "sigma_dl": sdl, "loglik": ll,
                  "grad_sigma": [float(g[0]), float(g[1])], "next_sigma_z": nz,
-                 "next_sigma_dl": ndl, "loglik_after_step": ll_new}
+                 "next_sigma_dl": ndl, "loglik_after_step": ll_new, "rate": rate}
         history.append(entry)
         if log:
             log(entry)

@@ -60,6 +60,35 @@ def workload_scanner(name, sigma_z=SIGMA_Z, sigma_dl=SIGMA_DL):
     return Scanner(R, L, n_pix, n_pix, size, size, sigma_z, sigma_dl)
 
 
+def expected_image(scanner, regions, n_events, sensitivity, sub=4):
+    """The expected image of detected emissions of a phantom: per pixel the
+    intensity of the top region (earlier regions on top, the generator's
+    non-additive rule) averaged over sub x sub points of the pixel, times
+    the detection probability (the scanner's sensitivity map at the pixel
+    centres, Scanner::sensitivity, src/2d/strip/scanner.h:147-171, which is
+    the fraction of emission angles seen by both strips), scaled to sum to
+    n_events.  Rows run from the top (y = +H ph / 2), columns from the left
+    (z = -W pw / 2), as the images of Reconstruction.  Regions are
+    (x, y, a, b, angle_deg, intensity) ellipses, x along z."""
+    W, H = scanner.n_z_pixels, scanner.n_y_pixels
+    pw, ph = scanner.pixel_width, scanner.pixel_height
+    off = (np.arange(sub) + 0.5) / sub
+    z = (-W * pw / 2 + scanner.center_z) + (np.arange(W)[:, None] + off[None, :]).ravel() * pw
+    y = (H * ph / 2 + scanner.center_y) - (np.arange(H)[:, None] + off[None, :]).ravel() * ph
+    zz, yy = np.meshgrid(z, y)
+    val = np.zeros(zz.shape)
+    free = np.ones(zz.shape, bool)
+    for x0, y0, a, b, ang, inten in regions:
+        c, s_ = np.cos(np.radians(ang)), np.sin(np.radians(ang))
+        rx, ry = zz - x0, yy - y0
+        ex, ey = c * rx + s_ * ry, -s_ * rx + c * ry
+        inside = (ex / a) ** 2 + (ey / b) ** 2 <= 1.0
+        val[inside & free] = inten
+        free &= ~inside
+    img = val.reshape(H, sub, W, sub).mean(axis=(1, 3)) * np.asarray(sensitivity, np.float64)
+    return (img * (n_events / img.sum())).astype(np.float32)
+
+
 def workload_events(name, n=None, seed=None):
     n_pix, size, n_ev, _, phantom, s = WORKLOADS[name]
     sc = workload_scanner(name)

@@ -206,11 +206,12 @@ def sensitivity_f32(sc, mode):
     return out
 
 
-def ref_mlem(sc, events, n_iter, n_threads=1):
+def ref_mlem(sc, events, n_iter, n_threads=1, rho0=None):
     r = ref()
     np_ = sc.n_y_pixels * sc.n_z_pixels
     ev = np.ascontiguousarray(events, dtype=np.float32).reshape(-1, 3)
-    rho = np.ones(np_, np.float32)
+    rho = np.ones(np_, np.float32) if rho0 is None else \
+        np.ascontiguousarray(rho0, dtype=np.float32).reshape(-1).copy()
     iters = np.zeros((n_iter, np_), np.float32)
     assert sc.pixel_width == sc.pixel_height
     r.ref_strip_mlem(sc.radius, sc.scintillator_length, sc.n_y_pixels, sc.n_z_pixels,

@@ -184,3 +184,35 @@ def test_bad_arguments(pkg):
         pkg.phantom_generate(sc, [rect(0, 0, -0.1, 0.1, 1)], 10)
     with pytest.raises(pkg.PetstripError):
         pkg.phantom_generate(sc, [point(0, 0, 0.0)], 10)
+
+
+def test_expected_image_matches_generated_emissions(pkg):
+    """workloads.expected_image (the phantom's expected image of detected
+    emissions, which the calibration test holds fixed) against a histogram
+    of the generator's own detected emission points on a 40 x 40 grid of
+    16 mm: chi^2 per pixel about 1.  The same image with the rotations
+    mirrored, or without the detection probability, is rejected."""
+    import oracle_lib as ol
+    regions = pkg.workloads.shepp6()
+    sc = pkg.Scanner(0.43, 0.5, 40, 40, 0.016, 0.016, 0.01, 0.04)
+    n = 400_000
+    res = pkg.phantom_generate(sc, [(0,) + tuple(r) for r in regions], n, seed=17,
+                               detected=True, full=True, device=-1)
+    pts = res["events"][:, :2].astype(np.float64)
+    col = np.floor((pts[:, 0] + 0.32) / 0.016).astype(int)
+    row = np.floor((0.32 - pts[:, 1]) / 0.016).astype(int)
+    hist = np.zeros((40, 40))
+    np.add.at(hist, (row, col), 1.0)
+    sens = ol.sensitivity_f32(sc, 0)
+
+    def chi2(img):
+        m = img > 5
+        return float(np.mean((hist[m] - img[m]) ** 2 / img[m]))
+    good = chi2(pkg.expected_image(sc, regions, len(pts), sens, sub=8))
+    mirrored = chi2(pkg.expected_image(sc, [r[:4] + (-r[4],) + r[5:] for r in regions],
+                                       len(pts), sens, sub=8))
+    flat = chi2(pkg.expected_image(sc, regions, len(pts), np.ones_like(sens), sub=8))
+    print(f"chi2 per pixel: {good:.2f}, mirrored rotations {mirrored:.1f}, "
+          f"no detection probability {flat:.1f}")
+    assert good < 1.6
+    assert mirrored > 5 * good and flat > 5 * good
