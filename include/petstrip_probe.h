@@ -21,7 +21,9 @@ enum {
   PETSTRIP_PROBE_GLOBAL_F32 = 0, /* global atomic add, float (the FP32 flush)  */
   PETSTRIP_PROBE_GLOBAL_F64 = 1, /* global atomic add, double (the FP64 flush) */
   PETSTRIP_PROBE_GLOBAL_U64 = 2, /* global atomic add, uint64 (fixed point)    */
-  PETSTRIP_PROBE_LDS_U64 = 3     /* LDS ds_add_u64 (the walks' updates)        */
+  PETSTRIP_PROBE_LDS_U64 = 3,    /* LDS ds_add_u64 (the walks' updates)        */
+  PETSTRIP_PROBE_LDS_U32 = 4,    /* LDS ds_add_u32                             */
+  PETSTRIP_PROBE_LDS_F32 = 5     /* LDS ds_add_f32                             */
 };
 
 /* Times one launch of `reps` atomics per lane on `device` and writes the

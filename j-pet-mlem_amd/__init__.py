@@ -9,7 +9,8 @@ from .petstrip import (  # noqa: F401
     UPDATE_SENSITIVITY_DENOMINATOR, PetstripError, Reconstruction, Scanner,
     device_count, load_library, phantom_events, phantom_generate, phantom_grid,
     phantom_tan_kernel_map, parse_text_floats, read_text_responses, REGION_ELLIPSE, REGION_POINT, REGION_RECTANGLE,
-    PROBE_GLOBAL_F32, PROBE_GLOBAL_F64, PROBE_GLOBAL_U64, PROBE_LDS_U64, probe_atomics,
+    PROBE_GLOBAL_F32, PROBE_GLOBAL_F64, PROBE_GLOBAL_U64, PROBE_LDS_U64, PROBE_LDS_U32,
+    PROBE_LDS_F32, probe_atomics,
 )
 from .workloads import (WORKLOADS, workload_scanner, workload_events,  # noqa: F401
                         HYBRID_WORKLOADS, hybrid_workload, expected_image)

@@ -4,7 +4,8 @@
 //   - global atomics in the pattern of its window flush: each wave adds to a
 //     tile of 16 columns x 4 rows of a W x H image (64-byte row segments),
 //     one atomic per lane, no return value, waves on distinct tiles;
-//   - LDS ds_add_u64 in the pattern of its walks: the 64 lanes of a wave on
+//   - LDS ds_add_u64 (and, for comparison, ds_add_u32 and ds_add_f32) in
+//     the pattern of its walks: the 64 lanes of a wave on
 //     64 consecutive 64-bit words (no bank conflicts), 16 waves per
 //     workgroup.
 // The rates give the ceilings that the kernel's atomic counts are compared
@@ -43,19 +44,22 @@ __global__ __launch_bounds__(kGlobalThreads) void k_global_atomics(T* img, int w
   }
 }
 
+// T = unsigned long long (ds_add_u64), unsigned (ds_add_u32) or float
+// (ds_add_f32).
+template <typename T>
 __global__ __launch_bounds__(kLdsThreads) void k_lds_atomics(unsigned long long* sink,
                                                              int reps) {
-  __shared__ unsigned long long buf[kLdsWords];
-  for (int i = threadIdx.x; i < kLdsWords; i += kLdsThreads) buf[i] = 0ull;
+  __shared__ T buf[kLdsWords];
+  for (int i = threadIdx.x; i < kLdsWords; i += kLdsThreads) buf[i] = T(0);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  unsigned long long* p = buf + wave * 256 + lane;
+  T* p = buf + wave * 256 + lane;
   for (int i = 0; i < reps; ++i)
-    __hip_atomic_fetch_add(p + (i & 3) * 64, 1ull, __ATOMIC_RELAXED,
+    __hip_atomic_fetch_add(p + (i & 3) * 64, T(1), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
   __syncthreads();
   // one word per workgroup keeps the updates observable
-  if (threadIdx.x == 0) sink[blockIdx.x] = buf[0] + buf[kLdsWords - 1];
+  if (threadIdx.x == 0) sink[blockIdx.x] = (unsigned long long)(buf[0] + buf[kLdsWords - 1]);
 }
 
 template <typename T>
@@ -93,6 +97,7 @@ hipError_t time_global(int width, int height, int reps, double* atomics, float* 
   return rc;
 }
 
+template <typename T>
 hipError_t time_lds(int reps, double* atomics, float* ms) {
   int dev = 0, ncu = 0;
   hipError_t rc = hipGetDevice(&dev);
@@ -105,12 +110,12 @@ hipError_t time_lds(int reps, double* atomics, float* ms) {
   if (rc == hipSuccess) rc = hipEventCreate(&e0);
   if (rc == hipSuccess) rc = hipEventCreate(&e1);
   if (rc == hipSuccess) {
-    hipLaunchKernelGGL(k_lds_atomics, dim3(nblocks), dim3(kLdsThreads), 0, 0, sink, 64);
+    hipLaunchKernelGGL(k_lds_atomics<T>, dim3(nblocks), dim3(kLdsThreads), 0, 0, sink, 64);
     rc = hipGetLastError();
   }
   if (rc == hipSuccess) rc = hipEventRecord(e0, 0);
   if (rc == hipSuccess) {
-    hipLaunchKernelGGL(k_lds_atomics, dim3(nblocks), dim3(kLdsThreads), 0, 0, sink, reps);
+    hipLaunchKernelGGL(k_lds_atomics<T>, dim3(nblocks), dim3(kLdsThreads), 0, 0, sink, reps);
     rc = hipGetLastError();
   }
   if (rc == hipSuccess) rc = hipEventRecord(e1, 0);
@@ -129,10 +134,10 @@ extern "C" int petstrip_probe_atomics(int device, int kind, int32_t width, int32
                                       int32_t reps, double* atomics_per_second,
                                       double* milliseconds) {
   if (!atomics_per_second || reps < 1 || kind < PETSTRIP_PROBE_GLOBAL_F32 ||
-      kind > PETSTRIP_PROBE_LDS_U64)
+      kind > PETSTRIP_PROBE_LDS_F32)
     return (int)hipErrorInvalidValue;
-  if (kind != PETSTRIP_PROBE_LDS_U64 && (width < 16 || height < 4 || width > (1 << 15) ||
-                                         height > (1 << 15)))
+  const bool lds = kind >= PETSTRIP_PROBE_LDS_U64;
+  if (!lds && (width < 16 || height < 4 || width > (1 << 15) || height > (1 << 15)))
     return (int)hipErrorInvalidValue;
   hipError_t rc = hipSetDevice(device);
   if (rc != hipSuccess) return (int)rc;
@@ -144,7 +149,9 @@ extern "C" int petstrip_probe_atomics(int device, int kind, int32_t width, int32
     case PETSTRIP_PROBE_GLOBAL_U64:
       rc = time_global<unsigned long long>(width, height, reps, &n, &ms);
       break;
-    default: rc = time_lds(reps, &n, &ms); break;
+    case PETSTRIP_PROBE_LDS_U32: rc = time_lds<unsigned>(reps, &n, &ms); break;
+    case PETSTRIP_PROBE_LDS_F32: rc = time_lds<float>(reps, &n, &ms); break;
+    default: rc = time_lds<unsigned long long>(reps, &n, &ms); break;
   }
   if (rc != hipSuccess) return (int)rc;
   *atomics_per_second = ms > 0.0f ? n / (ms * 1e-3) : 0.0;

@@ -220,6 +220,7 @@ def device_count():
 
 PROBE_LIB_PATH = os.path.join(HERE, "libpetstrip_probe.so")
 PROBE_GLOBAL_F32, PROBE_GLOBAL_F64, PROBE_GLOBAL_U64, PROBE_LDS_U64 = 0, 1, 2, 3
+PROBE_LDS_U32, PROBE_LDS_F32 = 4, 5
 _probe_lib = None
 
 

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 def test_probe_rates(gpu_pkg):
     pkg = gpu_pkg
     for kind in (pkg.PROBE_GLOBAL_F32, pkg.PROBE_GLOBAL_F64, pkg.PROBE_GLOBAL_U64,
-                 pkg.PROBE_LDS_U64):
+                 pkg.PROBE_LDS_U64, pkg.PROBE_LDS_U32, pkg.PROBE_LDS_F32):
         print(f"case: probe kind {kind}", flush=True)
         rate, ms = pkg.probe_atomics(kind, 256, 256, reps=64)
         print(f"  {rate:.3e} atomics/s in {ms:.3f} ms", flush=True)
