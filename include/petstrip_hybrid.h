@@ -113,6 +113,77 @@ int petstrip_hybrid_get_weights(petstrip_hybrid* h, float* weights, float* sensi
 int petstrip_hybrid_set_sensitivity_3d(petstrip_hybrid* h, const float* sensitivity,
                                        int32_t depth);
 
+/* Per-plane LOR-pixel weights, the reference's multi-plane geometry
+ * (GeometrySOA with n_planes_half weight planes, src/2d/barrel/
+ * geometry_soa.h:44-58; one system matrix per half plane,
+ * src/3d/hybrid/reconstruction_cmd.cpp:251-270).  raw_weights holds
+ * n_planes_half planes of n_pixel_infos weights (plane-major).  The
+ * sensitivity volume of n_planes_half planes is then
+ *   sensitivity_mode 0: per (pixel, plane) the sum of that plane's weights
+ *                       (calculate_sensitivity, reconstruction.h:236-250);
+ *                    1: one (--sens-to-one);
+ *                    2: the given volume (--sensitivity, n_planes_half
+ *                       planes of rows x columns, x + W (y + H p)),
+ * each plane's weights are divided by it (normalize_geometry_weights,
+ * reconstruction.h:252-271), and the iteration takes for voxel plane iz the
+ * weights and sensitivity of plane |iz - n_planes_half|
+ * (reconstruction.h:345-356; |iz - n_planes_half| = n_planes_half, at
+ * iz = 0, lies one plane past the reference's arrays: plane
+ * n_planes_half - 1 is used).  The frame events' grid filter keeps reading
+ * plane 0's weights (reconstruction.h:152-161).  n_planes_half = 1 sets the
+ * geometry weights and the 2D map instead. */
+int petstrip_hybrid_set_plane_weights(petstrip_hybrid* h, const float* raw_weights,
+                                      int32_t n_planes_half, int32_t sensitivity_mode,
+                                      const float* sensitivity);
+/* The normalised weights and the sensitivity the iteration uses, as the
+ * reference holds them after normalize_geometry_weights (GeometrySOA::
+ * pixel_weights, plane-major; Reconstruction::sensitivity, planes of rows x
+ * columns): n_weight_planes = the weight planes (1 for one set),
+ * n_sensitivity_planes = the volume's depth (1 for the 2D map).  Either
+ * array may be NULL (call with both NULL for the counts). */
+int petstrip_hybrid_get_plane_weights(petstrip_hybrid* h, float* weights, float* sensitivity,
+                                      int32_t* n_weight_planes,
+                                      int32_t* n_sensitivity_planes);
+
+/* --- system matrices (the --system option; host only) --------------------- */
+
+/* A 2D barrel system matrix file (src/2d/barrel/sparse_matrix.h:43-67,
+ * 163-238), full forms (PETP, TOFP); triangular forms fail as they do in the
+ * reference's 3D hybrid command.  Elements are kept sorted by LOR, TOF
+ * position and pixel (sort_by_lor_n_pixel). */
+typedef struct petstrip_system_matrix petstrip_system_matrix;
+int petstrip_system_matrix_read(const char* path, petstrip_system_matrix** out);
+void petstrip_system_matrix_free(petstrip_system_matrix* m);
+int petstrip_system_matrix_info(const petstrip_system_matrix* m, int32_t* n_pixels_in_row,
+                                int32_t* n_detectors, int64_t* n_emissions,
+                                int64_t* n_elements);
+/* GeometrySOA::load_weights_from_matrix_file (geometry_soa.h:210-267): each
+ * element's hits / emissions added to the geometry's pixel info of the same
+ * LOR and pixel (none: dropped); weights (n_pixel_infos) are overwritten. */
+int petstrip_system_matrix_weights(const petstrip_system_matrix* m,
+                                   const petstrip_hybrid_geometry* g, float* weights);
+/* Detector centres from a scanner description file, one detector per line
+ * of "x y" vertex pairs (ScannerBuilder::deserialize, src/2d/barrel/
+ * scanner_builder.h:21-35; Polygon::center, src/2d/geometry/polygon.h:56-68).
+ * centres (nullable) receives 2 floats per detector, up to capacity. */
+int petstrip_read_detector_centres(const char* path, float* centres, int32_t capacity,
+                                   int32_t* n_detectors);
+/* The geometry of a system matrix and detector centres (the GeometrySOA
+ * constructors of geometry_soa.h:71-182): pixel infos of each LOR from its
+ * matrix elements with weight hits / emissions, LORs of inactive detectors
+ * empty, segment from the second detector's centre to the first's, pixel
+ * positions along it (projection_scaled of the pixel centre on the grid
+ * with the given lower-left corner), sorted.  Call with segments = NULL for
+ * *n_pixel_infos, then with arrays of that many pixel infos and
+ * n_detectors (n_detectors + 1) / 2 LORs. */
+int petstrip_hybrid_geometry_from_matrix(const petstrip_system_matrix* m, const float* centres,
+                                         int32_t n_columns, int32_t n_rows, double pixel_size,
+                                         double lower_left_x, double lower_left_y,
+                                         const int16_t* inactive, int32_t n_inactive,
+                                         float* segments, float* widths, int64_t* info_begin,
+                                         int64_t* info_end, int16_t* pixels, float* positions,
+                                         float* weights, int64_t* n_pixel_infos);
+
 /* The voxel image, planes x rows x columns (voxel index x + W (y + H z)). */
 int petstrip_hybrid_set_rho(petstrip_hybrid* h, const float* rho);
 int petstrip_hybrid_get_rho(petstrip_hybrid* h, float* rho);

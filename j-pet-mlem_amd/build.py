@@ -26,7 +26,7 @@ ARCH = os.environ.get("PETSTRIP_ARCH", "gfx950")
 SOURCES = ["strip_kernels.hip", "strip_bins.hip", "strip_fused.hip", "petstrip_capi.hip",
            "phantom.hip", "hybrid.hip", "noalloc.hip"]
 PROBE_SOURCES = ["atomic_probe.hip"]
-HOST_SOURCES = ["text_responses.cpp"]
+HOST_SOURCES = ["text_responses.cpp", "system_matrix.cpp"]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall",
               "-I", os.path.join(ROOT, "include"), "-I", CSRC]

@@ -21,10 +21,12 @@
 //     k_hybrid_finalize makes it the image.  Image and accumulator are
 //     plane-fastest on the device.
 // The reference's backprojection reads each voxel's weight from a
-// voxel-indexed cache, so when an event's range holds one pixel twice (the
+// voxel-indexed cache, so when an event's range holds one pixel twice (a
+// LOR's pixel infos from a matrix with TOF positions repeat pixels, and the
 // extra pixel info one past the LOR's end, reconstruction.h:133-138, can
-// repeat a pixel of the range) both entries add the later entry's weight;
-// the walk here does the same.
+// repeat a pixel of the range) every entry adds the last entry's weight;
+// the walk here does the same through a per-pixel-info link to the next
+// entry of the same pixel in its LOR.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -237,11 +239,21 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 //         exp(-isz pq^2), pq = dz - dy tan)
 //   irm, irp  1 / (R - y), 1 / (R + y) of the frame sensitivity
 //   ws    weight x sensitivity (the denominator's factor)
-//   w2    the backprojection's weight (the extra pixel info's for a pixel it
-//         repeats)
+//   w2    the backprojection's weight: that of the range's last pixel info
+//         of the same pixel (the extra pixel info's for a pixel it repeats)
 //   vox0  the pixel's first voxel in the plane-fastest layout
+// With per-plane weights the two weights are read per voxel and the entry
+// holds their pixel infos instead (info, winfo).
 struct PixTerm {
-  float y, dyt, kpix, irm, irp, ws, w2;
+  float y, dyt, kpix, irm, irp;
+  union {
+    struct {
+      float ws, w2;
+    };
+    struct {
+      int32_t info, winfo;
+    };
+  };
   int32_t vox0;
 };
 constexpr int kMaxPix = 192;  // table entries per wave; longer ranges walk directly
@@ -274,14 +286,18 @@ __device__ __forceinline__ float voxel_kernel(const HKernel& k, const PixTerm& T
 // One wave per event.  The image and the accumulator are plane-fastest
 // (voxel (x, y, z) at z + planes (x + W y)), so the lanes of a wave, which
 // walk consecutive planes of a pixel, read and add contiguous words.
-// S3: with an external 3D sensitivity volume (sens3, sens_depth planes).
-template <bool S3>
+// S3: with a 3D sensitivity volume (sens3, sens_depth planes).  MP: with
+// per-plane weights as well (wmp: sens_depth weights per pixel info,
+// plane-fastest), the reference's multi-plane geometry
+// (reconstruction.h:345-356).
+template <bool S3, bool MP = false>
 __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
     const SegD* __restrict__ seg, const int32_t* __restrict__ pixels,
     const float* __restrict__ weights, const float* __restrict__ sens, int32_t n_infos,
     HGrid grid, HKernel k, const HEvent* __restrict__ events, int64_t n,
     const float* __restrict__ rho, float* __restrict__ acc, const float* __restrict__ sens3,
-    int sens_depth) {
+    int sens_depth, const float* __restrict__ wmp, const int32_t* __restrict__ next_same) {
+  static_assert(S3 || !MP, "per-plane weights come with their sensitivity volume");
   __shared__ PixTerm table_all[kWavesPerBlock][kMaxPix];
   __shared__ float cache_all[kWavesPerBlock][kCacheVox];
   const int lane = threadIdx.x & 63;
@@ -321,9 +337,18 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
       T.irm = __builtin_amdgcn_rcpf(s.R - T.y);
       T.irp = __builtin_amdgcn_rcpf(s.R + T.y);
       const int32_t pix = px + grid.w * py;
-      const float w = weights[info];
-      T.ws = w * sens[pix];
-      T.w2 = p == extra_pixel ? extra_w : w;
+      // the range's last pixel info of this pixel (the links run forward
+      // within the LOR; the extra one past its end comes last of all)
+      int32_t last = info;
+      for (int32_t nx = next_same[info]; nx >= 0 && nx < end; nx = next_same[nx]) last = nx;
+      if constexpr (MP) {
+        T.info = info;
+        T.winfo = p == extra_pixel ? ev.extra : last;
+      } else {
+        const float w = weights[info];
+        T.ws = w * sens[pix];
+        T.w2 = p == extra_pixel ? extra_w : (last == info ? w : weights[last]);
+      }
       T.vox0 = pix * planes;
       return T;
     };
@@ -350,13 +375,14 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
         float kr = 0.0f;
         if (!(k2 <= 0)) {
           kr = k2 * rho[T.vox0 + iz];
-          float ws = T.ws;
+          float ws = MP ? 1.0f : T.ws;
           if constexpr (S3) {
-            // an external volume of D planes, read at plane |iz - D|
-            // (reconstruction.h:357-362); |iz - D| = D is one plane past the
+            // a volume of D planes, read at plane |iz - D|
+            // (reconstruction.h:345-362); |iz - D| = D is one plane past the
             // volume there and reads plane D - 1 here
             int p = iz < sens_depth ? sens_depth - iz : iz - sens_depth;
             p = p < sens_depth - 1 ? p : sens_depth - 1;
+            if constexpr (MP) ws = wmp[(int64_t)T.info * sens_depth + p];
             ws *= sens3[T.vox0 + p];
           }
           den += kr * ws;
@@ -387,7 +413,15 @@ __global__ __launch_bounds__(kBlock) void k_hybrid_mlem(
           const float k2 = voxel_kernel(k, T, zc, ev.right);
           kr = !(k2 <= 0) ? k2 * rho[vox] : 0.0f;
         }
-        if (kr != 0.0f) unsafeAtomicAdd(acc_e + vox, kr * T.w2 * inv);
+        float w2;
+        if constexpr (MP) {
+          int p = iz < sens_depth ? sens_depth - iz : iz - sens_depth;
+          p = p < sens_depth - 1 ? p : sens_depth - 1;
+          w2 = wmp[(int64_t)T.winfo * sens_depth + p];
+        } else {
+          w2 = T.w2;
+        }
+        if (kr != 0.0f) unsafeAtomicAdd(acc_e + vox, kr * w2 * inv);
         i += step_i;
         z += step_z;
         if (z >= n_pl) {
@@ -475,12 +509,21 @@ struct petstrip_hybrid {
   int64_t* d_lend = nullptr;
   float* d_pos = nullptr;
   int32_t* d_pix = nullptr;
+  int32_t* d_next = nullptr;  // next pixel info of the same pixel in the LOR, or -1
   float* d_w = nullptr;
   float* d_sens = nullptr;
   // external sensitivity volume (sens_depth planes, 0: none), plane-fastest
   // with the image's stride: plane p of pixel pix at pix * planes + p
   float* d_sens3 = nullptr;
   int sens_depth = 0;
+  // per-plane weights (0: one set), sens_depth of them per pixel info,
+  // plane-fastest
+  int mp = 0;
+  float* d_wmp = nullptr;
+  // host copies for read-back: the normalised weight planes (mp of
+  // n_infos, plane-major; empty for one set) and the sensitivity volume
+  // (sens_depth planes of rows x columns; empty for the 2D map)
+  std::vector<float> plane_weights, sens_volume;
   float* d_rho = nullptr;
   float* d_acc = nullptr;
   HEvent* d_events = nullptr;
@@ -538,11 +581,13 @@ int one_iteration(petstrip_hybrid* h) {
     // one wave per event; enough waves to fill the device several times over
     const int64_t blocks = std::min<int64_t>((h->n_events + kWavesPerBlock - 1) / kWavesPerBlock,
                                              256 * 64);
-    auto kern = h->sens_depth > 0 ? k_hybrid_mlem<true> : k_hybrid_mlem<false>;
+    auto kern = h->mp > 0           ? k_hybrid_mlem<true, true>
+                : h->sens_depth > 0 ? k_hybrid_mlem<true>
+                                    : k_hybrid_mlem<false>;
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kBlock), 0, h->stream, h->d_seg,
                        h->d_pix, h->d_w, h->d_sens, (int32_t)h->n_infos, h->grid, h->kern,
                        h->d_events, h->n_events, h->d_rho, h->d_acc, h->d_sens3,
-                       h->sens_depth);
+                       h->sens_depth, h->d_wmp, h->d_next);
     HTRY(hipGetLastError());
   }
   HTRY(hipEventRecord(h->k1, h->stream));
@@ -644,6 +689,20 @@ int petstrip_hybrid_create(const petstrip_hybrid_geometry* g, double sigma_z, do
   std::vector<int32_t> pix(h->n_infos);
   for (int64_t i = 0; i < h->n_infos; ++i)
     pix[i] = (int32_t)(uint16_t)g->pixels[2 * i] | ((int32_t)g->pixels[2 * i + 1] << 16);
+  // links between a LOR's pixel infos of one pixel, walked backwards with
+  // the last index seen per pixel
+  std::vector<int32_t> next(h->n_infos, -1);
+  {
+    std::vector<int32_t> seen(np, -1);
+    for (int64_t l = 0; l < n_lors; ++l) {
+      for (int64_t i = g->info_end[l] - 1; i >= g->info_begin[l]; --i) {
+        int32_t& s = seen[h->pix_index[i]];
+        next[i] = s;
+        s = (int32_t)i;
+      }
+      for (int64_t i = g->info_begin[l]; i < g->info_end[l]; ++i) seen[h->pix_index[i]] = -1;
+    }
+  }
 
   auto fail_free = [&](int rc) {
     petstrip_hybrid_destroy(h);
@@ -657,7 +716,8 @@ int petstrip_hybrid_create(const petstrip_hybrid_geometry* g, double sigma_z, do
   int rc;
   if ((rc = dalloc(&h->d_seg, n_lors)) || (rc = dalloc(&h->d_lbeg, n_lors)) ||
       (rc = dalloc(&h->d_lend, n_lors)) || (rc = dalloc(&h->d_pos, h->n_infos)) ||
-      (rc = dalloc(&h->d_pix, h->n_infos)) || (rc = dalloc(&h->d_w, h->n_infos)) ||
+      (rc = dalloc(&h->d_pix, h->n_infos)) || (rc = dalloc(&h->d_next, h->n_infos)) ||
+      (rc = dalloc(&h->d_w, h->n_infos)) ||
       (rc = dalloc(&h->d_sens, np)) || (rc = dalloc(&h->d_rho, h->n_voxels_grid)) ||
       (rc = dalloc(&h->d_acc, kAccCopies * h->n_voxels_grid)))
     return fail_free(rc);
@@ -671,6 +731,8 @@ int petstrip_hybrid_create(const petstrip_hybrid_geometry* g, double sigma_z, do
     e = hipMemcpy(h->d_pos, g->positions, h->n_infos * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(h->d_pix, pix.data(), h->n_infos * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->d_next, next.data(), h->n_infos * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(h->d_rho, ones.data(), h->n_voxels_grid * sizeof(float),
                   hipMemcpyHostToDevice);
@@ -691,9 +753,11 @@ void petstrip_hybrid_destroy(petstrip_hybrid* h) {
   dfree(h->d_lend);
   dfree(h->d_pos);
   dfree(h->d_pix);
+  dfree(h->d_next);
   dfree(h->d_w);
   dfree(h->d_sens);
   dfree(h->d_sens3);
+  dfree(h->d_wmp);
   dfree(h->d_rho);
   dfree(h->d_acc);
   dfree(h->d_events);
@@ -843,6 +907,9 @@ int petstrip_hybrid_set_weights(petstrip_hybrid* h, const float* weights,
   if (sensitivity) {
     std::memcpy(h->sens.data(), sensitivity, h->sens.size() * sizeof(float));
     h->sens_depth = 0;  // the map replaces an external volume
+    h->mp = 0;
+    h->plane_weights.clear();
+    h->sens_volume.clear();
   }
   HTRY(hipSetDevice(h->device));
   return upload_weights(h);
@@ -870,7 +937,96 @@ int petstrip_hybrid_set_sensitivity_3d(petstrip_hybrid* h, const float* sensitiv
   HTRY(hipMemcpyAsync(h->d_sens3, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice,
                       h->stream));
   h->sens_depth = depth;
+  h->mp = 0;
+  h->plane_weights.clear();
+  h->sens_volume.assign(sensitivity, sensitivity + (size_t)depth * np);
   return upload_weights(h);
+}
+
+int petstrip_hybrid_set_plane_weights(petstrip_hybrid* h, const float* raw_weights,
+                                      int32_t n_planes_half, int32_t sensitivity_mode,
+                                      const float* sensitivity) {
+  if (!h || !raw_weights || sensitivity_mode < 0 || sensitivity_mode > 2 ||
+      (sensitivity_mode == 2 && !sensitivity))
+    return set_error(PETSTRIP_EINVAL, "bad argument");
+  const int planes = h->grid.planes;
+  const int64_t ni = h->n_infos;
+  const size_t np = (size_t)h->grid.w * h->grid.h;
+  if (n_planes_half == 1) {
+    // one set: the geometry weights with the 2D map or the external volume
+    h->raw_weights.assign(raw_weights, raw_weights + ni);
+    if (sensitivity_mode == 2) return petstrip_hybrid_set_sensitivity_3d(h, sensitivity, planes / 2);
+    h->sens.assign(np, sensitivity_mode == 1 ? 1.0f : 0.0f);
+    if (sensitivity_mode == 0)
+      for (int64_t i = 0; i < ni; ++i) h->sens[h->pix_index[i]] += raw_weights[i];
+    for (int64_t i = 0; i < ni; ++i) h->weights[i] = raw_weights[i] / h->sens[h->pix_index[i]];
+    h->sens_depth = 0;
+    h->mp = 0;
+    h->plane_weights.clear();
+    h->sens_volume.clear();
+    HTRY(hipSetDevice(h->device));
+    return upload_weights(h);
+  }
+  if (n_planes_half < 1 || n_planes_half != planes / 2)
+    return set_error(PETSTRIP_EINVAL,
+                     "you must specify either one or half no. planes matrices");
+  const int H = n_planes_half;
+  // the sensitivity volume (calculate_sensitivity: per pixel and plane, the
+  // plane's weights in pixel-info order) and the normalised weights
+  std::vector<float> vol((size_t)H * np, sensitivity_mode == 1 ? 1.0f : 0.0f);
+  if (sensitivity_mode == 2) {
+    std::memcpy(vol.data(), sensitivity, vol.size() * sizeof(float));
+  } else if (sensitivity_mode == 0) {
+    for (int64_t i = 0; i < ni; ++i)
+      for (int p = 0; p < H; ++p) vol[h->pix_index[i] + np * p] += raw_weights[p * ni + i];
+  }
+  std::vector<float> w((size_t)ni * H);  // plane-fastest per pixel info
+  for (int64_t i = 0; i < ni; ++i)
+    for (int p = 0; p < H; ++p)
+      w[(size_t)i * H + p] = raw_weights[p * ni + i] / vol[h->pix_index[i] + np * p];
+  std::vector<float> t((size_t)h->n_voxels_grid, 0.0f);
+  for (int p = 0; p < H; ++p)
+    for (size_t pix = 0; pix < np; ++pix) t[pix * planes + p] = vol[pix + np * p];
+  // plane 0 is what the frame events' grid filter reads (reconstruction.h:152-161)
+  h->raw_weights.assign(raw_weights, raw_weights + ni);
+  for (int64_t i = 0; i < ni; ++i) h->weights[i] = w[(size_t)i * H];
+  h->sens.assign(np, 1.0f);
+  HTRY(hipSetDevice(h->device));
+  int rc;
+  if (!h->d_sens3 && (rc = dalloc(&h->d_sens3, (size_t)h->n_voxels_grid))) return rc;
+  dfree(h->d_wmp);
+  if ((rc = dalloc(&h->d_wmp, w.size()))) return rc;
+  HTRY(hipMemcpyAsync(h->d_sens3, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice,
+                      h->stream));
+  HTRY(hipMemcpyAsync(h->d_wmp, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice,
+                      h->stream));
+  h->sens_depth = H;
+  h->mp = H;
+  h->plane_weights.resize(w.size());
+  for (int p = 0; p < H; ++p)
+    for (int64_t i = 0; i < ni; ++i) h->plane_weights[p * ni + i] = w[(size_t)i * H + p];
+  h->sens_volume = std::move(vol);
+  return upload_weights(h);
+}
+
+int petstrip_hybrid_get_plane_weights(petstrip_hybrid* h, float* weights, float* sensitivity,
+                                      int32_t* n_weight_planes,
+                                      int32_t* n_sensitivity_planes) {
+  if (!h) return set_error(PETSTRIP_EINVAL, "bad argument");
+  const size_t np = (size_t)h->grid.w * h->grid.h;
+  const int32_t wp = h->mp > 0 ? h->mp : 1;
+  const int32_t sp = h->sens_depth > 0 ? h->sens_depth : 1;
+  if (n_weight_planes) *n_weight_planes = wp;
+  if (n_sensitivity_planes) *n_sensitivity_planes = sp;
+  if (weights) {
+    const float* src = h->mp > 0 ? h->plane_weights.data() : h->weights.data();
+    std::memcpy(weights, src, (size_t)wp * h->n_infos * sizeof(float));
+  }
+  if (sensitivity) {
+    const float* src = h->sens_depth > 0 ? h->sens_volume.data() : h->sens.data();
+    std::memcpy(sensitivity, src, (size_t)sp * np * sizeof(float));
+  }
+  return PETSTRIP_OK;
 }
 
 int petstrip_hybrid_get_weights(petstrip_hybrid* h, float* weights, float* sensitivity) {

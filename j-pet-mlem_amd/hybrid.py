@@ -7,6 +7,7 @@ as 3d_hybrid_reconstruction drives it (src/3d/hybrid/reconstruction_cmd.cpp:
 it for n iterations, read rho (planes x rows x columns).
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -165,6 +166,34 @@ class HybridReconstruction:
                              f"got {v.shape}")
         _check(_lib().petstrip_hybrid_set_sensitivity_3d(self._h, _p(v), v.shape[0]))
 
+    def set_plane_weights(self, raw, sensitivity_mode=0, sensitivity=None):
+        """Per-plane LOR-pixel weights, the reference's multi-plane geometry
+        (one system matrix per half plane): raw (n_planes_half, n_pixel_infos);
+        sensitivity_mode 0 sums each plane's weights per pixel, 1 sets one, 2
+        takes `sensitivity` (n_planes_half, rows, columns); each plane's weights
+        are divided by it, and voxel plane iz takes plane |iz - n_planes_half|
+        (include/petstrip_hybrid.h)."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        if raw.ndim == 1:
+            raw = raw[None, :]
+        sv = None if sensitivity is None else np.ascontiguousarray(sensitivity, np.float32)
+        _check(_bind_system().petstrip_hybrid_set_plane_weights(
+            self._h, _p(raw), raw.shape[0], int(sensitivity_mode), _p(sv)))
+
+    def plane_weights(self):
+        """The normalised weights (weight planes, n_pixel_infos) and the
+        sensitivity (planes, rows, columns) the iteration uses: one plane
+        each for the geometry weights and the 2D map."""
+        lib = _bind_system()
+        nw, ns = C.c_int32(), C.c_int32()
+        _check(lib.petstrip_hybrid_get_plane_weights(self._h, None, None, C.byref(nw),
+                                                     C.byref(ns)))
+        w = np.zeros((nw.value, self.geometry.n_pixel_infos), np.float32)
+        s = np.zeros((ns.value,) + self.shape[1:], np.float32)
+        _check(lib.petstrip_hybrid_get_plane_weights(self._h, _p(w), _p(s), C.byref(nw),
+                                                     C.byref(ns)))
+        return w, s
+
     @property
     def rho(self):
         out = np.zeros(self.shape, np.float32)
@@ -206,3 +235,119 @@ class HybridReconstruction:
         s = CHybridStats()
         _check(_lib().petstrip_hybrid_stats_get(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in CHybridStats._fields_}
+
+
+def _bind_system():
+    lib = _lib()
+    vp, i64, i32, d = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+    sig = {
+        "petstrip_hybrid_set_plane_weights": ([vp, vp, i32, i32, vp], C.c_int),
+        "petstrip_hybrid_get_plane_weights": ([vp, vp, vp, C.POINTER(C.c_int32),
+                                               C.POINTER(C.c_int32)], C.c_int),
+        "petstrip_system_matrix_read": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+        "petstrip_system_matrix_free": ([vp], None),
+        "petstrip_system_matrix_info": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+        "petstrip_system_matrix_weights": ([vp, C.POINTER(CHybridGeometry), vp], C.c_int),
+        "petstrip_read_detector_centres": ([C.c_char_p, vp, i32, C.POINTER(C.c_int32)],
+                                           C.c_int),
+        "petstrip_hybrid_geometry_from_matrix": ([vp, vp, i32, i32, d, d, d, vp, i32, vp, vp,
+                                                  vp, vp, vp, vp, vp, C.POINTER(C.c_int64)],
+                                                 C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    return lib
+
+
+def _cgeom(g):
+    return CHybridGeometry(g.n_detectors, g.n_columns, g.n_rows, g.n_planes, g.pixel_size,
+                           g.lower_left[0], g.lower_left[1], g.z_left, g.n_pixel_infos,
+                           _p(g.segments), _p(g.widths), _p(g.info_begin), _p(g.info_end),
+                           _p(g.pixels), _p(g.positions), _p(g.weights))
+
+
+class SystemMatrix:
+    """A 2D barrel system matrix file, full form (the reference's
+    SparseMatrix, src/2d/barrel/sparse_matrix.h:163-238), read by the
+    library (host only)."""
+
+    def __init__(self, path):
+        lib = _bind_system()
+        self._m = C.c_void_p()
+        _check(lib.petstrip_system_matrix_read(os.fsencode(path), C.byref(self._m)))
+        npr, nd = C.c_int32(), C.c_int32()
+        ne, nel = C.c_int64(), C.c_int64()
+        _check(lib.petstrip_system_matrix_info(self._m, C.byref(npr), C.byref(nd),
+                                               C.byref(ne), C.byref(nel)))
+        self.n_pixels_in_row, self.n_detectors = npr.value, nd.value
+        self.n_emissions, self.n_elements = ne.value, nel.value
+
+    def close(self):
+        if self._m:
+            _bind_system().petstrip_system_matrix_free(self._m)
+            self._m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def weights(self, g):
+        """load_weights_from_matrix_file (geometry_soa.h:210-267) for geometry
+        g: hits / emissions per pixel info (n_pixel_infos floats)."""
+        w = np.zeros(g.n_pixel_infos, np.float32)
+        cg = _cgeom(g)
+        _check(_bind_system().petstrip_system_matrix_weights(self._m, C.byref(cg), _p(w)))
+        return w
+
+    def geometry(self, centres, pixel_size, n_planes=None, z_left=None, inactive=()):
+        """The GeometrySOA of this matrix and the detector centres
+        (geometry_soa.h:71-182) on the matrix's centred grid of pixel_size,
+        the LORs of the inactive detectors empty; n_planes (default the
+        matrix's pixels per row) and z_left (default centred) as the
+        command line assumes them (reconstruction_cmd.cpp:380-397)."""
+        lib = _bind_system()
+        n = self.n_pixels_in_row
+        n_planes = n if n_planes is None else int(n_planes)
+        ps = np.float32(pixel_size)
+        # PixelGrid(n, n, s): lower left (-s n / 2, -s n / 2) in float
+        ll = float(np.float32(-ps * np.float32(n)) / np.float32(2))
+        z_left = float(-n_planes * float(pixel_size) / 2) if z_left is None else float(z_left)
+        cen = np.ascontiguousarray(centres, np.float32).reshape(-1, 2)
+        ina = np.ascontiguousarray(inactive, np.int16)
+        ni = C.c_int64()
+        _check(lib.petstrip_hybrid_geometry_from_matrix(
+            self._m, _p(cen), n, n, float(ps), ll, ll, _p(ina) if len(ina) else None, len(ina),
+            None, None, None, None, None, None, None, C.byref(ni)))
+        nd = self.n_detectors
+        n_lors = nd * (nd + 1) // 2
+        seg = np.zeros((n_lors, 4), np.float32)
+        wid = np.zeros(n_lors, np.float32)
+        beg = np.zeros(n_lors, np.int64)
+        end = np.zeros(n_lors, np.int64)
+        pix = np.zeros((max(ni.value, 1), 2), np.int16)
+        pos = np.zeros(max(ni.value, 1), np.float32)
+        w = np.zeros(max(ni.value, 1), np.float32)
+        _check(lib.petstrip_hybrid_geometry_from_matrix(
+            self._m, _p(cen), n, n, float(ps), ll, ll, _p(ina) if len(ina) else None, len(ina),
+            _p(seg), _p(wid), _p(beg), _p(end), _p(pix), _p(pos), _p(w), C.byref(ni)))
+        m = ni.value
+        ring = float(np.mean(np.hypot(cen[:, 0], cen[:, 1]))) if len(cen) else 0.0
+        return HybridGeometry(nd, n, n, float(ps), (ll, ll), z_left, n_planes, seg, wid, beg,
+                              end, pix[:m], pos[:m], w[:m], ring)
+
+
+def read_detector_centres(path):
+    """Detector centres (n, 2) of a scanner description file
+    (ScannerBuilder::deserialize, Polygon::center)."""
+    lib = _bind_system()
+    n = C.c_int32()
+    _check(lib.petstrip_read_detector_centres(os.fsencode(path), None, 0, C.byref(n)))
+    out = np.zeros((n.value, 2), np.float32)
+    _check(lib.petstrip_read_detector_centres(os.fsencode(path), _p(out), n.value,
+                                              C.byref(n)))
+    return out

@@ -282,3 +282,104 @@ def write_geometry(g, path):
                 pi["t"] = g.positions[b:e]
                 pi["weight"] = g.weights[b:e]
                 f.write(pi.tobytes())
+
+
+# --- system matrices and scanner description files (the --system inputs) ----
+
+def write_system_matrix(path, n_pixels_in_row, n_emissions, n_detectors, elements,
+                        n_tof_positions=None):
+    """Writes a 2D barrel system matrix file in the full form (magic "PETP",
+    or "TOFP" with n_tof_positions; src/2d/barrel/sparse_matrix.h:43-67): the
+    header (with the TOF position count after the detectors), then per group
+    of consecutive elements with the same (a, b) detectors: a, b (uint16),
+    the count (uint32) and per element [the TOF position (uint32)], x, y
+    (uint16) and hits (uint32).  elements: rows (a, b, x, y, hits), or
+    (a, b, position, x, y, hits) with TOF, written in the given order."""
+    tof = n_tof_positions is not None
+    el = np.asarray(elements, dtype=np.int64).reshape(-1, 6 if tof else 5)
+    if not tof:
+        el = np.insert(el, 2, 0, axis=1)
+    with open(path, "wb") as f:
+        f.write(b"TOFP" if tof else b"PETP")
+        f.write(np.array([n_pixels_in_row, n_emissions, n_detectors], "<u4").tobytes())
+        if tof:
+            f.write(np.array([n_tof_positions], "<u4").tobytes())
+        fields = ([("position", "<u4")] if tof else []) + [("x", "<u2"), ("y", "<u2"),
+                                                           ("hits", "<u4")]
+        k = 0
+        while k < len(el):
+            j = k
+            while j < len(el) and el[j, 0] == el[k, 0] and el[j, 1] == el[k, 1]:
+                j += 1
+            f.write(np.array([el[k, 0], el[k, 1]], "<u2").tobytes())
+            f.write(np.array([j - k], "<u4").tobytes())
+            rec = np.zeros(j - k, dtype=fields)
+            if tof:
+                rec["position"] = el[k:j, 2]
+            rec["x"], rec["y"], rec["hits"] = el[k:j, 3], el[k:j, 4], el[k:j, 5]
+            f.write(rec.tobytes())
+            k = j
+
+
+def synthetic_system_matrix(g, n_emissions=10000, seed=1, plane=0):
+    """Elements (a, b, x, y, hits) of a system matrix for geometry g: every
+    pixel info of every LOR with hits falling off with its distance to the
+    LOR (and with `plane`, for per-plane matrices), plus a few pixels the
+    geometry's LOR lacks, the LORs in shuffled order, each LOR's pixels
+    shuffled, detectors of half the LORs swapped (a < b)."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for first in range(g.n_detectors):
+        for second in range(first):
+            li = first * (first + 1) // 2 + second
+            b, e = int(g.info_begin[li]), int(g.info_end[li])
+            if e <= b:
+                continue
+            px = g.pixels[b:e].astype(np.int64)
+            t = g.positions[b:e].astype(np.float64)
+            hits = (n_emissions * 0.05 * np.exp(-4 * (t - 0.5) ** 2) / (1 + plane)
+                    * rng.uniform(0.5, 1.5, e - b)).astype(np.int64) + 1
+            a_, b_ = (first, second) if rng.random() < 0.5 else (second, first)
+            lor_rows = [(a_, b_, int(x), int(y), int(h)) for (x, y), h in zip(px, hits)]
+            if rng.random() < 0.2:  # a pixel outside the geometry's range
+                lor_rows.append((a_, b_, int(rng.integers(0, g.n_columns)),
+                                 int(rng.integers(0, g.n_rows)), int(rng.integers(1, 50))))
+            rng.shuffle(lor_rows)
+            rows.append(lor_rows)
+    rng.shuffle(rows)
+    return np.array([r for lr in rows for r in lr], dtype=np.int64)
+
+
+def synthetic_tof_matrix(g, n_emissions=10000, seed=1, n_tof_positions=3):
+    """Elements (a, b, position, x, y, hits) of a TOF system matrix: those of
+    synthetic_system_matrix with each (LOR, pixel) split over one to
+    n_tof_positions TOF positions, so that a LOR holds its pixels several
+    times."""
+    rng = np.random.default_rng(seed + 7919)
+    rows = []
+    for a, b, x, y, h in synthetic_system_matrix(g, n_emissions, seed):
+        k = int(rng.integers(1, n_tof_positions + 1))
+        pos = rng.choice(n_tof_positions, size=k, replace=False)
+        for p in pos:
+            rows.append((a, b, int(p), x, y, max(1, int(h * rng.uniform(0.2, 1.0)))))
+    return np.array(rows, dtype=np.int64)
+
+
+def write_detector_file(path, n_detectors, ring_radius, width=0.007, height=0.019):
+    """A scanner description file (ScannerBuilder::deserialize,
+    src/2d/barrel/scanner_builder.h:21-35): per detector one line of its
+    four corners "x y x y x y x y", a width x height rectangle centred on
+    the ring's detector centre, its height along the radius."""
+    c = detector_centres(n_detectors, ring_radius)
+    lines = []
+    for x, y in c:
+        r = np.hypot(x, y)
+        ux, uy = x / r, y / r  # radial
+        tx, ty = -uy, ux       # tangential
+        pts = []
+        for sw, sh in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+            pts += [x + sw * width / 2 * tx + sh * height / 2 * ux,
+                    y + sw * width / 2 * ty + sh * height / 2 * uy]
+        lines.append(" ".join(f"{v:.9g}" for v in pts))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

@@ -293,11 +293,14 @@ static inline size_t sens_index(size_t pix, size_t np, int iz, int depth) {
  * denominator over (pixel info, plane) in that order, each voxel's weight
  * kept in a voxel-indexed cache, then every (pixel info, plane) adds its
  * voxel's cached weight over the denominator; rho becomes the sum. */
-void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
-                            double sigma_dl, double length, const float* weights,
-                            const float* sensitivity, int sens_depth,
-                            const oracle_hybrid_event* ev, int64_t n, int n_iter,
-                            const float* rho0, float* out) {
+/* weight_planes > 1: per-plane weights (weight_planes x n_pixel_infos),
+ * voxel plane iz reading plane |iz - D| of them as of the sensitivity
+ * volume (D = sens_depth = weight_planes, reconstruction.h:345-356). */
+void oracle_hybrid_mlem_planes_f32(const oracle_hybrid_geometry* g, double sigma_z,
+                                   double sigma_dl, double length, const float* weights,
+                                   int weight_planes, const float* sensitivity,
+                                   int sens_depth, const oracle_hybrid_event* ev, int64_t n,
+                                   int n_iter, const float* rho0, float* out) {
   const k32 k = kernel_consts_f(sigma_z, sigma_dl);
   const grid32 r = grid_f(g);
   const float L = (float)length, z_left = (float)g->z_left;
@@ -316,7 +319,6 @@ void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
       float denom = 0;
       for (int64_t i = v->begin; i < v->end; ++i) {
         const int16_t px = g->pixels[2 * i], py = g->pixels[2 * i + 1];
-        const float w = weights[i];
         const size_t pix = (size_t)px + (size_t)g->n_columns * py;
         float cx, cy;
         center_f(&r, px, py, &cx, &cy);
@@ -330,8 +332,11 @@ void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
             cache[vox] = 0;
             continue;
           }
+          const size_t si = sens_index(pix, np, iz, sens_depth);
+          const float w =
+              weight_planes > 1 ? weights[(si / np) * (size_t)g->n_pixel_infos + i] : weights[i];
           const float wt = k2 * w * rho[vox];
-          denom += wt * sensitivity[sens_index(pix, np, iz, sens_depth)];
+          denom += wt * sensitivity[si];
           cache[vox] = wt;
         }
       }
@@ -354,11 +359,20 @@ void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
   free(cache);
 }
 
-void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
-                            double sigma_dl, double length, const double* weights,
-                            const double* sensitivity, int sens_depth,
+void oracle_hybrid_mlem_f32(const oracle_hybrid_geometry* g, double sigma_z,
+                            double sigma_dl, double length, const float* weights,
+                            const float* sensitivity, int sens_depth,
                             const oracle_hybrid_event* ev, int64_t n, int n_iter,
-                            const double* rho0, double* out) {
+                            const float* rho0, float* out) {
+  oracle_hybrid_mlem_planes_f32(g, sigma_z, sigma_dl, length, weights, 1, sensitivity,
+                                sens_depth, ev, n, n_iter, rho0, out);
+}
+
+void oracle_hybrid_mlem_planes_f64(const oracle_hybrid_geometry* g, double sigma_z,
+                                   double sigma_dl, double length, const double* weights,
+                                   int weight_planes, const double* sensitivity,
+                                   int sens_depth, const oracle_hybrid_event* ev, int64_t n,
+                                   int n_iter, const double* rho0, double* out) {
   const double isz = 1 / (sigma_z * sigma_z), isdl = 1 / (sigma_dl * sigma_dl);
   const double c = 1 / (2 * M_PI * M_PI) * sqrt(isz * isz * isdl);
   const size_t np = (size_t)g->n_columns * g->n_rows;
@@ -390,8 +404,11 @@ void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
             cache[vox] = 0;
             continue;
           }
-          const double wt = k2 * weights[i] * rho[vox];
-          denom += wt * sensitivity[sens_index(pix, np, iz, sens_depth)];
+          const size_t si = sens_index(pix, np, iz, sens_depth);
+          const double w =
+              weight_planes > 1 ? weights[(si / np) * (size_t)g->n_pixel_infos + i] : weights[i];
+          const double wt = k2 * w * rho[vox];
+          denom += wt * sensitivity[si];
           cache[vox] = wt;
         }
       }
@@ -411,4 +428,53 @@ void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
   free(rho);
   free(next);
   free(cache);
+}
+
+void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
+                            double sigma_dl, double length, const double* weights,
+                            const double* sensitivity, int sens_depth,
+                            const oracle_hybrid_event* ev, int64_t n, int n_iter,
+                            const double* rho0, double* out) {
+  oracle_hybrid_mlem_planes_f64(g, sigma_z, sigma_dl, length, weights, 1, sensitivity,
+                                sens_depth, ev, n, n_iter, rho0, out);
+}
+
+/* Per-plane weights (reconstruction.h:236-271 over n_planes_half planes):
+ * the sensitivity volume per (pixel, plane) as the sum of the plane's
+ * weights in pixel-info order (sens_mode 0), one (1) or given (2, left in
+ * sensitivity as is), then each plane's weights divided by it. */
+void oracle_hybrid_plane_weights_f32(const oracle_hybrid_geometry* g, const float* raw,
+                                     int planes, int sens_mode, float* weights,
+                                     float* sensitivity) {
+  const size_t np = (size_t)g->n_columns * g->n_rows, ni = (size_t)g->n_pixel_infos;
+  if (sens_mode != 2)
+    for (size_t k = 0; k < np * (size_t)planes; ++k) sensitivity[k] = sens_mode ? 1.0f : 0.0f;
+  if (sens_mode == 0)
+    for (size_t i = 0; i < ni; ++i)
+      for (int p = 0; p < planes; ++p)
+        sensitivity[g->pixels[2 * i] + (size_t)g->n_columns * g->pixels[2 * i + 1] + np * p] +=
+            raw[p * ni + i];
+  for (size_t i = 0; i < ni; ++i)
+    for (int p = 0; p < planes; ++p)
+      weights[p * ni + i] =
+          raw[p * ni + i] /
+          sensitivity[g->pixels[2 * i] + (size_t)g->n_columns * g->pixels[2 * i + 1] + np * p];
+}
+
+void oracle_hybrid_plane_weights_f64(const oracle_hybrid_geometry* g, const double* raw,
+                                     int planes, int sens_mode, double* weights,
+                                     double* sensitivity) {
+  const size_t np = (size_t)g->n_columns * g->n_rows, ni = (size_t)g->n_pixel_infos;
+  if (sens_mode != 2)
+    for (size_t k = 0; k < np * (size_t)planes; ++k) sensitivity[k] = sens_mode ? 1.0 : 0.0;
+  if (sens_mode == 0)
+    for (size_t i = 0; i < ni; ++i)
+      for (int p = 0; p < planes; ++p)
+        sensitivity[g->pixels[2 * i] + (size_t)g->n_columns * g->pixels[2 * i + 1] + np * p] +=
+            raw[p * ni + i];
+  for (size_t i = 0; i < ni; ++i)
+    for (int p = 0; p < planes; ++p)
+      weights[p * ni + i] =
+          raw[p * ni + i] /
+          sensitivity[g->pixels[2 * i] + (size_t)g->n_columns * g->pixels[2 * i + 1] + np * p];
 }

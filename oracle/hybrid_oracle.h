@@ -77,6 +77,34 @@ void oracle_hybrid_mlem_f64(const oracle_hybrid_geometry* g, double sigma_z,
                             const oracle_hybrid_event* ev, int64_t n, int n_iter,
                             const double* rho, double* out);
 
+/* The iteration with per-plane weights (weight_planes x n_pixel_infos,
+ * plane-major), voxel plane iz reading weight plane |iz - D| as it reads the
+ * sensitivity volume's (D = sens_depth = weight_planes; the reference's
+ * multi-plane geometry, reconstruction.h:345-356); weight_planes = 1 is
+ * oracle_hybrid_mlem_*. */
+void oracle_hybrid_mlem_planes_f32(const oracle_hybrid_geometry* g, double sigma_z,
+                                   double sigma_dl, double length, const float* weights,
+                                   int weight_planes, const float* sensitivity,
+                                   int sens_depth, const oracle_hybrid_event* ev, int64_t n,
+                                   int n_iter, const float* rho, float* out);
+void oracle_hybrid_mlem_planes_f64(const oracle_hybrid_geometry* g, double sigma_z,
+                                   double sigma_dl, double length, const double* weights,
+                                   int weight_planes, const double* sensitivity,
+                                   int sens_depth, const oracle_hybrid_event* ev, int64_t n,
+                                   int n_iter, const double* rho, double* out);
+
+/* Sensitivity volume and normalised weights of per-plane weights
+ * (reconstruction.h:236-271 over `planes` planes): raw and weights are
+ * planes x n_pixel_infos; sensitivity planes x rows x columns is computed
+ * (sens_mode 0: sums of each plane's weights per pixel in pixel-info order;
+ * 1: one) or given (2). */
+void oracle_hybrid_plane_weights_f32(const oracle_hybrid_geometry* g, const float* raw,
+                                     int planes, int sens_mode, float* weights,
+                                     float* sensitivity);
+void oracle_hybrid_plane_weights_f64(const oracle_hybrid_geometry* g, const double* raw,
+                                     int planes, int sens_mode, double* weights,
+                                     double* sensitivity);
+
 #ifdef __cplusplus
 }
 #endif

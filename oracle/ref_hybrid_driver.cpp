@@ -14,6 +14,7 @@
 // runs the iterations (one OpenMP thread unless asked for more: with one,
 // the accumulation order is the reference's sequential event order; the
 // iterations' wall time is returned for the CPU baseline).
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -25,6 +26,8 @@
 #endif
 
 #include "2d/barrel/generic_scanner.h"
+#include "2d/barrel/sparse_matrix.h"
+#include "common/types.h"
 #include "2d/barrel/geometry.h"
 #include "2d/barrel/scanner_builder.h"
 #include "2d/barrel/square_detector.h"
@@ -42,6 +45,8 @@ using Reconstruction = PET3D::Hybrid::Reconstruction<Scanner, Kernel>;
 using Geometry = PET2D::Barrel::Geometry<F, S>;
 using LOR = PET2D::Barrel::LOR<S>;
 using Point2D = PET2D::Point<F>;
+using Pixel = PET2D::Pixel<S>;
+using Matrix = PET2D::Barrel::SparseMatrix<Pixel, LOR, Hit>;
 }  // namespace
 
 extern "C" {
@@ -138,6 +143,175 @@ int ref_hybrid_run(int n_detectors, int n_columns, int n_rows, float pixel_size,
       if (out_rho) std::memcpy(out_rho + (size_t)it * n_vox, rec.rho.data, sizeof(F) * n_vox);
     }
     if (iteration_seconds) *iteration_seconds = seconds;
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+// The reference's --system paths (reconstruction_cmd.cpp:241-450):
+//   mode 0: a geometry (arrays as ref_hybrid_run) with n_matrices system
+//           matrix files, one or n_planes / 2, loaded into the weight planes
+//           (GeometrySOA(geometry, n_matrices),
+//           load_weights_from_matrix_file);
+//   mode 1: the geometry of one matrix file and the detector centres of a
+//           scanner description file (ScannerBuilder::deserialize), LORs of
+//           the inactive detectors left out (GeometrySOA(matrix, centres,
+//           grid, inactive)); the grid is the matrix's pixels of
+//           pixel_size, centred.
+// Then, as run_reconstruction does with --system (no calculate_weight): the
+// sensitivity (sens_mode 0 calculated, 1 one, 2 the given volume of
+// n_planes / 2 planes), normalize_geometry_weights, the responses through
+// the stream reader and n_iterations iterations with one thread.
+// Outputs: the geometry arrays (mode 1; out_n_infos = the reference's
+// n_pixel_infos, which counts the matrix's elements, skipped ones
+// included), the weights of every plane, the sensitivity (depth planes),
+// frames and images as ref_hybrid_run.
+int ref_hybrid_run_system(int mode, int n_detectors, int n_columns, int n_rows,
+                          float pixel_size, float lower_left_x, float lower_left_y,
+                          const float* segments, const float* widths,
+                          const int64_t* info_begin, const int64_t* info_end,
+                          const int16_t* pixels, const float* positions,
+                          const char* const* matrix_paths, int n_matrices,
+                          const char* detector_file, const int16_t* inactive, int n_inactive,
+                          float z_left, int n_planes, float sigma_z, float sigma_dl,
+                          float length, int sens_mode, const float* sensitivity_3d,
+                          const int16_t* lors, const float* zzd, int64_t n_events,
+                          int n_iterations, int64_t* out_n_infos, int32_t* out_depth,
+                          float* out_segments, int64_t* out_begin, int64_t* out_end,
+                          int16_t* out_pixels, float* out_positions, float* out_weights,
+                          float* out_sensitivity, int64_t* out_n_kept, double* out_frames,
+                          float* out_rho) {
+  try {
+#if _OPENMP
+    omp_set_num_threads(1);
+#endif
+    Reconstruction::Geometry* soa = nullptr;
+    Geometry::Grid* grid = nullptr;
+    Scanner2D barrel = PET2D::Barrel::ScannerBuilder<Scanner2D>::build_single_ring(
+        F(1), 4, F(0.1), F(0.1));
+    if (mode == 0) {
+      grid = new Geometry::Grid(n_columns, n_rows, pixel_size,
+                                Point2D(lower_left_x, lower_left_y));
+      Geometry geometry(n_detectors, *grid);
+      for (S first = 0; first < n_detectors; ++first) {
+        for (S second = 0; second <= first; ++second) {
+          LOR lor(first, second);
+          const auto i = lor.index();
+          auto& lg = geometry[lor];
+          lg.lor = lor;
+          lg.segment = PET2D::LineSegment<F>(
+              Point2D(segments[4 * i], segments[4 * i + 1]),
+              Point2D(segments[4 * i + 2], segments[4 * i + 3]));
+          lg.width = widths[i];
+          lg.pixel_infos.clear();
+          for (int64_t k = info_begin[i]; k < info_end[i]; ++k) {
+            Geometry::PixelInfo pi;
+            pi.pixel = Pixel(pixels[2 * k], pixels[2 * k + 1]);
+            pi.t = positions[k];
+            pi.distance = 0;
+            pi.fill = 0;
+            pi.weight = 0;
+            lg.pixel_infos.push_back(pi);
+          }
+        }
+      }
+      soa = new Reconstruction::Geometry(geometry, (size_t)n_matrices);
+      for (int p = 0; p < n_matrices; ++p)
+        soa->load_weights_from_matrix_file<Hit>(matrix_paths[p], (size_t)p);
+    } else {
+      util::ibstream in_matrix(matrix_paths[0]);
+      if (!in_matrix.is_open()) return -1;
+      Matrix matrix(in_matrix);
+      std::ifstream in_dets(detector_file);
+      barrel = PET2D::Barrel::ScannerBuilder<Scanner2D>::deserialize(in_dets);
+      grid = new Geometry::Grid(matrix.n_pixels_in_row(), matrix.n_pixels_in_row(),
+                                pixel_size);
+      soa = new Reconstruction::Geometry(matrix, barrel.detector_centers(), *grid,
+                                         n_inactive ? inactive : nullptr,
+                                         (size_t)n_inactive);
+      // The constructor sizes its arrays by the matrix's element count but
+      // fills one slot per element of an active LOR (geometry_soa.h:81-115);
+      // the slots of skipped elements keep default-initialised pixels
+      // (Pixel() = default under new Pixel[n]) that normalize_geometry_weights
+      // (reconstruction.h:253-272) then uses as sensitivity indices. Those
+      // slots are given pixel (0, 0) and weight 0 here so that the run is
+      // defined; no LOR's range reaches them.
+      size_t filled = 0;
+      for (size_t l = 0; l < soa->n_lors; ++l)
+        filled = std::max(filled, (size_t)soa->lor_pixel_info_end[l]);
+      for (size_t i = filled; i < soa->n_pixel_infos; ++i) {
+        soa->pixels[i] = Pixel(0, 0);
+        soa->pixel_positions[i] = 0;
+        soa->pixel_weights[i] = 0;
+      }
+    }
+    *out_n_infos = (int64_t)soa->n_pixel_infos;
+    if (mode == 1 && out_segments) {
+      for (size_t l = 0; l < soa->n_lors; ++l) {
+        out_segments[4 * l] = soa->lor_line_segments[l].start.x;
+        out_segments[4 * l + 1] = soa->lor_line_segments[l].start.y;
+        out_segments[4 * l + 2] = soa->lor_line_segments[l].end.x;
+        out_segments[4 * l + 3] = soa->lor_line_segments[l].end.y;
+        out_begin[l] = (int64_t)soa->lor_pixel_info_begin[l];
+        out_end[l] = (int64_t)soa->lor_pixel_info_end[l];
+      }
+      for (size_t i = 0; i < soa->n_pixel_infos; ++i) {
+        out_pixels[2 * i] = soa->pixels[i].x;
+        out_pixels[2 * i + 1] = soa->pixels[i].y;
+        out_positions[i] = soa->pixel_positions[i];
+      }
+    }
+    if (!out_weights) {  // sizes only
+      delete soa;
+      delete grid;
+      return 0;
+    }
+    Scanner scanner(barrel, length);
+    scanner.set_sigmas(sigma_z, sigma_dl);
+    Reconstruction::Grid grid3d(*grid, z_left, (S)n_planes);
+    Reconstruction rec(scanner, grid3d, *soa, sensitivity_3d != nullptr);
+    const size_t np = (size_t)grid->n_columns * grid->n_rows;
+    const int depth = rec.sensitivity.depth;
+    *out_depth = depth;
+    if (sens_mode == 2 && sensitivity_3d)
+      std::memcpy(rec.sensitivity.data, sensitivity_3d, sizeof(F) * np * depth);
+    else if (sens_mode == 1)
+      rec.set_sensitivity_to_one();
+    else
+      rec.calculate_sensitivity();
+    rec.normalize_geometry_weights();
+    std::memcpy(out_weights, soa->pixel_weights,
+                sizeof(F) * soa->n_pixel_infos * soa->n_planes_half);
+    std::memcpy(out_sensitivity, rec.sensitivity.data, sizeof(F) * np * depth);
+    std::stringstream text;
+    text.precision(9);
+    for (int64_t e = 0; e < n_events; ++e)
+      text << lors[2 * e] << ' ' << lors[2 * e + 1] << ' ' << zzd[3 * e] << ' '
+           << zzd[3 * e + 1] << ' ' << zzd[3 * e + 2] << '\n';
+    rec << text;
+    *out_n_kept = rec.n_events();
+    for (int i = 0; i < rec.n_events(); ++i) {
+      const auto ev = rec.frame_event(i);
+      double* o = out_frames + 10 * (size_t)i;
+      o[0] = ev.lor.first;
+      o[1] = ev.lor.second;
+      o[2] = ev.up;
+      o[3] = ev.right;
+      o[4] = ev.tan;
+      o[5] = ev.sec;
+      o[6] = (double)ev.pixel_info_begin;
+      o[7] = (double)ev.pixel_info_end;
+      o[8] = ev.plane_begin;
+      o[9] = ev.plane_end;
+    }
+    const size_t n_vox = np * (size_t)n_planes;
+    for (int it = 0; it < n_iterations; ++it) {
+      rec();
+      std::memcpy(out_rho + (size_t)it * n_vox, rec.rho.data, sizeof(F) * n_vox);
+    }
+    delete soa;
+    delete grid;
     return 0;
   } catch (...) {
     return -1;

@@ -265,6 +265,11 @@ def _hyb():
                                              vp, vp]
         o.oracle_hybrid_mlem_f64.argtypes = [G, d, d, d, vp, vp, C.c_int, vp, i64, C.c_int,
                                              vp, vp]
+        for t in ("f32", "f64"):
+            getattr(o, f"oracle_hybrid_mlem_planes_{t}").argtypes = [
+                G, d, d, d, vp, C.c_int, vp, C.c_int, vp, i64, C.c_int, vp, vp]
+            getattr(o, f"oracle_hybrid_plane_weights_{t}").argtypes = [G, vp, C.c_int, C.c_int,
+                                                                       vp, vp]
         _h = o
     return _h
 
@@ -380,4 +385,109 @@ def ref_hybrid_read_geometry(path):
     keys = ["segments", "widths", "info_begin", "info_end", "pixels", "positions", "weights"]
     if r.ref_hybrid_read_geometry(*args, *[P(out[k]) for k in keys]) != 0:
         raise RuntimeError("reference geometry reader failed")
+    return out
+
+
+def hybrid_plane_weights(g, raw, sens_mode=0, sensitivity=None, f64=False):
+    """(weights, sensitivity) of per-plane raw weights (planes, n_pixel_infos):
+    the volume (planes, rows, columns) summed (0), one (1) or given (2), each
+    plane's weights divided by it."""
+    t = np.float64 if f64 else np.float32
+    raw = np.ascontiguousarray(raw, t)
+    planes = raw.shape[0]
+    w = np.zeros_like(raw)
+    sv = (np.zeros((planes, g.n_rows, g.n_columns), t) if sensitivity is None else
+          np.array(sensitivity, t).reshape(planes, g.n_rows, g.n_columns))
+    og = ohybrid(g)
+    fn = getattr(_hyb(), f"oracle_hybrid_plane_weights_{'f64' if f64 else 'f32'}")
+    fn(C.byref(og), P(raw), planes, int(sens_mode), P(w), P(sv))
+    return w, sv
+
+
+def hybrid_mlem_planes(g, sigma_z, sigma_dl, length, weights, sensitivity, events, n_iter,
+                       rho0=None, f64=False):
+    """Images after every iteration with per-plane weights (D, n_pixel_infos)
+    and a sensitivity volume (D, rows, columns), voxel plane iz reading plane
+    |iz - D| of both."""
+    t = np.float64 if f64 else np.float32
+    weights = np.ascontiguousarray(weights, t)
+    sensitivity = np.ascontiguousarray(sensitivity, t)
+    shape = (g.n_planes, g.n_rows, g.n_columns)
+    rho = np.ones(shape, t) if rho0 is None else np.ascontiguousarray(rho0, t)
+    out = np.zeros((n_iter,) + shape, t)
+    og = ohybrid(g)
+    events = np.ascontiguousarray(events, HYBRID_EVENT)
+    fn = getattr(_hyb(), f"oracle_hybrid_mlem_planes_{'f64' if f64 else 'f32'}")
+    fn(C.byref(og), sigma_z, sigma_dl, length, P(weights), weights.shape[0], P(sensitivity),
+       sensitivity.shape[0], P(events), len(events), n_iter, P(rho), P(out))
+    return out
+
+
+def ref_hybrid_system(mode, g, sigma_z, sigma_dl, length, lors, zzd, n_iter,
+                      matrix_paths, detector_file=None, inactive=(), sens_mode=0,
+                      pixel_size=None, n_planes=None, z_left=None):
+    """The reference's --system paths through oracle/_ref/libref_hybrid.so
+    (ref_hybrid_run_system): mode 0 with geometry g and one matrix file per
+    weight plane, mode 1 from one matrix file and a scanner description file
+    (g ignored; pixel_size, n_planes, z_left give the grid).  Returns a dict
+    with the geometry arrays (mode 1), weights (planes, n_pixel_infos),
+    sensitivity, frames and the images after every iteration."""
+    r = C.CDLL(REF_HYBRID_SO)
+    vp, i64, f, ci = C.c_void_p, C.c_int64, C.c_float, C.c_int
+    r.ref_hybrid_run_system.argtypes = (
+        [ci, ci, ci, ci, f, f, f] + [vp] * 6 + [vp, ci, C.c_char_p, vp, ci, f, ci, f, f, f, ci,
+                                                vp, vp, vp, i64, ci] + [vp] * 12)
+    r.ref_hybrid_run_system.restype = ci
+    paths = (C.c_char_p * len(matrix_paths))(*[os.fsencode(p) for p in matrix_paths])
+    lors = np.ascontiguousarray(lors, np.int16)
+    zzd = np.ascontiguousarray(zzd, np.float32)
+    ina = np.ascontiguousarray(inactive, np.int16)
+    if mode == 0:
+        geo = [g.n_detectors, g.n_columns, g.n_rows, g.pixel_size, g.lower_left[0],
+               g.lower_left[1]]
+        arrays = [P(g.segments), P(g.widths), P(g.info_begin), P(g.info_end), P(g.pixels),
+                  P(g.positions)]
+        n_planes, z_left, nd, ncol = g.n_planes, g.z_left, g.n_detectors, g.n_columns
+    else:
+        geo = [0, 0, 0, pixel_size, 0.0, 0.0]
+        arrays = [None] * 6
+        nd = None
+    n_infos, depth = C.c_int64(), C.c_int32()
+    common = [float(z_left), int(n_planes), sigma_z, sigma_dl, length, int(sens_mode), None,
+              P(lors), P(zzd), len(lors), n_iter]
+    tail = [C.byref(n_infos), C.byref(depth)] + [None] * 10
+    rc = r.ref_hybrid_run_system(mode, *geo, *arrays, paths, len(matrix_paths),
+                                 None if detector_file is None else os.fsencode(detector_file),
+                                 P(ina) if len(ina) else None, len(ina), *common, *tail)
+    if rc != 0:
+        raise RuntimeError("reference hybrid harness failed (sizes)")
+    ni = n_infos.value
+    if mode == 1:
+        with open(matrix_paths[0], "rb") as fh:
+            hdr = np.frombuffer(fh.read(16), "<u4")
+        ncol, nd = int(hdr[1]), int(hdr[3])
+    n_lors = nd * (nd + 1) // 2
+    planes = len(matrix_paths) if mode == 0 else 1
+    out = {"segments": np.zeros((n_lors, 4), np.float32),
+           "begin": np.zeros(n_lors, np.int64), "end": np.zeros(n_lors, np.int64),
+           "pixels": np.zeros((max(ni, 1), 2), np.int16),
+           "positions": np.zeros(max(ni, 1), np.float32),
+           "weights": np.zeros((planes, max(ni, 1)), np.float32)}
+    np_ = ncol * ncol
+    sens = np.zeros(np_ * max(planes, int(n_planes) // 2), np.float32)
+    frames = np.zeros((len(lors), 10), np.float64)
+    rho = np.zeros((n_iter, int(n_planes), ncol, ncol), np.float32)
+    kept = C.c_int64()
+    tail = [C.byref(n_infos), C.byref(depth), P(out["segments"]), P(out["begin"]),
+            P(out["end"]), P(out["pixels"]), P(out["positions"]), P(out["weights"]), P(sens),
+            C.byref(kept), P(frames), P(rho)]
+    rc = r.ref_hybrid_run_system(mode, *geo, *arrays, paths, len(matrix_paths),
+                                 None if detector_file is None else os.fsencode(detector_file),
+                                 P(ina) if len(ina) else None, len(ina), *common, *tail)
+    if rc != 0:
+        raise RuntimeError("reference hybrid harness failed")
+    out["n_pixel_infos"] = ni
+    out["sensitivity"] = sens[:np_ * depth.value].reshape(depth.value, ncol, ncol)
+    out["frames"] = frames[:kept.value]
+    out["rho"] = rho
     return out
