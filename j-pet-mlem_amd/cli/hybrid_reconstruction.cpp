@@ -1,10 +1,21 @@
 // 3d_hybrid_reconstruction: the 3D hybrid list-mode reconstruction of a barrel
 // scanner on an AMD Instinct GPU (libpetstrip_hip.so, include/
-// petstrip_hybrid.h), with the reference tool's command line and files for
-// its --geometry path (grey277/j-pet-mlem src/3d/hybrid/reconstruction_cmd.cpp:
-// 147-242,281-365,456-640):
-//   3d_hybrid_reconstruction --geometry geom.bin [options] response...
-// The geometry file is 2d_barrel_geometry's binary output (PETg); responses
+// petstrip_hybrid.h), with the reference tool's command line and files
+// (grey277/j-pet-mlem src/3d/hybrid/reconstruction_cmd.cpp:147-640):
+//   3d_hybrid_reconstruction --geometry geom.bin [--system m.bin[,m2.bin...]]
+//                            [options] response...
+//   3d_hybrid_reconstruction --system m.bin --detector-file dets.txt
+//                            [--inactive i,j,...] [options] response...
+// With --geometry the LOR-pixel weights are the geometry's Gaussian weights
+// (calculate_weight), or with --system those of one system matrix or of one
+// per half plane (the multi-plane geometry, reconstruction_cmd.cpp:281-365;
+// without --system the reference indexes its empty matrix list,
+// reconstruction_cmd.cpp:350-357, where this build takes the geometry's
+// weights).  With --system alone the geometry is the matrix's (full form,
+// PETP or TOFP) with the detector centres of --detector-file, the matrix's
+// .cfg beside it loaded as options, --n-pixels the matrix's pixels per row
+// and --s-pixel required (reconstruction_cmd.cpp:203-225,367-455).  The
+// geometry file is 2d_barrel_geometry's binary output (PETg); responses
 // are text lines "d1 d2 z_up z_dn dl" (3d_tool_convert's output) or binary
 // records (int16 d1, d2, float z_up, z_dn, dl).  -i blocks of -I iterations
 // run on the GPU; with -o base the naive image (base + .nrrd, int voxel
@@ -14,9 +25,11 @@
 // runs).  --sensitivity reads an external volume of n_planes / 2 planes of
 // raw floats (the reference's binary VoxelMap, voxel_map.h:175-178); the
 // reference's text reader assigns to copies of the voxels and leaves the
-// volume unset (voxel_map.h:153-158), so a .txt volume is refused.  The
-// reference's per-plane system matrices (--system) and inactive detectors are
-// not supported.
+// volume unset (voxel_map.h:153-158), so a .txt volume is refused.  With
+// inactive detectors the reference sizes its pixel-info arrays by the
+// matrix's element count and reads the slots of skipped elements, which it
+// never fills, in calculate_sensitivity (geometry_soa.h:81-115,
+// reconstruction.h:240-251); here the geometry holds the filled slots only.
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -24,6 +37,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -39,7 +53,7 @@ using namespace strip_cli;
 
 const std::vector<OptSpec> kOptions = {
     {"config", 'c', true, "", true, kDontSave, "load config file"},
-    {"system", 's', true, "", true, kSave, "system matrix file (not supported)"},
+    {"system", 's', true, "", true, kSave, "system matrix file"},
     {"geometry", 0, true, "", true, kSave, "geometry information file"},
     {"n-planes", 0, true, "0", true, kSave, "number of voxels in z direction"},
     {"z-left", 0, true, "0", true, kSave, "left extent in z direction"},
@@ -48,7 +62,7 @@ const std::vector<OptSpec> kOptions = {
     {"s-dl", 0, true, "0.060", true, kAlwaysSave, "TOF sigma axis"},
     {"sens-to-one", 0, false, "", true, kSave, "sets sensitivity to one"},
     {"sensitivity", 0, true, "", true, kSave, "external 3D sensitivity"},
-    {"inactive", 0, true, "", true, kSave, "list of inactive detectors (not supported)"},
+    {"inactive", 0, true, "", true, kSave, "list of inactive detectors"},
     {"crop", 0, true, "0", true, kSave, "numer of pixel to crop the output image"},
     {"crop-x", 0, true, "0", true, kSave, "crop origin pixel x"},
     {"crop-y", 0, true, "0", true, kSave, "crop origin pixel y"},
@@ -79,8 +93,9 @@ const std::vector<OptSpec> kOptions = {
     {"iterations", 'I', true, "1", true, kSave, "number of iterations (per block)"},
     {"z-position", 'z', true, "0", true, kSave, "position of the z plane"},
     {"rho", 0, true, "", true, kSave, "start rho (eg. existing iteration)"},
-    {"detector-file", 0, true, "", true, kSave, "detector description file (unused)"},
-    {"detector-file-sym", 0, true, "", true, kSave, "detector symmetries file (unused)"},
+    {"detector-file", 0, true, "", true, kSave, "detector description file"},
+    {"detector-file-sym", 0, true, "", true, kSave,
+     "detector symmetries description file (unused)"},
     {"help", '?', false, "", false, kDontSave, "print this message"},
 };
 
@@ -153,6 +168,73 @@ Geometry read_geometry(const std::string& fn) {
     total += (int64_t)n;
     g.end[li] = total;
   }
+  return g;
+}
+
+void check(int rc);
+
+struct Matrix {
+  petstrip_system_matrix* m = nullptr;
+  int32_t n_pixels_in_row = 0, n_detectors = 0;
+  int64_t n_emissions = 0, n_elements = 0;
+  explicit Matrix(const std::string& fn) {
+    if (petstrip_system_matrix_read(fn.c_str(), &m) != PETSTRIP_OK)
+      throw std::runtime_error(petstrip_last_error());
+    petstrip_system_matrix_info(m, &n_pixels_in_row, &n_detectors, &n_emissions, &n_elements);
+  }
+  ~Matrix() { petstrip_system_matrix_free(m); }
+  Matrix(const Matrix&) = delete;
+  Matrix& operator=(const Matrix&) = delete;
+};
+
+petstrip_hybrid_geometry c_geometry(const Geometry& g, int n_planes, double z_left) {
+  return petstrip_hybrid_geometry{g.n_detectors, g.n_columns, g.n_rows, n_planes,
+                                  g.pixel_size, g.llx, g.lly, z_left,
+                                  (int64_t)g.positions.size(), g.segments.data(),
+                                  g.widths.data(), g.begin.data(), g.end.data(),
+                                  g.pixels.data(), g.positions.data(), g.weights.data()};
+}
+
+std::vector<std::string> split(const std::string& s, char sep) {
+  std::vector<std::string> out;
+  std::istringstream in(s);
+  std::string item;
+  while (std::getline(in, item, sep)) out.push_back(item);
+  return out;
+}
+
+// The geometry of a matrix and the detector centres on the matrix's centred
+// grid of pixel_size (GeometrySOA(matrix, centres, grid, inactive),
+// geometry_soa.h:71-182; PixelGrid(n, n, s) has its lower left at
+// (-s n / 2, -s n / 2) in float, pixel_grid.h:33-37).
+Geometry geometry_from_matrix(const Matrix& mx, const std::vector<float>& centres,
+                              double s_pixel, const std::vector<int16_t>& inactive) {
+  Geometry g;
+  g.n_detectors = mx.n_detectors;
+  g.n_columns = g.n_rows = mx.n_pixels_in_row;
+  g.pixel_size = (float)s_pixel;
+  g.llx = g.lly = -g.pixel_size * (float)(int16_t)mx.n_pixels_in_row / 2;
+  const int16_t* ina = inactive.empty() ? nullptr : inactive.data();
+  int64_t ni = 0;
+  check(petstrip_hybrid_geometry_from_matrix(mx.m, centres.data(), g.n_columns, g.n_rows,
+                                             g.pixel_size, g.llx, g.lly, ina,
+                                             (int32_t)inactive.size(), nullptr, nullptr,
+                                             nullptr, nullptr, nullptr, nullptr, nullptr, &ni));
+  const int64_t n_lors = (int64_t)g.n_detectors * (g.n_detectors + 1) / 2;
+  g.segments.resize(4 * n_lors);
+  g.widths.resize(n_lors);
+  g.begin.resize(n_lors);
+  g.end.resize(n_lors);
+  g.pixels.resize(2 * std::max<int64_t>(ni, 1));
+  g.positions.resize(std::max<int64_t>(ni, 1));
+  g.weights.resize(std::max<int64_t>(ni, 1));
+  check(petstrip_hybrid_geometry_from_matrix(
+      mx.m, centres.data(), g.n_columns, g.n_rows, g.pixel_size, g.llx, g.lly, ina,
+      (int32_t)inactive.size(), g.segments.data(), g.widths.data(), g.begin.data(),
+      g.end.data(), g.pixels.data(), g.positions.data(), g.weights.data(), &ni));
+  g.pixels.resize(2 * ni);
+  g.positions.resize(ni);
+  g.weights.resize(ni);
   return g;
 }
 
@@ -244,32 +326,77 @@ int run(int argc, char** argv) {
           "note: All length options below should be expressed in meters.");
     return 0;
   }
-  for (const char* o : {"system", "inactive"})
-    if (a.exist(o))
-      throw std::runtime_error(std::string("--") + o + " is not supported by this build");
-  if (!a.exist("geometry"))
+  const bool with_geometry = a.exist("geometry");
+  if (!with_geometry && !a.exist("system"))
     throw std::runtime_error("either --geometry or --system matrix must be specified");
-  const Geometry g = read_geometry(a.get("geometry"));
   const int verbose = a.flag("verbose");
-  // --n-pixels and --s-pixel from the geometry unless given, then the planes
-  // (reconstruction_cmd.cpp:287-315)
-  const int n_pixels = a.exist("n-pixels") ? (int)to_long(a.get("n-pixels"))
-                                           : std::max(g.n_columns, g.n_rows);
-  // the reference passes the geometry's pixel size through the option parser
-  // as text with six significant digits
+  Geometry g;
+  std::vector<std::unique_ptr<Matrix>> matrices;
+  int n_pixels;
   double s_pixel;
-  if (a.exist("s-pixel")) {
-    s_pixel = to_double(a.get("s-pixel"));
+  if (with_geometry) {
+    g = read_geometry(a.get("geometry"));
+    // --n-pixels and --s-pixel from the geometry unless given, then the
+    // planes (reconstruction_cmd.cpp:287-315)
+    n_pixels = a.exist("n-pixels") ? (int)to_long(a.get("n-pixels"))
+                                   : std::max(g.n_columns, g.n_rows);
+    // the reference passes the geometry's pixel size through the option
+    // parser as text with six significant digits
+    if (a.exist("s-pixel")) {
+      s_pixel = to_double(a.get("s-pixel"));
+    } else {
+      std::ostringstream ps;
+      ps << g.pixel_size;
+      s_pixel = to_double(ps.str());
+    }
+    if (a.exist("system"))
+      for (const auto& fn : split(a.get("system"), ','))
+        matrices.emplace_back(new Matrix(fn));
   } else {
-    std::ostringstream ps;
-    ps << g.pixel_size;
-    s_pixel = to_double(ps.str());
+    // the matrix, then its .cfg as options (reconstruction_cmd.cpp:203-215)
+    const std::string fn = a.get("system");
+    matrices.emplace_back(new Matrix(fn));
+    const std::string cfg = strip_io::without_ext(fn) + ".cfg";
+    if (std::ifstream(cfg).is_open()) load_config(a, cfg, a.flag("verbose"));
+    const Matrix& mx = *matrices[0];
+    // the triangular forms are refused by the reader
+    n_pixels = a.exist("n-pixels") ? (int)to_long(a.get("n-pixels")) : mx.n_pixels_in_row;
+    if (!a.exist("s-pixel"))
+      throw std::runtime_error("--s-pixel is required with --system alone (or in the "
+                               "matrix's .cfg)");
+    s_pixel = to_double(a.get("s-pixel"));
+    // the detector centres of the scanner description file
+    // (reconstruction_cmd.cpp:127-146,409-411)
+    const std::string df = a.exist("detector-file") ? a.get("detector-file") : std::string();
+    std::vector<float> centres;
+    int32_t nd = 0;
+    if (petstrip_read_detector_centres(df.c_str(), nullptr, 0, &nd) != PETSTRIP_OK) {
+      std::cerr << "cannot open detector description file `" << df << "'\n";
+      nd = 0;
+    } else {
+      centres.resize(2 * (size_t)std::max(nd, 1));
+      check(petstrip_read_detector_centres(df.c_str(), centres.data(), nd, &nd));
+    }
+    if (nd != mx.n_detectors) throw std::runtime_error("n_detectors mismatch");
+    std::vector<int16_t> inactive;
+    if (a.exist("inactive"))
+      for (const auto& d : split(a.get("inactive"), ',')) {
+        const long v = to_long(d);
+        if (v < 0 || v >= nd) throw std::runtime_error("inactive detector out of range: " + d);
+        inactive.push_back((int16_t)v);
+      }
+    g = geometry_from_matrix(mx, centres, s_pixel, inactive);
   }
   const int n_planes = a.exist("n-planes") ? (int)to_long(a.get("n-planes")) : n_pixels;
   const double z_left = a.exist("z-left") ? to_double(a.get("z-left")) : -n_planes * s_pixel / 2;
+  if (with_geometry && !matrices.empty() && matrices.size() != 1 &&
+      matrices.size() != (size_t)(n_planes / 2))
+    throw std::runtime_error("you must specify either one or half no. planes matrices");
   if (verbose)
-    std::cerr << "3D hybrid reconstruction w/geometry:\n    detectors = " << g.n_detectors
-              << "\n  planes = " << n_planes << ", z-left = " << z_left << std::endl;
+    std::cerr << "3D hybrid reconstruction w/" << (with_geometry ? "geometry" : "system matrix")
+              << ":\n    detectors = " << g.n_detectors << "\n   pixel grid = " << g.n_columns
+              << " x " << g.n_rows << " / " << g.pixel_size << "\n  planes = " << n_planes
+              << ", z-left = " << z_left << std::endl;
 
   // reconstruction_cmd.cpp:505-519: an external volume takes precedence over
   // --sens-to-one and normalises the geometry weights; read before any
@@ -291,22 +418,40 @@ int run(int argc, char** argv) {
                                std::to_string(g.n_rows) + " floats: " + sn);
   }
 
-  petstrip_hybrid_geometry hg{g.n_detectors, g.n_columns, g.n_rows, n_planes, g.pixel_size,
-                              g.llx, g.lly, z_left, (int64_t)g.positions.size(),
-                              g.segments.data(), g.widths.data(), g.begin.data(),
-                              g.end.data(), g.pixels.data(), g.positions.data(),
-                              g.weights.data()};
+  petstrip_hybrid_geometry hg = c_geometry(g, n_planes, z_left);
+  // with --geometry and matrices: each matrix's weights on the geometry's
+  // pixel infos (load_weights_from_matrix_file), one plane per matrix
+  std::vector<float> raw;
+  if (with_geometry && !matrices.empty()) {
+    const size_t ni = g.positions.size();
+    raw.resize(matrices.size() * ni);
+    for (size_t p = 0; p < matrices.size(); ++p) {
+      if (verbose) std::cerr << "system matrix = " << split(a.get("system"), ',')[p] << std::endl;
+      check(petstrip_system_matrix_weights(matrices[p]->m, &hg, raw.data() + p * ni));
+    }
+  }
+  // without --geometry the weights are the matrix's; with --geometry and no
+  // matrix, the Gaussian weights computed from the geometry
+  const int32_t flags = (a.exist("sens-to-one") ? PETSTRIP_HYBRID_SENSITIVITY_ONE : 0) |
+                        (with_geometry ? 0 : PETSTRIP_HYBRID_GEOMETRY_WEIGHTS);
   petstrip_hybrid* h = nullptr;
   check(petstrip_hybrid_create(&hg, to_double(a.get("s-z")), to_double(a.get("s-dl")),
-                               to_double(a.get("length")),
-                               a.exist("sens-to-one") ? PETSTRIP_HYBRID_SENSITIVITY_ONE : 0,
+                               to_double(a.get("length")), flags,
                                (int32_t)to_long(a.get("cuda-device")), &h));
   struct Guard {
     petstrip_hybrid* h;
     ~Guard() { petstrip_hybrid_destroy(h); }
   } guard{h};
 
-  if (!vol.empty()) check(petstrip_hybrid_set_sensitivity_3d(h, vol.data(), n_planes / 2));
+  // the sensitivity: the external volume, one, or calculated
+  // (reconstruction_cmd.cpp:505-521)
+  if (!raw.empty()) {
+    const int mode = !vol.empty() ? 2 : a.exist("sens-to-one") ? 1 : 0;
+    check(petstrip_hybrid_set_plane_weights(h, raw.data(), (int32_t)matrices.size(), mode,
+                                            vol.empty() ? nullptr : vol.data()));
+  } else if (!vol.empty()) {
+    check(petstrip_hybrid_set_sensitivity_3d(h, vol.data(), n_planes / 2));
+  }
 
   const size_t nv = (size_t)g.n_columns * g.n_rows * n_planes;
   std::vector<float> rho(nv, 1.0f);

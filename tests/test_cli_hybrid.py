@@ -1,16 +1,24 @@
 """3d_hybrid_reconstruction (j-pet-mlem_amd/cli/hybrid_reconstruction.cpp):
-the reference's --geometry command line (src/3d/hybrid/
-reconstruction_cmd.cpp) over the hybrid C ABI.  The option checks run on the
-CPU; the run itself on the GPU, against the Python host side on the same
-geometry file and responses."""
+the reference's command line (src/3d/hybrid/reconstruction_cmd.cpp), its
+--geometry and --system paths, over the hybrid C ABI.  The option checks run
+on the CPU; the runs themselves on the GPU, against the Python host side on
+the same geometry file and responses, and against the images the
+reference's own code made of the same system matrices
+(tests/golden/hybrid_system.npz)."""
 import os
+import shutil
 import subprocess
+import sys
 
 import numpy as np
 import pytest
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
 TOOL = os.path.join(ROOT, "j-pet-mlem_amd", "cli", "3d_hybrid_reconstruction")
+SYS = os.path.join(HERE, "golden", "hybrid_system")
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_hybrid_system_golden as ms  # noqa: E402
 
 
 def run(args, cwd):
@@ -23,9 +31,39 @@ def test_hybrid_cli_option_errors(tmp_path):
     r = run(["resp.txt"], tmp_path)
     assert r.returncode != 0 and "--geometry" in r.stderr
     r = run(["--system", "m.bin", "resp.txt"], tmp_path)
-    assert r.returncode != 0 and "not supported" in r.stderr
+    assert r.returncode != 0 and "cannot open system matrix" in r.stderr
     r = run(["--geometry", "missing.bin", "resp.txt"], tmp_path)
     assert r.returncode != 0 and "cannot open geometry" in r.stderr
+
+
+def test_hybrid_cli_system_errors(pkg, tmp_path):
+    """--system: a triangular matrix, no pixel size, a missing or mismatched
+    detector file, an inactive detector out of range and a matrix count
+    other than one or n_planes / 2 are refused before any device work."""
+    m0 = os.path.join(SYS, "matrix_0.bin")
+    det = os.path.join(SYS, "detectors.txt")
+    (tmp_path / "resp.txt").write_text("")
+    (tmp_path / "tri.bin").write_bytes(b"PETp" + np.array([12, 100, 24], "<u4").tobytes())
+    r = run(["--system", "tri.bin", "resp.txt"], tmp_path)
+    assert r.returncode != 0 and "full form" in r.stderr
+    r = run(["--system", m0, "resp.txt"], tmp_path)
+    assert r.returncode != 0 and "--s-pixel is required" in r.stderr
+    r = run(["--system", m0, "-p", "0.01", "resp.txt"], tmp_path)
+    assert r.returncode != 0 and "cannot open detector description" in r.stderr
+    assert "n_detectors mismatch" in r.stderr
+    (tmp_path / "few.txt").write_text("".join(open(det).readlines()[:20]))
+    r = run(["--system", m0, "-p", "0.01", "--detector-file", "few.txt", "resp.txt"], tmp_path)
+    assert r.returncode != 0 and "n_detectors mismatch" in r.stderr
+    r = run(["--system", m0, "-p", "0.01", "--detector-file", det, "--inactive", "3,24",
+             "resp.txt"], tmp_path)
+    assert r.returncode != 0 and "inactive detector out of range" in r.stderr
+    hg = pkg.hybrid_geometry
+    g, _, _ = ms.inputs(pkg)
+    hg.write_geometry(g, str(tmp_path / "geom.bin"))
+    two = ",".join(os.path.join(SYS, f"matrix_{p}.bin") for p in range(2))
+    r = run(["--geometry", "geom.bin", "--system", two, "--n-planes", "12", "resp.txt"],
+            tmp_path)
+    assert r.returncode != 0 and "either one or half no. planes" in r.stderr
 
 
 def test_hybrid_cli_sensitivity_errors(pkg, tmp_path):
@@ -113,4 +151,55 @@ def test_hybrid_cli_sensitivity_matches_library(gpu_pkg, tmp_path):
     got = np.fromfile(tmp_path / "out_2.bin", np.float32).reshape(want.shape)
     rel = np.linalg.norm(got - want) / np.linalg.norm(want)
     print(f"  rel L2 command line vs library: {rel:.2e}", flush=True)
+    assert rel < 1e-5
+
+
+def write_binary_responses(path, lors, zzd):
+    """Responses as binary records (int16 d1, d2, float z_up, z_dn, dl;
+    3d/hybrid/scanner.h:80-88): the floats as they are, without a text
+    round trip."""
+    rec = np.zeros(len(lors), dtype=[("l", "<i2", 2), ("v", "<f4", 3)])
+    rec["l"], rec["v"] = lors, zzd
+    rec.tofile(path)
+
+
+SYSTEM_CASES = {
+    # name: (arguments beyond the sigmas, length and grid)
+    "am": ["--geometry", "geom.bin", "--system",
+           ",".join(os.path.join(SYS, f"matrix_{p}.bin") for p in range(ms.N_MATRICES))],
+    "a1": ["--geometry", "geom.bin", "--system", os.path.join(SYS, "matrix_0.bin")],
+    "b": ["--system", "m.bin", "--detector-file", os.path.join(SYS, "detectors.txt")],
+    "bi": ["--system", "m.bin", "--detector-file", os.path.join(SYS, "detectors.txt"),
+           "--inactive", "3,10", "--sens-to-one"],
+    "bt": ["--system", "tof.bin", "--detector-file", os.path.join(SYS, "detectors.txt")],
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(SYSTEM_CASES))
+def test_hybrid_cli_system_vs_reference(gpu_pkg, tmp_path, name):
+    """The --system paths of the command line against the reference's image
+    after two iterations on the same matrices and responses; with --system
+    alone the pixel size, planes and z-left come from the matrix's .cfg."""
+    pkg = gpu_pkg
+    F = np.load(os.path.join(HERE, "golden", "hybrid_system.npz"))
+    g, lors, zzd = ms.inputs(pkg)
+    pkg.hybrid_geometry.write_geometry(g, str(tmp_path / "geom.bin"))
+    write_binary_responses(tmp_path / "resp.bin", lors, zzd)
+    shutil.copy(os.path.join(SYS, "matrix_0.bin"), tmp_path / "m.bin")
+    shutil.copy(os.path.join(SYS, "matrix_tof.bin"), tmp_path / "tof.bin")
+    for stem in ("m", "tof"):
+        (tmp_path / f"{stem}.cfg").write_text(
+            "# the matrix's options\n--s-pixel=0.01\n--n-planes=12\n--z-left=-0.06\n"
+            "--n-detectors=24\n")
+    grid = [] if name.startswith("b") else ["--n-planes", "12", "--z-left", "-0.06"]
+    print(f"case {name}: {' '.join(SYSTEM_CASES[name][:2])} ..., 1500 binary responses, "
+          f"2 iterations", flush=True)
+    r = run([*SYSTEM_CASES[name], *grid, "--s-z", str(ms.SZ), "--s-dl", str(ms.SDL), "-l",
+             str(ms.LENGTH), "-i", "1", "-I", "2", "-o", "out.bin", "resp.bin"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    want = F[f"{name}_rho"][1]
+    got = np.fromfile(tmp_path / "out_2.bin", np.float32).reshape(want.shape)
+    rel = np.linalg.norm(got.astype(np.float64) - want) / np.linalg.norm(want)
+    print(f"  rel L2 command line vs reference: {rel:.2e}", flush=True)
     assert rel < 1e-5
