@@ -254,6 +254,11 @@ constexpr int kFixed64Bits = 40;
 // 1e-6 of itself, and values up to 2^10 times the typical one (cold pixels,
 // small denominators) still take the single conversion.
 constexpr int kGradTypicalBits = 20;
+// The FP32 MLEM iteration sums the same values K / lambda (its image is
+// rho times the sums): a typical value at 2^24 units is rounded to about
+// 3e-8 of itself, as fine as the K rho / lambda values in 2^-31 units, and
+// values up to 90 times the typical one take the single conversion.
+constexpr int kMlemTypicalBits = 24;
 
 // Along a box row the reference's float ellipse test has no holes when the
 // quadratic's column-to-column curvature dominates the rounding of q.  With
@@ -604,6 +609,8 @@ int banded_pipeline(petstrip_handle* h, const petstrip::FusedArgs& a, int mode1,
   return PETSTRIP_OK;
 }
 
+int grad_unit_bits(const petstrip_handle* h, int typical_bits = kGradTypicalBits);
+
 int one_iteration(petstrip_handle* h) {
   const int64_t n = h->n_events;
   if (h->fused_ok) {
@@ -612,21 +619,29 @@ int one_iteration(petstrip_handle* h) {
     a.acc_kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
                  : accm == PETSTRIP_ACCUM_COMPENSATED ? petstrip::GLOBAL_F64
                                                        : petstrip::GLOBAL_F32;
+    // FP32: pass 2 adds K / lambda_e (as the gradient does) and the sums are
+    // multiplied by rho at the end; its unit puts a typical value at
+    // 2^kGradTypicalBits.  The other forms add K rho / lambda_e.
+    const bool k_form = accm == PETSTRIP_ACCUM_FP32;
+    const int mode2 = k_form ? petstrip::MODE_MLEM_K : petstrip::MODE_MLEM;
     if (accm == PETSTRIP_ACCUM_FIXED64) {
       // the deterministic image sums every value in 2^-40 units
       a.log2_scale = (float)kFixed64Bits;
       a.split = 1;
       a.inv_fixed_scale = std::ldexp(1.0, -kFixed64Bits);
+    } else if (k_form) {
+      const int k = grad_unit_bits(h, kMlemTypicalBits);
+      a.log2_scale = (float)k;
+      a.inv_fixed_scale = std::ldexp(1.0, -k);
     }
     HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
     if (h->banded) {
       int rc;
-      if ((rc = banded_pipeline(h, a, petstrip::MODE_MLEM, false, petstrip::MODE_MLEM,
-                                h->lds_bytes)))
-        return rc;
+      const size_t lds2 = petstrip::fused_lds_bytes(h->max_ww, h->col_stride, mode2, 2);
+      if ((rc = banded_pipeline(h, a, petstrip::MODE_MLEM, false, mode2, lds2))) return rc;
     } else {
-      HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_MLEM, h->col_stride,
+      HIP_TRY(petstrip::launch_strip_fused(h->stream, mode2, h->col_stride,
                                            accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
                                            h->n_items, h->lds_bytes));
     }
@@ -634,9 +649,13 @@ int one_iteration(petstrip_handle* h) {
     // float accumulators already hold image units; the u64 one holds 2^-40
     double unit = a.acc_kind == petstrip::GLOBAL_U64 ? a.inv_fixed_scale : 1.0;
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, a.acc_kind, unit, 0.0f, 1,
-                                      h->d_flag, h->d_out, (int64_t)h->n_pix));
+                                      h->d_flag, h->d_out, (int64_t)h->n_pix,
+                                      k_form ? h->d_rho : nullptr));
     std::swap(h->d_rho, h->d_out);
     h->lambda_valid = false;
+    // the next unit's image scale: sum rho' = events with a positive
+    // denominator, at most n
+    h->rho_sum = (double)n;
     return PETSTRIP_OK;
   }
   HIP_TRY(hipEventRecord(h->evk0, h->stream));
@@ -1104,14 +1123,15 @@ int backward_per_event(petstrip_handle* h, const float* g, float* grad_rho,
   return PETSTRIP_OK;
 }
 
-// log2 of the fixed-point unit of the log-likelihood gradient's values K /
-// lambda_e: a typical value is 1 / (mean rho x mean ellipse pixels).
-int grad_unit_bits(const petstrip_handle* h) {
+// log2 of the fixed-point unit of the values K / lambda_e (the
+// log-likelihood gradient's, the FP32 MLEM iteration's): a typical value,
+// 1 / (mean rho x mean ellipse pixels), at 2^typical_bits units.
+int grad_unit_bits(const petstrip_handle* h, int typical_bits) {
   const double mean_rho = h->rho_sum / (double)std::max<size_t>(h->n_pix, 1);
   const double mean_ell = (double)h->stats.n_ellipse_pixels /
                           (double)std::max<int64_t>(h->stats.n_events, 1);
   if (!(mean_rho > 0.0) || !(mean_ell > 0.0)) return 26;
-  const int k = (int)std::floor(kGradTypicalBits + std::log2(mean_rho * mean_ell));
+  const int k = (int)std::floor(typical_bits + std::log2(mean_rho * mean_ell));
   return std::max(-60, std::min(60, k));
 }
 
@@ -1134,7 +1154,7 @@ int gradient_fused(petstrip_handle* h, float* grad_rho, double grad_sigma[2],
     int rc;
     if ((rc = banded_pipeline(
              h, a, petstrip::MODE_FORWARD, true, petstrip::MODE_GRAD,
-             petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD))))
+             petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD, 2))))
       return rc;
   } else {
     HIP_TRY(petstrip::launch_strip_fused(

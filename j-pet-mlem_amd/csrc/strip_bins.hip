@@ -87,25 +87,27 @@ __global__ __launch_bounds__(256) void k_item_windows(const EventGeom* __restric
   }
 }
 
-// rho' from the accumulator; the accumulator is cleared for the next use.
+// rho' from the accumulator (times mul[i], the image, for sums of K /
+// lambda_e); the accumulator is cleared for the next use.
 __global__ void k_finalize(void* __restrict__ acc, int kind, double unit, float bias,
                            int check_sign, int* __restrict__ overflow,
-                           float* __restrict__ out, int64_t n) {
+                           float* __restrict__ out, int64_t n, const float* __restrict__ mul) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double u = mul ? unit * (double)mul[i] : unit;
   if (kind == GLOBAL_F32) {
     float* a = reinterpret_cast<float*>(acc);
-    out[i] = (float)((double)a[i] * unit) + bias;
+    out[i] = (float)((double)a[i] * u) + bias;
     a[i] = 0.0f;
   } else if (kind == GLOBAL_F64) {
     double* a = reinterpret_cast<double*>(acc);
-    out[i] = (float)(a[i] * unit) + bias;
+    out[i] = (float)(a[i] * u) + bias;
     a[i] = 0.0;
   } else {
     long long* a = reinterpret_cast<long long*>(acc);
     long long v = a[i];
     if (check_sign && v < 0 && overflow) *overflow = 1;
-    out[i] = (float)((double)v * unit) + bias;
+    out[i] = (float)((double)v * u) + bias;
     a[i] = 0;
   }
 }
@@ -141,9 +143,9 @@ hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
 
 hipError_t launch_finalize(hipStream_t st, void* acc, int kind, double unit,
                            float bias, int check_sign, int* overflow, float* out,
-                           int64_t n) {
+                           int64_t n, const float* mul) {
   hipLaunchKernelGGL(k_finalize, dim3(nblk(n, 256)), dim3(256), 0, st, acc, kind,
-                     unit, bias, check_sign, overflow, out, n);
+                     unit, bias, check_sign, overflow, out, n, mul);
   return hipGetLastError();
 }
 

@@ -19,7 +19,12 @@ struct WorkItem {
 };
 
 enum { GLOBAL_F32 = 0, GLOBAL_F64 = 1, GLOBAL_U64 = 2 };
-enum { MODE_MLEM = 0, MODE_FORWARD = 1, MODE_BACKWARD = 2, MODE_GRAD = 3 };
+// MODE_MLEM_K: the MLEM iteration with pass-2 values K / lambda_e (no rho),
+// the image then multiplied by rho per pixel (rho is constant within an
+// iteration, so sum_e K rho / lambda_e = rho sum_e K / lambda_e); MODE_MLEM
+// adds K rho / lambda_e per value (the fixed 2^-40 format of the
+// deterministic accumulation and the compensated form keep it).
+enum { MODE_MLEM = 0, MODE_FORWARD = 1, MODE_BACKWARD = 2, MODE_GRAD = 3, MODE_MLEM_K = 4 };
 
 struct FusedArgs {
   StripConst sg;             // constants the ellipse/box support was built with
@@ -79,8 +84,9 @@ hipError_t launch_row_counts(hipStream_t st, const StripConst& sg, const EventGe
                              int64_t n, int scan, int32_t* nbox, int32_t* nell);
 
 // LDS bytes per workgroup for windows of at most max_ww columns with column
-// stride col_stride.
-size_t fused_lds_bytes(int max_ww, int col_stride, int mode);
+// stride col_stride; band = 2 (pass 2 of the banded form) holds the rho
+// window only for MODE_MLEM.
+size_t fused_lds_bytes(int max_ww, int col_stride, int mode, int band = 0);
 // Column stride for windows of at most max_wh rows: the next multiple of
 // 64, or 0 when above kMaxColStride.
 int fused_col_stride(int max_wh);
@@ -100,11 +106,12 @@ hipError_t launch_strip_banded(hipStream_t st, int mode, int band, const FusedAr
 hipError_t launch_band_combine(hipStream_t st, const FusedArgs& args, int64_t n_sorted,
                                int sigma);
 
-// out = acc * unit + bias (acc read as kind; 64-bit values as signed), the
-// accumulator cleared; *overflow (nullable) set to 1 if a 64-bit value is
-// negative where only non-negative sums are expected (check_sign).
+// out = acc * unit (* mul, when given) + bias (acc read as kind; 64-bit
+// values as signed), the accumulator cleared; *overflow (nullable) set to 1
+// if a 64-bit value is negative where only non-negative sums are expected
+// (check_sign).
 hipError_t launch_finalize(hipStream_t st, void* acc, int kind, double unit,
                            float bias, int check_sign, int* overflow, float* out,
-                           int64_t n);
+                           int64_t n, const float* mul = nullptr);
 
 }  // namespace petstrip

@@ -21,6 +21,9 @@
 //   MLEM      pass 1 + pass 2 with w_e = 1/lambda_e; each value K rho /
 //             lambda_e <= 1 is added in 2^-31 units, or in 2^-40 units
 //             (split conversion) for the deterministic accumulation
+//   MLEM_K    MLEM with pass-2 values K / lambda_e, as GRAD adds them (no
+//             rho read or multiply per pixel); rho multiplies the sums
+//             once per pixel afterwards
 //   FORWARD   pass 1 only, plus the sigma derivative of lambda_e and the
 //             log-likelihood terms, reduced per workgroup into doubles
 //   GRAD      FORWARD and a pass 2 with w_e = 1/lambda_e (no rho): the
@@ -504,6 +507,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   constexpr bool kPass2 = BAND == 2 || (BAND == 0 && MODE != MODE_FORWARD);
   constexpr bool kRho = kPass1 || (kPass2 && MODE == MODE_MLEM);  // rho window in LDS
   constexpr bool kSigma = kPass1 && (MODE == MODE_FORWARD || MODE == MODE_GRAD);
+  // pass-2 values K / lambda_e, without rho
+  constexpr bool kNoRho2 = MODE == MODE_GRAD || MODE == MODE_MLEM_K;
   constexpr int kThreads = 64 * kWaves;
   // pass 1's group loop unrolled four times, except with the sigma sums in
   // the two-pass kernel, where the extra registers slowed the whole kernel
@@ -661,7 +666,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
       // lw = log2 of the event multiplier in fixed-point units
       float lw;
       bool neg = false;
-      if constexpr (MODE == MODE_MLEM || MODE == MODE_GRAD) {
+      if constexpr (MODE == MODE_MLEM || kNoRho2) {
         if (!(lam > 0.0f)) continue;
         lw = log2_shift(a.log2_scale, lam, -1.0f);
       } else if (a.g) {
@@ -709,11 +714,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
               backproject_rows<HP, true, false, false, false>(s, du, lw, lam, unit_scale,
                                                               gtab, rho0, acc_w);
             }
-          } else if constexpr (MODE == MODE_GRAD) {
+          } else if constexpr (kNoRho2) {
             // K / lambda has no bound: rows whose values stay below 2^30.5
             // (or 2^52) fixed-point units take the single rounding (split)
             // conversion, the rare rest is added to the global image in
-            // doubles
+            // its own format
             const float vmax = s.lsc + lw;
             if (!__any(vmax > 30.5f)) {
               backproject_rows<HP, false, true, false, false, true>(
@@ -723,12 +728,16 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
                                                              gtab, rho0, acc_w);
             } else if (s.len > 0) {
               const int col0 = it.wx0 + (s.w0 - (row0 + r + it.wx0 * HP)) / HP;
-              double* img = reinterpret_cast<double*>(a.acc) +
-                            (size_t)(ry0 + r) * sg.W + col0;
+              const size_t p0 = (size_t)(ry0 + r) * sg.W + col0;
               for (int k = 0; k < s.len; ++k) {
                 const float uu = fmaf((float)k, du, s.u0);
                 const float kv = __builtin_amdgcn_exp2f(fmaf(-uu, uu, s.lsc));
-                unsafeAtomicAdd(img + k, (double)kv / (double)lam);
+                if (a.acc_kind == GLOBAL_F64)
+                  unsafeAtomicAdd(reinterpret_cast<double*>(a.acc) + p0 + k,
+                                  (double)kv / (double)lam);
+                else
+                  unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + p0 + k,
+                                  (float)((double)kv / (double)lam));
               }
             }
           } else if (a.split) {
@@ -878,11 +887,10 @@ hipError_t launch_row_counts(hipStream_t st, const StripConst& sg, const EventGe
 
 // ---------------------------------------------------------------------------
 
-size_t fused_lds_bytes(int max_ww, int col_stride, int mode) {
+size_t fused_lds_bytes(int max_ww, int col_stride, int mode, int band) {
   const size_t hp = (size_t)col_stride;
-  size_t rho = mode == MODE_BACKWARD
-                   ? 0
-                   : ((((size_t)max_ww + kSpareCols) * hp * 4 + 15) & ~(size_t)15);
+  const bool no_rho = mode == MODE_BACKWARD || (band == 2 && mode != MODE_MLEM);
+  size_t rho = no_rho ? 0 : ((((size_t)max_ww + kSpareCols) * hp * 4 + 15) & ~(size_t)15);
   size_t acc = mode == MODE_FORWARD ? 0 : (size_t)max_ww * hp * 8;
   return rho + acc + 16;
 }
@@ -1013,6 +1021,8 @@ hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int comp
                  : launch_h<MODE_FORWARD, false>(col_stride, st, a, n_items, lds_bytes);
     case MODE_GRAD:
       return launch_h<MODE_GRAD, false>(col_stride, st, a, n_items, lds_bytes);
+    case MODE_MLEM_K:
+      return launch_h<MODE_MLEM_K, false>(col_stride, st, a, n_items, lds_bytes);
     default:
       return launch_h<MODE_BACKWARD, false>(col_stride, st, a, n_items, lds_bytes);
   }
@@ -1029,6 +1039,8 @@ hipError_t launch_strip_banded(hipStream_t st, int mode, int band, const FusedAr
   switch (mode) {
     case MODE_MLEM: return launch_t<MODE_MLEM, false, kBandRows, 2>(st, a, n_items, lds_bytes);
     case MODE_GRAD: return launch_t<MODE_GRAD, false, kBandRows, 2>(st, a, n_items, lds_bytes);
+    case MODE_MLEM_K:
+      return launch_t<MODE_MLEM_K, false, kBandRows, 2>(st, a, n_items, lds_bytes);
     default: return launch_t<MODE_BACKWARD, false, kBandRows, 2>(st, a, n_items, lds_bytes);
   }
 }
