@@ -191,6 +191,44 @@ def gradient_report(pkg, name):
             "grad_ms": g_ms, "grad_over_mlem": g_ms / mlem}
 
 
+def variant_report(pkg, name, ev, no_cpu, threads):
+    """The reference --gpu iteration as an option (--update gpu: the analytic
+    kernel and the 5-point pixel sensitivity in the denominator,
+    cuda/reconstruction.cu:59-113) on the same events: it runs on the
+    per-event kernels (one wave per event over its box, FP64 image atomics),
+    since the analytic kernel has no row factorisation for the fused walks.
+    Iteration time, events/s and, as the checker, one iteration on a prefix
+    against the oracle's FP64 variant."""
+    import numpy as np
+    sc = pkg.workload_scanner(name)
+    kw = dict(kernel=1, update=pkg.UPDATE_SENSITIVITY_DENOMINATOR)
+    out = {"workload": name, "kernel": "analytic", "denominator": "sensitivity-weighted",
+           "path": "per-event kernels, FP64 image atomics", "events": len(ev)}
+    with pkg.Reconstruction(sc, device=0, **kw) as rec:
+        rec.set_events(ev)
+        rec(1)
+        ms = []
+        for _ in range(2):
+            rec(1)
+            ms.append(rec.stats()["last_iteration_ms"])
+    out["ms_per_iteration"] = float(np.median(ms))
+    out["events_per_s_iter"] = len(ev) / (out["ms_per_iteration"] * 1e-3)
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        m = min(20000, len(ev))
+        sens = ol.sensitivity_f32(sc, 1).astype(np.float64)
+        ref = ol.mlem_variant_f64(sc, ev[:m], 1, kernel=1, sens=sens, n_threads=threads)[0]
+        with pkg.Reconstruction(sc, device=0, **kw) as rec:
+            rec.set_events(ev[:m])
+            rec(1)
+            got = rec.rho.copy()
+        l2, mx = _errors(got, ref)
+        out["parity"] = {"rel_l2_vs_fp64": l2, "max_pixel_rel_vs_fp64": mx,
+                         "sample": f"one iteration over the first {m} events"}
+    return out
+
+
 CFG5 = "strip-1024px-0.5mm-1e8ev"
 
 
@@ -202,8 +240,9 @@ def cfg5_report(pkg, rounds, threads, no_cpu):
     with the support held at the start (calibration.py), its recovered
     sigma, events/s and, as the checker, one more iteration from the final
     image against FP64 on a prefix; and the same sigma ascent with the image
-    held at the phantom's expected image of detected emissions, for half as
-    many rounds (the estimator's check: it recovers the simulated widths)."""
+    held at the phantom's expected image of detected emissions, for at most
+    six rounds, within which it settles (the estimator's check: it recovers
+    the simulated widths)."""
     import numpy as np
     import torch
     n_pix, size, n_ev, _, phantom, seed = pkg.WORKLOADS[CFG5]
@@ -243,7 +282,7 @@ def cfg5_report(pkg, rounds, threads, no_cpu):
         # the estimator on the true image: sigma steps only
         rec.set_sigma(sc.sigma_z, sc.sigma_dl)
         rec.rho = pkg.expected_image(sc, phantom, n_ev, rec.sensitivity(0))
-        n_exp = max(1, rounds // 2)
+        n_exp = max(1, min(6, rounds))
         exp = pkg.calibrate_sigma(rec, (0.015, 0.06), rounds=n_exp, mlem_per_round=0)
         out["calibration_expected_image"] = {
             "recovered": [exp["sigma_z"], exp["sigma_dl"]], "rounds": n_exp,
@@ -280,6 +319,8 @@ def parse():
                    help="3D hybrid workload measured beside the headline line "
                         "('none' skips it)")
     p.add_argument("--no-grad", action="store_true")
+    p.add_argument("--no-variant", action="store_true",
+                   help="skip the reference --gpu variant (analytic kernel) line")
     p.add_argument("--grad-workload", default="strip-256px-2mm-5e6ev",
                    help="workload of the gradient-vs-MLEM-iteration line "
                         "('none' skips it)")
@@ -556,6 +597,9 @@ def main():
     if args.cfg5_rounds > 0 and name != CFG5:
         print(f"calibration workload {CFG5}", file=sys.stderr, flush=True)
         out["cfg5"] = cfg5_report(pkg, args.cfg5_rounds, args.cpu_threads, args.no_cpu)
+    if not args.no_variant:
+        print(f"reference --gpu variant on {name}", file=sys.stderr, flush=True)
+        out["variant"] = variant_report(pkg, name, ev, args.no_cpu, args.cpu_threads)
     if args.hybrid != "none":
         print(f"hybrid workload {args.hybrid}", file=sys.stderr, flush=True)
         out["hybrid"] = hybrid_report(pkg, args.hybrid, args.steps, args.no_cpu,

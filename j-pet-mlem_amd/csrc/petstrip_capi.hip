@@ -658,17 +658,26 @@ int one_iteration(petstrip_handle* h) {
     h->rho_sum = (double)n;
     return PETSTRIP_OK;
   }
+  // per-event kernels: compensated denominators in the compensated mode;
+  // the deterministic mode sums the image in 2^-40 units (exact, any order),
+  // the others in FP64
+  const int accm = h->opts.accumulation;
+  const bool fixed = accm == PETSTRIP_ACCUM_FIXED64;
   HIP_TRY(hipEventRecord(h->evk0, h->stream));
   HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
                                        h->d_rho, denominator_weight(h),
-                                       h->d_lambda, h->d_inv, h->opts.kernel));
-  // the FP64 accumulator is zero between uses (finalize clears it)
-  HIP_TRY(petstrip::launch_backproject(h->stream, h->sg, h->sk, h->d_geom, n,
-                                       h->d_rho, h->d_inv, (double*)h->d_acc,
-                                       h->opts.kernel));
+                                       h->d_lambda, h->d_inv, h->opts.kernel,
+                                       accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0));
+  // the accumulator is zero between uses (finalize clears it)
+  HIP_TRY(petstrip::launch_backproject(
+      h->stream, h->sg, h->sk, h->d_geom, n, h->d_rho, h->d_inv,
+      fixed ? nullptr : (double*)h->d_acc, h->opts.kernel,
+      fixed ? (unsigned long long*)h->d_acc : nullptr, std::ldexp(1.0, kFixed64Bits)));
   HIP_TRY(hipEventRecord(h->evk1, h->stream));
-  HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0, 0.0f,
-                                    0, nullptr, h->d_out, (int64_t)h->n_pix));
+  HIP_TRY(petstrip::launch_finalize(
+      h->stream, h->d_acc, fixed ? petstrip::GLOBAL_U64 : petstrip::GLOBAL_F64,
+      fixed ? std::ldexp(1.0, -kFixed64Bits) : 1.0, 0.0f, fixed ? 1 : 0,
+      fixed ? h->d_flag : nullptr, h->d_out, (int64_t)h->n_pix));
   std::swap(h->d_rho, h->d_out);
   h->lambda_valid = false;
   return PETSTRIP_OK;
@@ -1068,9 +1077,9 @@ int petstrip_forward(petstrip_handle* h, float* lambda) {
     HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
     if ((rc = fused_forward(h, a, false))) return rc;
   } else {
-    HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
-                                         h->d_rho, nullptr, h->d_lambda, h->d_inv,
-                                         h->opts.kernel));
+    HIP_TRY(petstrip::launch_denominator(
+        h->stream, h->sg, h->sk, h->d_geom, n, h->d_rho, nullptr, h->d_lambda, h->d_inv,
+        h->opts.kernel, h->opts.accumulation == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0));
   }
   if (lambda && n > 0)
     HIP_TRY(hipMemcpyAsync(lambda, h->d_lambda, n * sizeof(float),

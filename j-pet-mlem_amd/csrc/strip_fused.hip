@@ -56,6 +56,7 @@
 
 #include "strip_fused.h"
 #include "strip_geometry.h"
+#include "wave_ops.h"
 
 namespace petstrip {
 
@@ -63,23 +64,6 @@ namespace {
 
 __device__ __forceinline__ int uniform(int v) {
   return __builtin_amdgcn_readfirstlane(v);
-}
-
-// Wave64 sum, fixed order: DPP within rows of 16, then the four row sums.
-__device__ __forceinline__ float wave_sum(float v) {
-  v += __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xb1, 0xf, 0xf, false));
-  v += __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4e, 0xf, 0xf, false));
-  v += __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false));
-  v += __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false));
-  float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-  float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-  float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-  float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-  return (r0 + r1) + (r2 + r3);
 }
 
 // Wave64 maximum of a non-negative int (same DPP pattern as wave_sum).
@@ -93,37 +77,9 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return max(max(r0, r1), max(r2, r3));
 }
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 __device__ __forceinline__ float wave_max_f(float v) {
   for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
   return v;
-}
-
-__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
-  s = a + b;
-  float bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
-
-// Compensated wave sum of (hi, lo) pairs; partners combine in lane order so
-// both compute the same value.
-__device__ __forceinline__ float wave_sum_comp(float hi, float lo) {
-  for (int off = 1; off < 64; off <<= 1) {
-    float oh = __shfl_xor(hi, off, 64);
-    float ol = __shfl_xor(lo, off, 64);
-    bool upper = (__lane_id() & off) != 0;
-    float a = upper ? oh : hi, b = upper ? hi : oh;
-    float la = upper ? ol : lo, lb = upper ? lo : ol;
-    float s, e;
-    two_sum(a, b, s, e);
-    hi = s;
-    lo = e + (la + lb);
-  }
-  return hi + lo;
 }
 
 // Exact reference membership of column iz in the row with (Ady2, Bdy).

@@ -12,14 +12,18 @@ hipError_t launch_event_setup(hipStream_t st, const StripConst& s,
 hipError_t launch_footprint(hipStream_t st, const StripConst& s,
                             const EventGeom* geom, int64_t n, int32_t* nbox,
                             int32_t* nell);
+// comp = 1: compensated (two-sum) denominators.
 hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* weight,
-                              float* lambda, float* inv, int analytic = 0);
+                              float* lambda, float* inv, int analytic = 0, int comp = 0);
+// out (FP64 sums), or out_u64 (non-negative values as integers in units
+// 1 / fixed_scale) when given.
 hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* w,
-                              double* out, int analytic = 0);
+                              double* out, int analytic = 0,
+                              unsigned long long* out_u64 = nullptr, double fixed_scale = 0.0);
 // Analytic-kernel map of tan bins (w x h x planes floats into out).
 hipError_t launch_tan_kernel_map(hipStream_t st, const StripConst& k, int w, int h,
                                  int planes, float pixel, float llx, float lly,

@@ -17,36 +17,12 @@
 
 #include "strip_geometry.h"
 #include "strip_kernels.h"
+#include "wave_ops.h"
 
 namespace petstrip {
 
 // ---------------------------------------------------------------------------
 // Wave64 helpers.
-
-// Sum over the 64 lanes of a wave; every lane returns the same value and the
-// order of additions is fixed (bit-reproducible).  Rows of 16 lanes are
-// reduced with DPP (quad_perm 1032, quad_perm 2301, row_ror 4, row_ror 8),
-// then the four row sums are combined in a fixed order.
-__device__ __forceinline__ float wave_sum(float v) {
-  v += __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xb1, 0xf, 0xf, false));
-  v += __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4e, 0xf, 0xf, false));
-  v += __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false));
-  v += __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false));
-  float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-  float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-  float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-  float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-  return (r0 + r1) + (r2 + r3);
-}
-
-__device__ __forceinline__ double wave_sum_d(double v) {
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 __device__ __forceinline__ int uniform_i(int v) {
   return __builtin_amdgcn_readfirstlane(v);
@@ -154,6 +130,8 @@ __global__ __launch_bounds__(256) void k_footprint(StripConst s,
 // Pass 1: denominators.  weight (nullable) multiplies rho (the reference
 // --gpu variant weights the denominator by the pixel sensitivity).
 
+// comp: each lane's terms and the wave's lanes summed with two-sums (the
+// compensated accumulation mode); otherwise plain FP32 in a fixed order.
 __global__ __launch_bounds__(256) void k_denominator(StripConst sg, StripConst sk,
                                                      const EventGeom* __restrict__ geom,
                                                      int64_t n,
@@ -161,14 +139,14 @@ __global__ __launch_bounds__(256) void k_denominator(StripConst sg, StripConst s
                                                      const float* __restrict__ weight,
                                                      float* __restrict__ lambda,
                                                      float* __restrict__ inv,
-                                                     int analytic) {
+                                                     int analytic, int comp) {
   const int lane = threadIdx.x & 63;
   int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
   if (e >= n) return;
   EventGeom g = load_geom(geom, e);
   int bw = g.x1 - g.x0, bh = g.y1 - g.y0;
   int nbox = (bw > 0 && bh > 0) ? bw * bh : 0;
-  float acc = 0.0f;
+  float acc = 0.0f, err = 0.0f;
   for (int k = lane; k < nbox; k += 64) {
     int row = k / bw, col = k - row * bw;
     int iy = g.y0 + row, iz = g.x0 + col;
@@ -178,10 +156,18 @@ __global__ __launch_bounds__(256) void k_denominator(StripConst sg, StripConst s
       int p = iy * sg.W + iz;
       float r = rho[p];
       if (weight) r *= weight[p];
-      acc += event_kernel(sk, g, dy, dz, analytic) * r;
+      const float t = event_kernel(sk, g, dy, dz, analytic) * r;
+      if (comp) {
+        float s, d;
+        two_sum(acc, t, s, d);
+        acc = s;
+        err += d;
+      } else {
+        acc += t;
+      }
     }
   }
-  float d = wave_sum(acc);
+  float d = comp ? wave_sum_comp(acc, err) : wave_sum(acc);
   if (lane == 0) {
     lambda[e] = d;
     inv[e] = (d > 0.0f) ? 1.0f / d : 0.0f;
@@ -191,7 +177,9 @@ __global__ __launch_bounds__(256) void k_denominator(StripConst sg, StripConst s
 // ---------------------------------------------------------------------------
 // Pass 2: backprojection out[p] += K rho[p] * w_e (w_e = inv_e for MLEM,
 // rho == nullptr gives out[p] += K * w_e, the adjoint of the projector).
-// The sums are FP64: a pixel can take millions of adds at 1e8 events.
+// The sums are FP64 (a pixel can take millions of adds at 1e8 events), or,
+// with out_u64, exact 64-bit integer sums of the non-negative values in
+// units 1 / fixed_scale (run-to-run identical images).
 
 __global__ __launch_bounds__(256) void k_backproject(StripConst sg, StripConst sk,
                                                      const EventGeom* __restrict__ geom,
@@ -199,7 +187,9 @@ __global__ __launch_bounds__(256) void k_backproject(StripConst sg, StripConst s
                                                      const float* __restrict__ rho,
                                                      const float* __restrict__ w,
                                                      double* __restrict__ out,
-                                                     int analytic) {
+                                                     int analytic,
+                                                     unsigned long long* __restrict__ out_u64,
+                                                     double fixed_scale) {
   const int lane = threadIdx.x & 63;
   int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + uniform_i(threadIdx.x >> 6);
   if (e >= n) return;
@@ -217,7 +207,11 @@ __global__ __launch_bounds__(256) void k_backproject(StripConst sg, StripConst s
       int p = iy * sg.W + iz;
       float kv = event_kernel(sk, g, dy, dz, analytic);
       if (rho) kv *= rho[p];
-      unsafeAtomicAdd(&out[p], (double)(kv * we));
+      if (out_u64) {
+        const double v = (double)(kv * we);
+        if (v > 0.0) atomicAdd(out_u64 + p, (unsigned long long)fma(v, fixed_scale, 0.5));
+      } else
+        unsafeAtomicAdd(&out[p], (double)(kv * we));
     }
   }
 }
@@ -421,12 +415,13 @@ hipError_t launch_footprint(hipStream_t st, const StripConst& s,
 hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* weight,
-                              float* lambda, float* inv, int analytic) {
+                              float* lambda, float* inv, int analytic, int comp) {
   if (n <= 0) return hipSuccess;
   for (int64_t e0 = 0; e0 < n; e0 += kWaveSlice) {
     const int64_t m = std::min(kWaveSlice, n - e0);
     hipLaunchKernelGGL(k_denominator, dim3(blocks_for(m, 4)), dim3(256), 0, st,
-                       sg, sk, geom + e0, m, rho, weight, lambda + e0, inv + e0, analytic);
+                       sg, sk, geom + e0, m, rho, weight, lambda + e0, inv + e0, analytic,
+                       comp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -436,12 +431,14 @@ hipError_t launch_denominator(hipStream_t st, const StripConst& sg,
 hipError_t launch_backproject(hipStream_t st, const StripConst& sg,
                               const StripConst& sk, const EventGeom* geom,
                               int64_t n, const float* rho, const float* w,
-                              double* out, int analytic) {
+                              double* out, int analytic, unsigned long long* out_u64,
+                              double fixed_scale) {
   if (n <= 0) return hipSuccess;
   for (int64_t e0 = 0; e0 < n; e0 += kWaveSlice) {
     const int64_t m = std::min(kWaveSlice, n - e0);
     hipLaunchKernelGGL(k_backproject, dim3(blocks_for(m, 4)), dim3(256), 0, st,
-                       sg, sk, geom + e0, m, rho, w + e0, out, analytic);
+                       sg, sk, geom + e0, m, rho, w + e0, out, analytic, out_u64,
+                       fixed_scale);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -93,7 +93,7 @@ def test_denominators(gpu_pkg):
 
 
 MODES = [("auto", "fp32"), ("auto", "compensated"), ("auto", "fixed64"),
-         ("per_event", "fp32")]
+         ("per_event", "fp32"), ("per_event", "compensated"), ("per_event", "fixed64")]
 
 
 def opts(pkg, path, acc):
@@ -190,15 +190,45 @@ def test_sensitivity(gpu_pkg):
             assert np.max(np.abs(s - ref)) < 2e-6
 
 
-def test_fixed64_bit_reproducible(gpu_pkg):
-    sc, ev = small(gpu_pkg, n=30000)
+@pytest.mark.parametrize("path,variant", [("auto", False), ("per_event", False),
+                                          ("per_event", True)])
+def test_fixed64_bit_reproducible(gpu_pkg, path, variant):
+    """The deterministic accumulation gives the same bits run after run: the
+    fused walks and the per-event kernels (also with the reference --gpu
+    variant's analytic kernel and sensitivity-weighted denominators)."""
+    pkg = gpu_pkg
+    sc, ev = small(pkg, n=30000)
+    kw = opts(pkg, path, "fixed64")
+    if variant:
+        kw.update(kernel=1, update=pkg.UPDATE_SENSITIVITY_DENOMINATOR)
     outs = []
     for _ in range(2):
-        with gpu_pkg.Reconstruction(sc, accumulation=gpu_pkg.ACCUM_FIXED64) as rec:
+        with pkg.Reconstruction(sc, **kw) as rec:
             rec.set_events(ev)
             rec(3)
             outs.append(rec.rho)
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_per_event_compensated_denominators(gpu_pkg):
+    """The per-event kernels' compensated denominators (two-sum per lane and
+    across the wave) are closer to the FP64 sums than the plain FP32 ones."""
+    pkg = gpu_pkg
+    sc, ev = small(pkg, n=20000)
+    rho = np.random.default_rng(4).uniform(0.5, 2.0, (64, 64)).astype(np.float32)
+    lams = {}
+    for acc in ("fp32", "compensated"):
+        with pkg.Reconstruction(sc, **opts(pkg, "per_event", acc)) as rec:
+            rec.set_events(ev)
+            rec.rho = rho
+            lams[acc] = rec.forward()
+    _, ref = ol.loglik_f64(sc, ev, rho, want_lambda=True, want_grad=False)[:2]
+    pos = ref > 0
+    err = {k: np.abs(v[pos] - ref[pos]) / ref[pos] for k, v in lams.items()}
+    print(f"mean rel err vs FP64: fp32 {err['fp32'].mean():.2e}, compensated "
+          f"{err['compensated'].mean():.2e}", flush=True)
+    assert err["compensated"].mean() < err["fp32"].mean()
+    assert err["compensated"].max() < 2e-6
 
 
 def test_lambda_of_fused_iteration(gpu_pkg):
