@@ -158,6 +158,8 @@ struct petstrip_handle {
   petstrip::WorkItem* d_items = nullptr;
   size_t cap_items = 0;
   int n_items = 0;
+  // first items of the boxes at most 32 and at most 16 rows tall
+  int pack_begin[2] = {0, 0};
   int64_t window_px = 0;  // sum over work items of window pixels
   int max_rows = 1;
   size_t lds_bytes = 0;
@@ -313,8 +315,10 @@ int update_kernel_table(petstrip_handle* h) {
   return PETSTRIP_OK;
 }
 
-// Sorts events into kBinW x kBinH bins of their box centre, cuts each bin into
-// chunks of kChunk events and sizes each chunk's LDS window.
+// Sorts events by box height class (taller than 32 rows, 17-32, at most 16:
+// the walks take one, two or four events per wave) and then into kBinW x
+// kBinH bins of their box centre, cuts each bin into chunks of kChunk events
+// and sizes each chunk's LDS window.
 int prepare_fused(petstrip_handle* h) {
   h->fused_ok = false;
   h->n_items = 0;
@@ -323,12 +327,13 @@ int prepare_fused(petstrip_handle* h) {
   const int64_t n = h->n_events;
   const int W = h->sg.W, H = h->sg.H;
   const int nbx = (W + kBinW - 1) / kBinW, nby = (H + kBinH - 1) / kBinH;
-  const int nb = nbx * nby;
+  const int n_spatial = nbx * nby;
+  const int nb = n_spatial * petstrip::kHeightClasses;
   dfree(h->d_count);
   int rc;
   if ((rc = dalloc(&h->d_count, (size_t)nb * 2))) return rc;
   HIP_TRY(hipMemsetAsync(h->d_count, 0, (size_t)nb * 2 * sizeof(int32_t), h->stream));
-  HIP_TRY(petstrip::launch_bin_count(h->stream, h->d_geom, n, kBinW, kBinH, nbx,
+  HIP_TRY(petstrip::launch_bin_count(h->stream, h->d_geom, n, kBinW, kBinH, nbx, n_spatial,
                                      h->d_bin, h->d_count));
   std::vector<int32_t> cnt((size_t)nb), off((size_t)nb);
   HIP_TRY(hipMemcpyAsync(cnt.data(), h->d_count, nb * sizeof(int32_t),
@@ -337,6 +342,9 @@ int prepare_fused(petstrip_handle* h) {
   int64_t acc = 0;
   std::vector<petstrip::WorkItem> items;
   for (int b = 0; b < nb; ++b) {
+    // the first items of the 17-32 and the at most 16 row classes
+    if (b == n_spatial) h->pack_begin[0] = (int)items.size();
+    if (b == 2 * n_spatial) h->pack_begin[1] = (int)items.size();
     off[(size_t)b] = (int32_t)acc;
     for (int64_t s0 = acc; s0 < acc + cnt[(size_t)b]; s0 += kChunk) {
       petstrip::WorkItem it{};
@@ -457,6 +465,8 @@ int prepare_fused(petstrip_handle* h) {
       }
       items.swap(bands);
       h->n_items = (int)items.size();
+      // banded items walk one event per wave
+      h->pack_begin[0] = h->pack_begin[1] = h->n_items;
       if (items.size() > h->cap_items) {
         dfree(h->d_items);
         if ((rc = dalloc(&h->d_items, items.size()))) return rc;
@@ -643,7 +653,7 @@ int one_iteration(petstrip_handle* h) {
     } else {
       HIP_TRY(petstrip::launch_strip_fused(h->stream, mode2, h->col_stride,
                                            accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
-                                           h->n_items, h->lds_bytes));
+                                           h->n_items, h->lds_bytes, h->pack_begin));
     }
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     // float accumulators already hold image units; the u64 one holds 2^-40
@@ -1168,7 +1178,8 @@ int gradient_fused(petstrip_handle* h, float* grad_rho, double grad_sigma[2],
   } else {
     HIP_TRY(petstrip::launch_strip_fused(
         h->stream, petstrip::MODE_GRAD, h->col_stride, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD)));
+        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD),
+        h->pack_begin));
   }
   HIP_TRY(hipEventRecord(h->evk1, h->stream));
   HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0, -1.0f,

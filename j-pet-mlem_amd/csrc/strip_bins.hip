@@ -10,17 +10,20 @@
 namespace petstrip {
 
 
-// Bin of an event: the bin of its box centre; empty boxes get -1.
+// Bin of an event: its box height class (taller than 32 rows, 17-32, at
+// most 16), then the bin of its box centre; empty boxes get -1.
 __global__ void k_bin_count(const EventGeom* __restrict__ geom, int64_t n,
-                            int bin_w, int bin_h, int nbx, int32_t* __restrict__ bin,
-                            int32_t* __restrict__ count) {
+                            int bin_w, int bin_h, int nbx, int n_spatial,
+                            int32_t* __restrict__ bin, int32_t* __restrict__ count) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   EventGeom g = geom[e];
   int b = -1;
   if (g.x1 > g.x0 && g.y1 > g.y0) {
     int cx = (g.x0 + g.x1) >> 1, cy = (g.y0 + g.y1) >> 1;
-    b = (cy / bin_h) * nbx + (cx / bin_w);
+    const int rows = g.y1 - g.y0;
+    const int cls = rows <= 16 ? 2 : rows <= 32 ? 1 : 0;
+    b = cls * n_spatial + (cy / bin_h) * nbx + (cx / bin_w);
     atomicAdd(&count[b], 1);
   }
   bin[e] = b;
@@ -117,11 +120,11 @@ __global__ void k_finalize(void* __restrict__ acc, int kind, double unit, float 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_bin_count(hipStream_t st, const EventGeom* geom, int64_t n,
-                            int bin_w, int bin_h, int nbx, int32_t* bin,
+                            int bin_w, int bin_h, int nbx, int n_spatial, int32_t* bin,
                             int32_t* count) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bin_count, dim3(nblk(n, 256)), dim3(256), 0, st, geom, n,
-                     bin_w, bin_h, nbx, bin, count);
+                     bin_w, bin_h, nbx, n_spatial, bin, count);
   return hipGetLastError();
 }
 

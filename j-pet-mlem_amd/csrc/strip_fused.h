@@ -62,8 +62,12 @@ struct FusedArgs {
 constexpr int kScalStride = 8;       // doubles per workgroup in scal_part
 constexpr int kCombineBlocks = 1024; // workgroups of k_band_combine (grid-stride)
 
+// Bin of an event: (height class) x (spatial bin of its box centre), the
+// classes in the order of boxes taller than 32 rows, 17-32 rows, at most 16
+// (n_spatial spatial bins per class); empty boxes get -1.
+constexpr int kHeightClasses = 3;
 hipError_t launch_bin_count(hipStream_t st, const EventGeom* geom, int64_t n,
-                            int bin_w, int bin_h, int nbx, int32_t* bin,
+                            int bin_w, int bin_h, int nbx, int n_spatial, int32_t* bin,
                             int32_t* count);
 hipError_t launch_bin_scatter(hipStream_t st, const EventGeom* geom, int64_t n,
                               const int32_t* bin, int32_t* cursor,
@@ -91,9 +95,13 @@ size_t fused_lds_bytes(int max_ww, int col_stride, int mode, int band = 0);
 // 64, or 0 when above kMaxColStride.
 int fused_col_stride(int max_wh);
 // MODE_FORWARD and MODE_GRAD also add their five scalars to args.scalars,
-// through args.scal_part (n_items * kScalStride doubles).
+// through args.scal_part (n_items * kScalStride doubles).  pack_begin
+// (nullable, two entries): the first item whose boxes are at most 32 rows
+// tall and the first whose boxes are at most 16 (items sorted so); MLEM,
+// MLEM_K and GRAD then walk two or four of their events per wave at a time.
 hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int compensated,
-                              const FusedArgs& args, int n_items, size_t lds_bytes);
+                              const FusedArgs& args, int n_items, size_t lds_bytes,
+                              const int* pack_begin = nullptr);
 // Banded form (column stride kBandRows): band = 1 runs pass 1 into the band
 // partials (mode MODE_FORWARD with sigma sums, else lambda only), band = 2
 // runs pass 2 of mode (MLEM, GRAD or BACKWARD) with the combined lambda.

@@ -457,8 +457,17 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
 // (lambda, sigma_z and sigma_dl sums, largest row factor) per event and
 // band; k_band_combine adds them in a fixed order; BAND = 2 runs pass 2
 // with the complete lambda.
-template <int MODE, bool COMP, int HP, int BAND>
+//
+// PACK = 2, 4 (work items whose boxes are at most 32 or 16 rows tall): a wave
+// takes PACK events at a time, one per aligned segment of 64 / PACK lanes, so
+// that short events do not leave most lanes idle; the denominator is then a
+// segment sum and every per-event value lives per lane.
+template <int MODE, bool COMP, int HP, int BAND, int PACK>
 __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
+  static_assert(PACK == 1 || (BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
+                                            MODE == MODE_GRAD)),
+                "packed events: the unbanded iteration and gradient");
+  constexpr int kSeg = 64 / PACK;  // lanes per event
   constexpr bool kPass1 = BAND == 1 || (BAND == 0 && MODE != MODE_BACKWARD);
   constexpr bool kPass2 = BAND == 2 || (BAND == 0 && MODE != MODE_FORWARD);
   constexpr bool kRho = kPass1 || (kPass2 && MODE == MODE_MLEM);  // rho window in LDS
@@ -486,6 +495,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
       smem + (kRho ? (((size_t)nrho * 4 + 15) & ~(size_t)15) : 0));
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = __lane_id();
+  const int seg = PACK > 1 ? lane / kSeg : 0;      // the lane's event of the wave's PACK
+  const int lane_s = PACK > 1 ? lane - seg * kSeg : lane;  // the lane's row in it
 
   // Window load and flush visit tiles of 16 columns x 4 rows per wave, so
   // that the global reads and atomics of a wave cover 64-byte row segments.
@@ -519,11 +530,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   const float unit_scale = __builtin_amdgcn_exp2f(a.log2_scale - (a.split ? 32.0f : 0.0f));
   double ll = 0.0, gsz = 0.0, gsdl = 0.0, mmax = 0.0, msum = 0.0;
 
-  for (int e = it.begin + wave; e < it.end; e += kWaves) {
-    const EventGeom g = a.sgeom[e];
+  for (int e0 = it.begin + wave * PACK; e0 < it.end; e0 += kWaves * PACK) {
+    const int e = e0 + seg;
+    const bool live = PACK == 1 || e < it.end;  // lanes past the item's last event idle
+    const EventGeom g = a.sgeom[live ? e : e0];
     // box rows walked here: all of them, or those in this item's band
     // and columns: the box's, or those in this item's column band
-    int ry0 = g.y0, ry1 = g.y1, cx0 = g.x0, cx1 = g.x1;
+    int ry0 = g.y0, ry1 = live ? g.y1 : g.y0, cx0 = g.x0, cx1 = g.x1;
     if constexpr (BAND != 0) {
       ry0 = max(g.y0, it.wy0);
       ry1 = min(g.y1, it.wy0 + wh);
@@ -532,9 +545,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
       if (ry1 <= ry0 || cx1 <= cx0) continue;
     }
     const int nrows = ry1 - ry0;
-    const int ngroups = (nrows + 63) >> 6;
+    // packed events have at most kSeg rows: one row group for the wave
+    const int ngroups = PACK > 1 ? 1 : (nrows + 63) >> 6;
     const int row0 = (ry0 - it.wy0) - it.wx0 * HP;  // window index of (ry0, 0)
-    const int orig = a.sidx[e];
+    const int orig = a.sidx[live ? e : e0];
     const EventRows ev = event_rows(sg, g, inv_pw);
     // group 0 takes the 64 rows around the ellipse centre, later groups the
     // rows above and below it in order
@@ -552,9 +566,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
           if (gg >= ngroups) break;
           const int j = gg * 64 + lane - 64;
-          const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
+          const int r = PACK > 1 ? lane_s : gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
-          if (gg * 64 + lane < nrows)
+          if ((PACK > 1 ? lane_s : gg * 64 + lane) < nrows)
             s = row_setup<kSigma>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan, cx0,
                                   cx1);
           if (gi < kCachedGroups) cache[gi] = s;
@@ -576,7 +590,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           }
         }
       }
-      lam = COMP ? wave_sum_comp(acc, comp) : wave_sum(acc);
+      lam = COMP ? seg_sum_comp<kSeg>(acc, comp) : seg_sum<PACK>(acc);
       if constexpr (BAND == 1) {
         // this band's partial sums of the event
         const float wsz = kSigma ? wave_sum(sz) : 0.0f;
@@ -593,7 +607,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         }
         continue;
       }
-      if (lane == 0) a.lambda[orig] = lam;
+      if (lane_s == 0 && live) a.lambda[orig] = lam;
       if constexpr (kSigma) {
         // weight of this event: upstream g_e or 1/lambda_e (log-likelihood)
         const float ge =
@@ -604,7 +618,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         gsdl = fma((double)ge, (double)sdl, gsdl);
         float kmax = 0.0f;
         if constexpr (MODE == MODE_FORWARD) kmax = __builtin_amdgcn_exp2f(wave_max_f(lmax));
-        if (lane == 0) {
+        if (lane_s == 0 && live) {
           // ln lambda from the hardware log2; the sum over events is in doubles
           if (lam > 0.0f) ll += (double)__builtin_amdgcn_logf(lam) * 0.69314718055994531;
           if constexpr (MODE == MODE_FORWARD) {
@@ -623,8 +637,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
       float lw;
       bool neg = false;
       if constexpr (MODE == MODE_MLEM || kNoRho2) {
-        if (!(lam > 0.0f)) continue;
-        lw = log2_shift(a.log2_scale, lam, -1.0f);
+        // packed: the lanes of an event without a positive denominator walk
+        // empty rows (below)
+        if (PACK == 1 ? !(lam > 0.0f) : !__any(lam > 0.0f)) continue;
+        lw = log2_shift(a.log2_scale, lam > 0.0f ? lam : 1.0f, -1.0f);
       } else if (a.g) {
         const float ge = a.g[orig];
         if (ge == 0.0f) continue;
@@ -641,12 +657,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           if (gg >= ngroups) break;
           RowSeg s;
           const int j = gg * 64 + lane - 64;
-          const int r = gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
+          const int r = PACK > 1 ? lane_s : gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           if (kPass1 && gi < kCachedGroups) {
             s = cache[gi];
+            if (PACK > 1 && !(lam > 0.0f)) s.len = 0;
           } else {
             s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
-            if (gg * 64 + lane < nrows)
+            if ((PACK > 1 ? lane_s : gg * 64 + lane) < nrows && (PACK == 1 || lam > 0.0f))
               s = row_setup<false>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan, cx0,
                                    cx1);
           }
@@ -747,6 +764,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
     // them held its workgroup for milliseconds on small footprints.
     gsz = wave_sum_d(gsz);
     gsdl = wave_sum_d(gsdl);
+    if constexpr (PACK > 1) ll = wave_sum_d(ll);  // held by each event's first lane
     __syncthreads();  // every wave is done with the LDS windows
     double* red = reinterpret_cast<double*>(smem);
     if (lane == 0) {
@@ -859,21 +877,48 @@ int fused_col_stride(int max_wh) {
 
 namespace {
 
-template <int MODE, bool COMP, int HP, int BAND>
-hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds) {
-  auto kern = k_strip_fused<MODE, COMP, HP, BAND>;
+// Items [i_begin, i_end) with PACK events per wave at a time.
+template <int MODE, bool COMP, int HP, int BAND, int PACK>
+hipError_t launch_range(hipStream_t st, const FusedArgs& a, int i_begin, int i_end,
+                        size_t lds) {
+  if (i_end <= i_begin) return hipSuccess;
+  auto kern = k_strip_fused<MODE, COMP, HP, BAND, PACK>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   // 2^21 items of 1024 work-items per launch keep the grid within 32 bits
   constexpr int kSlice = 1 << 21;
-  for (int i0 = 0; i0 < n_items; i0 += kSlice) {
+  for (int i0 = i_begin; i0 < i_end; i0 += kSlice) {
     FusedArgs b = a;
     b.items = a.items + i0;
     b.scal_part = a.scal_part + (size_t)i0 * kScalStride;
-    hipLaunchKernelGGL(kern, dim3((unsigned)std::min(kSlice, n_items - i0)),
+    hipLaunchKernelGGL(kern, dim3((unsigned)std::min(kSlice, i_end - i0)),
                        dim3(64 * kWaves), lds, st, b);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// pack_begin (nullable): the first item of the boxes at most 32 rows tall
+// and of those at most 16 (items are in that order); the iteration and the
+// gradient pack two or four of their events into a wave.
+template <int MODE, bool COMP, int HP, int BAND>
+hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds,
+                    const int* pack_begin = nullptr) {
+  hipError_t e;
+  constexpr bool kPackable =
+      BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K || MODE == MODE_GRAD);
+  if (kPackable && pack_begin) {
+    const int p2 = std::min(pack_begin[0], n_items), p4 = std::min(pack_begin[1], n_items);
+    if ((e = launch_range<MODE, COMP, HP, BAND, 1>(st, a, 0, p2, lds)) != hipSuccess ||
+        (e = launch_range<MODE, COMP, HP, BAND, kPackable ? 2 : 1>(st, a, p2, p4, lds)) !=
+            hipSuccess ||
+        (e = launch_range<MODE, COMP, HP, BAND, kPackable ? 4 : 1>(st, a, p4, n_items, lds)) !=
+            hipSuccess)
+      return e;
+  } else if ((e = launch_range<MODE, COMP, HP, BAND, 1>(st, a, 0, n_items, lds)) !=
+             hipSuccess) {
+    return e;
   }
   if constexpr (BAND == 0 && (MODE == MODE_FORWARD || MODE == MODE_GRAD)) {
     hipLaunchKernelGGL(k_scalars_finish, dim3(1), dim3(256), 0, st, a.scal_part,
@@ -885,12 +930,12 @@ hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds)
 
 template <int MODE, bool COMP>
 hipError_t launch_h(int col_stride, hipStream_t st, const FusedArgs& a, int n_items,
-                    size_t lds) {
+                    size_t lds, const int* pack_begin) {
   switch (col_stride) {
-    case 64: return launch_t<MODE, COMP, 64, 0>(st, a, n_items, lds);
-    case 128: return launch_t<MODE, COMP, 128, 0>(st, a, n_items, lds);
-    case 192: return launch_t<MODE, COMP, 192, 0>(st, a, n_items, lds);
-    case 256: return launch_t<MODE, COMP, 256, 0>(st, a, n_items, lds);
+    case 64: return launch_t<MODE, COMP, 64, 0>(st, a, n_items, lds, pack_begin);
+    case 128: return launch_t<MODE, COMP, 128, 0>(st, a, n_items, lds, pack_begin);
+    case 192: return launch_t<MODE, COMP, 192, 0>(st, a, n_items, lds, pack_begin);
+    case 256: return launch_t<MODE, COMP, 256, 0>(st, a, n_items, lds, pack_begin);
     default: return hipErrorInvalidValue;
   }
 }
@@ -965,22 +1010,25 @@ __global__ __launch_bounds__(256) void k_band_combine(const float* part, int64_t
 }  // namespace
 
 hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int compensated,
-                              const FusedArgs& a, int n_items, size_t lds_bytes) {
+                              const FusedArgs& a, int n_items, size_t lds_bytes,
+                              const int* pack_begin) {
   if (n_items <= 0) return hipSuccess;
+  const int* pb = pack_begin;
   switch (mode) {
     case MODE_MLEM:
-      return compensated ? launch_h<MODE_MLEM, true>(col_stride, st, a, n_items, lds_bytes)
-                         : launch_h<MODE_MLEM, false>(col_stride, st, a, n_items, lds_bytes);
+      return compensated
+                 ? launch_h<MODE_MLEM, true>(col_stride, st, a, n_items, lds_bytes, pb)
+                 : launch_h<MODE_MLEM, false>(col_stride, st, a, n_items, lds_bytes, pb);
     case MODE_FORWARD:
       return compensated
-                 ? launch_h<MODE_FORWARD, true>(col_stride, st, a, n_items, lds_bytes)
-                 : launch_h<MODE_FORWARD, false>(col_stride, st, a, n_items, lds_bytes);
+                 ? launch_h<MODE_FORWARD, true>(col_stride, st, a, n_items, lds_bytes, pb)
+                 : launch_h<MODE_FORWARD, false>(col_stride, st, a, n_items, lds_bytes, pb);
     case MODE_GRAD:
-      return launch_h<MODE_GRAD, false>(col_stride, st, a, n_items, lds_bytes);
+      return launch_h<MODE_GRAD, false>(col_stride, st, a, n_items, lds_bytes, pb);
     case MODE_MLEM_K:
-      return launch_h<MODE_MLEM_K, false>(col_stride, st, a, n_items, lds_bytes);
+      return launch_h<MODE_MLEM_K, false>(col_stride, st, a, n_items, lds_bytes, pb);
     default:
-      return launch_h<MODE_BACKWARD, false>(col_stride, st, a, n_items, lds_bytes);
+      return launch_h<MODE_BACKWARD, false>(col_stride, st, a, n_items, lds_bytes, pb);
   }
 }
 
