@@ -520,7 +520,8 @@ def main():
                      "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / VALU_PEAK_TFLOPS,
                      "traffic": measured_traffic(name),
-                     "kernel": "k_strip_fused<MLEM>", "kernel_ms": k_ms,
+                     "kernel": ("k_strip_fused<MLEM_K>" if args.accumulation == "fp32"
+                                else "k_strip_fused<MLEM>"), "kernel_ms": k_ms,
                      "flop_per_launch": st["flop_alg"]},
         "iteration_ms": {"median": float(np.median(iter_ms)),
                          "min": float(np.min(iter_ms))},
