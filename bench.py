@@ -194,25 +194,33 @@ def gradient_report(pkg, name):
 def variant_report(pkg, name, ev, no_cpu, threads):
     """The reference --gpu iteration as an option (--update gpu: the analytic
     kernel and the 5-point pixel sensitivity in the denominator,
-    cuda/reconstruction.cu:59-113) on the same events: it runs on the
-    per-event kernels (one wave per event over its box, FP64 image atomics),
-    since the analytic kernel has no row factorisation for the fused walks.
-    Iteration time, events/s and, as the checker, one iteration on a prefix
-    against the oracle's FP64 variant."""
+    cuda/reconstruction.cu:59-113) on the same events: on the fused walks
+    with the analytic kernel evaluated per pixel (the FP32 form), and for
+    comparison on the per-event kernels (one wave per event over its box,
+    FP64 image atomics).  Iteration times, events/s and, as the checker, one
+    iteration on a prefix against the oracle's FP64 variant."""
     import numpy as np
     sc = pkg.workload_scanner(name)
     kw = dict(kernel=1, update=pkg.UPDATE_SENSITIVITY_DENOMINATOR)
     out = {"workload": name, "kernel": "analytic", "denominator": "sensitivity-weighted",
-           "path": "per-event kernels, FP64 image atomics", "events": len(ev)}
-    with pkg.Reconstruction(sc, device=0, **kw) as rec:
-        rec.set_events(ev)
-        rec(1)
-        ms = []
-        for _ in range(2):
+           "events": len(ev)}
+
+    def timed(path):
+        with pkg.Reconstruction(sc, device=0, path=path, **kw) as rec:
+            rec.set_events(ev)
+            fused = rec.stats()["fused"]
             rec(1)
-            ms.append(rec.stats()["last_iteration_ms"])
-    out["ms_per_iteration"] = float(np.median(ms))
-    out["events_per_s_iter"] = len(ev) / (out["ms_per_iteration"] * 1e-3)
+            ms = []
+            for _ in range(2):
+                rec(1)
+                ms.append(rec.stats()["last_iteration_ms"])
+        return float(np.median(ms)), fused
+
+    ms, fused = timed(pkg.PATH_AUTO)
+    out["path"] = "fused walks, analytic kernel per pixel" if fused else "per-event kernels"
+    out["ms_per_iteration"] = ms
+    out["events_per_s_iter"] = len(ev) / (ms * 1e-3)
+    out["per_event_ms_per_iteration"] = timed(pkg.PATH_PER_EVENT)[0]
     if not no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as ol

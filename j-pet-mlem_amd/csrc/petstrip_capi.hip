@@ -196,7 +196,8 @@ struct petstrip_handle {
   float* d_mtab = nullptr;  // (k G(k), k^2 G(k)) pairs
   size_t gtab_n = 0;
   float s_pix = 0.0f, du = -1.0f;
-  bool fused_ok = false;
+  bool fused_ok = false;    // forward, adjoint and gradient on the fused walks
+  bool fused_iter = false;  // the iteration on the fused walks
   void* d_acc = nullptr;  // W*H accumulator, 8 bytes per pixel
 };
 
@@ -490,10 +491,19 @@ int prepare_fused(petstrip_handle* h) {
       h->cap_scal_part = need;
     }
   }
-  h->fused_ok = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
-                h->opts.update == PETSTRIP_UPDATE_CPU &&
-                h->opts.kernel == PETSTRIP_KERNEL_GAUSSIAN &&
-                h->opts.path == PETSTRIP_PATH_AUTO;
+  const bool walks = h->lds_bytes <= h->lds_limit && rows_are_intervals(h->sg) &&
+                     h->opts.path == PETSTRIP_PATH_AUTO;
+  h->fused_ok = walks && h->opts.update == PETSTRIP_UPDATE_CPU &&
+                h->opts.kernel == PETSTRIP_KERNEL_GAUSSIAN;
+  // the FP32 iteration also takes the reference --gpu variant: rho times the
+  // sensitivity in pass 1 (MLEM_K), and the analytic kernel evaluated per
+  // pixel (MLEM_KA: one window, every row group of a box cached)
+  const bool fp32 = h->opts.accumulation == PETSTRIP_ACCUM_FP32;
+  h->fused_iter =
+      h->fused_ok ||
+      (walks && fp32 &&
+       (h->opts.kernel == PETSTRIP_KERNEL_GAUSSIAN ||
+        (h->opts.kernel == PETSTRIP_KERNEL_ANALYTIC && !h->banded && h->max_rows <= 256)));
   return PETSTRIP_OK;
 }
 
@@ -546,6 +556,7 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.sidx = h->d_sidx;
   a.items = h->d_items;
   a.rho = h->d_rho;
+  a.rho_weight = nullptr;
   a.lambda = h->d_lambda;
   a.g = nullptr;
   a.acc = h->d_acc;
@@ -623,9 +634,10 @@ int grad_unit_bits(const petstrip_handle* h, int typical_bits = kGradTypicalBits
 
 int one_iteration(petstrip_handle* h) {
   const int64_t n = h->n_events;
-  if (h->fused_ok) {
+  if (h->fused_iter) {
     const int accm = h->opts.accumulation;
     petstrip::FusedArgs a = fused_args(h);
+    a.rho_weight = denominator_weight(h);  // the --gpu variant's sensitivity
     a.acc_kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
                  : accm == PETSTRIP_ACCUM_COMPENSATED ? petstrip::GLOBAL_F64
                                                        : petstrip::GLOBAL_F32;
@@ -633,7 +645,9 @@ int one_iteration(petstrip_handle* h) {
     // multiplied by rho at the end; its unit puts a typical value at
     // 2^kGradTypicalBits.  The other forms add K rho / lambda_e.
     const bool k_form = accm == PETSTRIP_ACCUM_FP32;
-    const int mode2 = k_form ? petstrip::MODE_MLEM_K : petstrip::MODE_MLEM;
+    const int mode2 = h->opts.kernel == PETSTRIP_KERNEL_ANALYTIC ? petstrip::MODE_MLEM_KA
+                      : k_form                                    ? petstrip::MODE_MLEM_K
+                                                                  : petstrip::MODE_MLEM;
     if (accm == PETSTRIP_ACCUM_FIXED64) {
       // the deterministic image sums every value in 2^-40 units
       a.log2_scale = (float)kFixed64Bits;
@@ -1054,10 +1068,10 @@ int petstrip_sensitivity(petstrip_handle* h, int mode, float* out) {
 int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
   if (!h || !out) return fail(PETSTRIP_EINVAL, "bad argument");
   *out = h->stats;
-  out->fused = h->fused_ok ? (h->banded ? (h->ncb > 1 ? 3 : 2) : 1) : 0;
+  out->fused = h->fused_iter ? (h->banded ? (h->ncb > 1 ? 3 : 2) : 1) : 0;
   out->n_work_items = h->n_items;
   out->lds_bytes = (int64_t)h->lds_bytes;
-  out->n_window_pixels = h->fused_ok ? h->window_px : 0;
+  out->n_window_pixels = h->fused_iter ? h->window_px : 0;
   return PETSTRIP_OK;
 }
 

@@ -24,7 +24,16 @@ enum { GLOBAL_F32 = 0, GLOBAL_F64 = 1, GLOBAL_U64 = 2 };
 // iteration, so sum_e K rho / lambda_e = rho sum_e K / lambda_e); MODE_MLEM
 // adds K rho / lambda_e per value (the fixed 2^-40 format of the
 // deterministic accumulation and the compensated form keep it).
-enum { MODE_MLEM = 0, MODE_FORWARD = 1, MODE_BACKWARD = 2, MODE_GRAD = 3, MODE_MLEM_K = 4 };
+// MODE_MLEM_KA: MODE_MLEM_K with the analytic kernel (the reference --gpu
+// iteration's, analytic_kernel.h:28-62) evaluated per pixel in the walks.
+enum {
+  MODE_MLEM = 0,
+  MODE_FORWARD = 1,
+  MODE_BACKWARD = 2,
+  MODE_GRAD = 3,
+  MODE_MLEM_K = 4,
+  MODE_MLEM_KA = 5
+};
 
 struct FusedArgs {
   StripConst sg;             // constants the ellipse/box support was built with
@@ -33,6 +42,8 @@ struct FusedArgs {
   const int32_t* sidx;       // original index of each sorted event
   const WorkItem* items;
   const float* rho;          // W*H image (MLEM, FORWARD)
+  const float* rho_weight;   // W*H or null: pass 1 reads rho times it (the
+                             // sensitivity-weighted denominator); MLEM_K(A) only
   float* lambda;             // per-event denominators (written by MLEM/FORWARD)
   const float* g;            // upstream weights per event, or null for 1/lambda
   void* acc;                 // W*H global accumulator (MLEM, BACKWARD)

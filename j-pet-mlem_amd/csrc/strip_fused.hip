@@ -120,12 +120,57 @@ struct RowSeg {
   float ddl_row;  // d lnK / d sigma_dl (FORWARD)
 };
 
-template <bool SIGMA>
+// The analytic kernel (AnalyticKernel::operator(), analytic_kernel.h:28-62)
+// along a box row: with pq = pq0 + k pw at the walk's k-th pixel,
+//   norm = a S a + 2 o S b = n0 + n1 pq,  b S a = a1 pq + b1,
+//   b S b = 2 isz pq^2 + c1,  K = c / sqrt(norm) exp(-(bSb - (bSa)^2/norm)/2),
+// with the row's vectors a, o and br as the per-event kernel forms them
+// (strip_kernels.hip, analytic_value).  The exponent is rational in the
+// column, so there is no geometric chain: every pixel takes a reciprocal
+// square root, a reciprocal and an exponential.
+struct RowA {
+  float pq0;  // pq at the row's first ellipse pixel
+  float n0, n1, a1, b1, c1;
+};
+
+__device__ __forceinline__ RowA analytic_row(const StripConst& sk, const EventGeom& g,
+                                             float dy, float dz) {
+  RowA r;
+  const float sec_sq = g.sec * g.sec;
+  const float yp = dy + g.y;
+  const float ap = -(yp - sk.R) * sec_sq, aq = -(yp + sk.R) * sec_sq;
+  const float ar = -2.0f * yp * g.sec * g.t;
+  const float op = ap * g.t, oq = aq * g.t;
+  const float orr = -yp * g.sec * (1.0f + 2.0f * g.t * g.t);
+  const float br = -2.0f * dy * g.sec;
+  const float aia = ap * sk.isz * ap + aq * sk.isz * aq + ar * sk.isdl * ar;
+  r.a1 = sk.isz * ap + sk.isz * aq;
+  r.b1 = br * sk.isdl * ar;
+  r.c1 = br * sk.isdl * br;
+  r.n0 = aia + 2.0f * (orr * sk.isdl * br);
+  r.n1 = 2.0f * (sk.isz * op + sk.isz * oq);
+  r.pq0 = dz - dy * g.t;
+  return r;
+}
+
+// K at pq; bq = 2 isz.
+__device__ __forceinline__ float analytic_at(float c, float bq, const RowA& r, float pq) {
+  const float norm = fmaf(r.n1, pq, r.n0);
+  const float bia = fmaf(r.a1, pq, r.b1);
+  const float bib = fmaf(bq * pq, pq, r.c1);
+  const float e = bib - bia * bia * __builtin_amdgcn_rcpf(norm);
+  return c * __builtin_amdgcn_rsqf(norm) * __builtin_amdgcn_exp2f(e * (-0.5f * 1.44269504088896341f));
+}
+
+// ANALYTIC: also the row's analytic-kernel constants into *ra.
+template <bool SIGMA, bool ANALYTIC = false>
 __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
                                             const EventGeom& g, const EventRows& ev,
                                             int iy, int row_w, int cstride, float s_pix,
-                                            bool scan, int cx0, int cx1) {
+                                            bool scan, int cx0, int cx1,
+                                            RowA* ra = nullptr) {
   RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+  if constexpr (ANALYTIC) *ra = RowA{0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   const float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
   const float Ady2 = (dy * g.A) * dy;
   const float Bdy = dy * g.B;
@@ -176,6 +221,7 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   // pq at the first pixel, with the reference's rounding (gaussian_kernel.h:64)
   const float dz = fmaf((float)a, sg.pw, sg.tl_z_half_w) - g.z;
   s.u0 = fmaf(-dy, g.t, dz) * s_pix;
+  if constexpr (ANALYTIC) *ra = analytic_row(sk, g, dy, dz);
   // kernel row factor (gaussian_kernel.h:58-78)
   const float sec_sq = g.sec * g.sec;
   const float yp = dy + g.y;
@@ -441,6 +487,80 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
   for (int gk = 1; gk < maxng; ++gk) group(gk, false);
 }
 
+// Pass 1 of the analytic kernel over one row group, in the groups of
+// forward_rows (ng groups of four ending at the row's last pixel): s0 +=
+// sum K rho (rho: the window's values), kmax = the row's largest K.  Pixels
+// before the row or past its end get K = 0 by a select (their K may not be
+// finite).
+template <int HP>
+__device__ __forceinline__ void forward_rows_analytic(const RowSeg& s, const RowA& ra, float pw,
+                                                      float c, float bq, const float* rho0,
+                                                      float& s0, float& kmax) {
+  const int ng = (s.len + kUnroll - 1) >> 2;
+  const int pad = ng * kUnroll - s.len;
+  const float* rp = s.len > 0 ? rho0 + (s.w0 - pad * HP) : rho0 - kSpareCols * HP;
+  const int lastg = max(ng - 1, 0);
+  const int maxng = uniform(wave_max_i(ng));
+  float sum = 0.0f, m = 0.0f;
+  for (int gk = 0; gk < maxng; ++gk) {
+    float rv[kUnroll];
+    read4<HP>(rp + min(gk, lastg) * (kUnroll * HP), rv);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = gk * kUnroll + u - pad;  // the pixel's index in the row
+      const float kv = analytic_at(c, bq, ra, fmaf((float)k, pw, ra.pq0));
+      const float kk = (gk < ng && k >= 0) ? kv : 0.0f;
+      sum = fmaf(kk, rv[u], sum);
+      m = fmaxf(m, kk);
+    }
+  }
+  s0 += sum;
+  kmax = fmaxf(kmax, m);
+}
+
+// Pass 2 of the analytic kernel over one row group: x = K 2^lw (the event
+// multiplier in fixed-point units, applied as two factors so that neither
+// overflows), rounded by the conversion itself (values below 2^31) or, with
+// SPLIT, carried with a further 2^-32 and converted in two halves (below
+// 2^62); the same masks as backproject_rows.
+template <int HP, bool SPLIT>
+__device__ __forceinline__ void backproject_rows_analytic(const RowSeg& s, const RowA& ra,
+                                                          float pw, float c, float bq,
+                                                          float lw,
+                                                          unsigned long long* acc_w) {
+  const int ng = (s.len + kUnroll - 1) >> 2;
+  const int pad = ng * kUnroll - s.len;
+  const int w0 = s.w0 - pad * HP;
+  const int maxng = uniform(wave_max_i(ng));
+  const float e_add = SPLIT ? lw - 32.0f : lw;
+  const float f1 = __builtin_amdgcn_exp2f(0.5f * e_add);
+  const float f2 = __builtin_amdgcn_exp2f(e_add - 0.5f * e_add);
+  unsigned lo[kUnroll], hi[kUnroll];
+  unsigned long long pv[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    pv[u] = 0ull;
+    asm volatile("" : "+v"(pv[u]));
+  }
+  for (int gk = 0; gk < maxng; ++gk) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = gk * kUnroll + u - pad;
+      const float kv = analytic_at(c, bq, ra, fmaf((float)k, pw, ra.pq0));
+      const float kk = (gk < ng && k >= 0) ? kv : 0.0f;
+      const float x = (kk * f1) * f2;
+      to_fixed<SPLIT, !SPLIT>(x, lo[u], hi[u]);
+      set_pair(pv[u], lo[u], hi[u], !SPLIT);
+    }
+    if (gk < ng) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+        if (gk > 0 || u >= pad)
+          lds_accumulate<false>(acc_w + (w0 + (gk * kUnroll + u) * HP), pv[u]);
+    }
+  }
+}
+
 }  // namespace
 
 // One workgroup of kWaves waves per work item (a chunk of events sharing a
@@ -465,15 +585,17 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
 template <int MODE, bool COMP, int HP, int BAND, int PACK>
 __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
   static_assert(PACK == 1 || (BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
-                                            MODE == MODE_GRAD)),
+                                            MODE == MODE_MLEM_KA || MODE == MODE_GRAD)),
                 "packed events: the unbanded iteration and gradient");
+  static_assert(MODE != MODE_MLEM_KA || BAND == 0, "the analytic walks are unbanded");
+  constexpr bool kAnalytic = MODE == MODE_MLEM_KA;
   constexpr int kSeg = 64 / PACK;  // lanes per event
   constexpr bool kPass1 = BAND == 1 || (BAND == 0 && MODE != MODE_BACKWARD);
   constexpr bool kPass2 = BAND == 2 || (BAND == 0 && MODE != MODE_FORWARD);
   constexpr bool kRho = kPass1 || (kPass2 && MODE == MODE_MLEM);  // rho window in LDS
   constexpr bool kSigma = kPass1 && (MODE == MODE_FORWARD || MODE == MODE_GRAD);
   // pass-2 values K / lambda_e, without rho
-  constexpr bool kNoRho2 = MODE == MODE_GRAD || MODE == MODE_MLEM_K;
+  constexpr bool kNoRho2 = MODE == MODE_GRAD || MODE == MODE_MLEM_K || MODE == MODE_MLEM_KA;
   constexpr int kThreads = 64 * kWaves;
   // pass 1's group loop unrolled four times, except with the sigma sums in
   // the two-pass kernel, where the extra registers slowed the whole kernel
@@ -509,7 +631,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
       const int c = (t - tr * ntc) * 16 + tc_lane - 16, r = tr * 4 + tr_lane;
       if (c < -kSpareCols || c >= ww) continue;
       float v = 0.0f;
-      if (r < wh && c >= 0) v = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
+      if (r < wh && c >= 0) {
+        const size_t p = (size_t)(it.wy0 + r) * sg.W + (it.wx0 + c);
+        v = a.rho[p];
+        if (kPass1 && a.rho_weight) v *= a.rho_weight[p];  // the variant's rho s
+      }
       rho_w[(c + kSpareCols) * HP + r] = v;
     }
   }
@@ -555,6 +681,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
     const int mid = (int)((sg.tl_y - g.y) * inv_ph) - ry0;
     const int m0 = nrows > 64 ? min(max(mid - 32, 0), nrows - 64) : 0;
     RowSeg cache[kCachedGroups];
+    RowA acache[kAnalytic ? kCachedGroups : 1];
 
     // ---- pass 1: denominator (and sigma sums) -------------------------------
     float lam = 0.0f;
@@ -568,13 +695,23 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           const int j = gg * 64 + lane - 64;
           const int r = PACK > 1 ? lane_s : gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+          RowA ra{0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
           if ((PACK > 1 ? lane_s : gg * 64 + lane) < nrows)
-            s = row_setup<kSigma>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan, cx0,
-                                  cx1);
-          if (gi < kCachedGroups) cache[gi] = s;
+            s = row_setup<kSigma, kAnalytic>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan,
+                                             cx0, cx1, &ra);
           float s0 = 0.0f, s1 = 0.0f;
-          forward_rows<HP, COMP, kSigma, kPass1Unroll>(s, du, gtab, mtab, rho0, s0, comp,
-                                                       s1);
+          if constexpr (kAnalytic) {
+            // the row's largest K bounds its pass-2 values (the chain's row
+            // factor lsc does so for the Gaussian)
+            float km = 0.0f;
+            forward_rows_analytic<HP>(s, ra, sg.pw, sk.c, 2.0f * sk.isz, rho0, s0, km);
+            s.lsc = km > 0.0f ? __builtin_amdgcn_logf(km) : -1e30f;
+            if (gi < kCachedGroups) acache[gi] = ra;
+          } else {
+            forward_rows<HP, COMP, kSigma, kPass1Unroll>(s, du, gtab, mtab, rho0, s0, comp,
+                                                         s1);
+          }
+          if (gi < kCachedGroups) cache[gi] = s;
           if constexpr (COMP) {
             float sum, err;
             two_sum(acc, s0, sum, err);
@@ -656,11 +793,16 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
         for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
           if (gg >= ngroups) break;
           RowSeg s;
+          RowA ra{0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
           const int j = gg * 64 + lane - 64;
           const int r = PACK > 1 ? lane_s : gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
           if (kPass1 && gi < kCachedGroups) {
             s = cache[gi];
+            if constexpr (kAnalytic) ra = acache[gi];
             if (PACK > 1 && !(lam > 0.0f)) s.len = 0;
+          } else if constexpr (kAnalytic) {
+            // the host keeps boxes within the cached groups for the analytic walks
+            s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
           } else {
             s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
             if ((PACK > 1 ? lane_s : gg * 64 + lane) < nrows && (PACK == 1 || lam > 0.0f))
@@ -686,6 +828,24 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             } else {
               backproject_rows<HP, true, false, false, false>(s, du, lw, lam, unit_scale,
                                                               gtab, rho0, acc_w);
+            }
+          } else if constexpr (kAnalytic) {
+            const float vmax = s.lsc + lw;
+            if (!__any(vmax > 30.5f)) {
+              backproject_rows_analytic<HP, false>(s, ra, sg.pw, sk.c, 2.0f * sk.isz, lw,
+                                                   acc_w);
+            } else if (!__any(vmax > 52.0f)) {
+              backproject_rows_analytic<HP, true>(s, ra, sg.pw, sk.c, 2.0f * sk.isz, lw,
+                                                  acc_w);
+            } else if (s.len > 0) {
+              const int col0 = it.wx0 + (s.w0 - (row0 + r + it.wx0 * HP)) / HP;
+              const size_t p0 = (size_t)(ry0 + r) * sg.W + col0;
+              for (int k = 0; k < s.len; ++k) {
+                const float kv = analytic_at(sk.c, 2.0f * sk.isz, ra,
+                                             fmaf((float)k, sg.pw, ra.pq0));
+                unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + p0 + k,
+                                (float)((double)kv / (double)lam));
+              }
             }
           } else if constexpr (kNoRho2) {
             // K / lambda has no bound: rows whose values stay below 2^30.5
@@ -906,8 +1066,8 @@ template <int MODE, bool COMP, int HP, int BAND>
 hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds,
                     const int* pack_begin = nullptr) {
   hipError_t e;
-  constexpr bool kPackable =
-      BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K || MODE == MODE_GRAD);
+  constexpr bool kPackable = BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
+                                           MODE == MODE_MLEM_KA || MODE == MODE_GRAD);
   if (kPackable && pack_begin) {
     const int p2 = std::min(pack_begin[0], n_items), p4 = std::min(pack_begin[1], n_items);
     if ((e = launch_range<MODE, COMP, HP, BAND, 1>(st, a, 0, p2, lds)) != hipSuccess ||
@@ -1027,6 +1187,8 @@ hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int comp
       return launch_h<MODE_GRAD, false>(col_stride, st, a, n_items, lds_bytes, pb);
     case MODE_MLEM_K:
       return launch_h<MODE_MLEM_K, false>(col_stride, st, a, n_items, lds_bytes, pb);
+    case MODE_MLEM_KA:
+      return launch_h<MODE_MLEM_KA, false>(col_stride, st, a, n_items, lds_bytes, pb);
     default:
       return launch_h<MODE_BACKWARD, false>(col_stride, st, a, n_items, lds_bytes, pb);
   }
