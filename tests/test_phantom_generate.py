@@ -1,5 +1,7 @@
 """The phantom command's generator on host threads (the device runs the same
-slot functions; tests/test_gpu_phantom.py compares the two).
+slot functions; tests/test_gpu_phantom.py compares the two, and
+tests/test_gpu_phantom_statistics.py runs these checks on device-generated
+events).
 
 Semantics pinned against the reference's rules (src/2d/geometry/phantom.h,
 rectangle.h, ellipse.h, src/2d/strip/scanner.h:107-113,195-267,
@@ -17,6 +19,14 @@ import numpy as np
 import pytest
 
 R, L, SZ, SDL = 0.43, 0.5, 0.01, 0.04
+# where the generator runs: -1 host threads (these CPU tests), or a device
+# index (the same checks on the device's events)
+DEVICE = -1
+
+
+def generate(pkg, *args, **kwargs):
+    kwargs.setdefault("device", DEVICE)
+    return pkg.phantom_generate(*args, **kwargs)
 
 
 def scanner(pkg):
@@ -40,7 +50,7 @@ def test_detected_mode_equals_event_generator(pkg):
     sc = scanner(pkg)
     regs6 = [(-0.08, 0.03, 0.03, 0.03, 0.0, 0.0), (0.0, 0.0, 0.2, 0.15, 30.0, 1.0)]
     a = pkg.phantom_events(sc, regs6, 30000, seed=9)
-    b = pkg.phantom_generate(sc, [(0,) + r for r in regs6], 30000, seed=9, detected=True)
+    b = generate(pkg, sc, [(0,) + r for r in regs6], 30000, seed=9, detected=True)
     assert b["responses"].shape == (30000, 3)
     assert np.array_equal(a.view(np.uint32), b["responses"].view(np.uint32))
     assert b["n_emitted"] >= 30000
@@ -55,12 +65,12 @@ def test_region_weights_and_kinds(pkg):
             point(0.0, 0.08, 0.002)]
     w = np.array([2.0 * math.pi * 0.03 * 0.015, 3.0 * 4 * 0.02 * 0.01, 0.002])
     n = 400000
-    res = pkg.phantom_generate(sc, regs, n, seed=3, full=True)
+    res = generate(pkg, sc, regs, n, seed=3, full=True)
     ev = res["events"].astype(np.float64)
     # emission mode: the events returned are the detected ones; detection
     # depends on the point, so compare weights on the emitted image instead
     g = pkg.phantom_grid(256, 256, 0.001)
-    img = pkg.phantom_generate(sc, regs, n, seed=3, grid=g, images=True)["emitted"]
+    img = generate(pkg, sc, regs, n, seed=3, grid=g, images=True)["emitted"]
     assert img.sum() == n  # every emission lies inside the grid
     yy, xx = np.mgrid[0:256, 0:256]
     cx, cy = -0.128 + (xx + 0.5) * 0.001, -0.128 + (yy + 0.5) * 0.001
@@ -96,8 +106,8 @@ def test_non_additive_rejection_and_additive_overlap(pkg):
     cx, cy = -0.128 + (xx + 0.5) * 0.001, -0.128 + (yy + 0.5) * 0.001
     disc = np.hypot(cx, cy) < 0.028
     n = 200000
-    a = pkg.phantom_generate(sc, regs, n, seed=4, grid=g, images=True)["emitted"]
-    b = pkg.phantom_generate(sc, regs, n, seed=4, grid=g, images=True, additive=True)["emitted"]
+    a = generate(pkg, sc, regs, n, seed=4, grid=g, images=True)["emitted"]
+    b = generate(pkg, sc, regs, n, seed=4, grid=g, images=True, additive=True)["emitted"]
     assert a[disc].sum() == 0
     # additive: the background's share of the disc (area ratio)
     frac = b[disc].sum() / n
@@ -112,7 +122,7 @@ def test_detection_rule_and_noise(pkg):
     sc = scanner(pkg)
     regs = [ell(0.0, 0.0, 0.15, 0.1, 10.0, 1.0)]
     n = 300000
-    res = pkg.phantom_generate(sc, regs, n, seed=5, full=True)
+    res = generate(pkg, sc, regs, n, seed=5, full=True)
     assert res["n_emitted"] == n
     ev = res["events"].astype(np.float64)
     t = ev[:, 2] / ev[:, 3]
@@ -146,7 +156,7 @@ def test_images_and_tan_bins(pkg):
     sc = scanner(pkg)
     regs = [ell(0.01, -0.02, 0.06, 0.04, 0.0, 1.0), point(-0.0513, 0.0517, 0.01)]
     g = pkg.phantom_grid(64, 48, 0.004, origin=(0.002, -0.004), tan_bins=8, max_tan=0.6)
-    res = pkg.phantom_generate(sc, regs, 100000, seed=6, grid=g, images=True, full=True)
+    res = generate(pkg, sc, regs, 100000, seed=6, grid=g, images=True, full=True)
     ev = res["events"].astype(np.float64)
 
     def hist(x, y):
@@ -179,11 +189,11 @@ def test_images_and_tan_bins(pkg):
 def test_bad_arguments(pkg):
     sc = scanner(pkg)
     with pytest.raises(pkg.PetstripError):
-        pkg.phantom_generate(sc, [(7, 0, 0, 0.1, 0.1, 0, 1)], 10)
+        generate(pkg, sc, [(7, 0, 0, 0.1, 0.1, 0, 1)], 10)
     with pytest.raises(pkg.PetstripError):
-        pkg.phantom_generate(sc, [rect(0, 0, -0.1, 0.1, 1)], 10)
+        generate(pkg, sc, [rect(0, 0, -0.1, 0.1, 1)], 10)
     with pytest.raises(pkg.PetstripError):
-        pkg.phantom_generate(sc, [point(0, 0, 0.0)], 10)
+        generate(pkg, sc, [point(0, 0, 0.0)], 10)
 
 
 def test_expected_image_matches_generated_emissions(pkg):
@@ -196,8 +206,8 @@ def test_expected_image_matches_generated_emissions(pkg):
     regions = pkg.workloads.shepp6()
     sc = pkg.Scanner(0.43, 0.5, 40, 40, 0.016, 0.016, 0.01, 0.04)
     n = 400_000
-    res = pkg.phantom_generate(sc, [(0,) + tuple(r) for r in regions], n, seed=17,
-                               detected=True, full=True, device=-1)
+    res = generate(pkg, sc, [(0,) + tuple(r) for r in regions], n, seed=17,
+                   detected=True, full=True)
     pts = res["events"][:, :2].astype(np.float64)
     col = np.floor((pts[:, 0] + 0.32) / 0.016).astype(int)
     row = np.floor((0.32 - pts[:, 1]) / 0.016).astype(int)
