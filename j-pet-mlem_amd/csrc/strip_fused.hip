@@ -162,22 +162,18 @@ __device__ __forceinline__ float analytic_at(float c, float bq, const RowA& r, f
   return c * __builtin_amdgcn_rsqf(norm) * __builtin_amdgcn_exp2f(e * (-0.5f * 1.44269504088896341f));
 }
 
-// ANALYTIC: also the row's analytic-kernel constants into *ra.
-template <bool SIGMA, bool ANALYTIC = false>
-__device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
-                                            const EventGeom& g, const EventRows& ev,
-                                            int iy, int row_w, int cstride, float s_pix,
-                                            bool scan, int cx0, int cx1,
-                                            RowA* ra = nullptr) {
-  RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
-  if constexpr (ANALYTIC) *ra = RowA{0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  const float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
+// The row's ellipse columns [a, b] under the reference's float test
+// (in_ellipse, gaussian_kernel.h:131-137), within the event's box; false for
+// an empty row.  dy: the row centre's offset from the ellipse centre.
+__device__ __forceinline__ bool row_interval(const StripConst& sg, const EventGeom& g,
+                                             const EventRows& ev, float dy, bool scan,
+                                             int& a_out, int& b_out) {
   const float Ady2 = (dy * g.A) * dy;
   const float Bdy = dy * g.B;
   // C dz^2 + Bdy dz + Ady2 - 9: discriminant at the threshold; at 9.05 it
   // bounds what the float test can accept (its rounding is ~1e-6)
   const float d9 = fmaf(Bdy, Bdy, -ev.c4 * (Ady2 - 9.0f));
-  if (!(d9 + ev.c005 >= 0.0f)) return s;
+  if (!(d9 + ev.c005 >= 0.0f)) return false;
   const float sq = __builtin_amdgcn_sqrtf(fmaxf(d9, 0.0f));
   const float clo = fmaf(-(Bdy + sq), ev.kcol, ev.zc);  // first column, estimated
   const float chi = fmaf(sq - Bdy, ev.kcol, ev.zc);     // last column, estimated
@@ -210,11 +206,44 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
 #pragma nounroll
     while (b >= a && !exact_in(sg, g, Ady2, Bdy, b)) --b;
   } else if (none) {
+    return false;
+  }
+  a_out = max(a, g.x0);
+  b_out = min(b, g.x1 - 1);
+  return a_out <= b_out;
+}
+
+// Packed row interval of the precomputed table: (columns << 16) | (first
+// column - the box's first column); 0 for an empty row.
+__device__ __forceinline__ uint32_t pack_interval(const EventGeom& g, bool any, int a, int b) {
+  return any ? ((uint32_t)(b - a + 1) << 16) | (uint32_t)(a - g.x0) : 0u;
+}
+
+// ANALYTIC: also the row's analytic-kernel constants into *ra.  iv
+// (nullable): the row's precomputed interval (k_row_intervals), in place of
+// the boundary search.
+template <bool SIGMA, bool ANALYTIC = false>
+__device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
+                                            const EventGeom& g, const EventRows& ev,
+                                            int iy, int row_w, int cstride, float s_pix,
+                                            bool scan, int cx0, int cx1,
+                                            RowA* ra = nullptr,
+                                            const uint32_t* iv = nullptr) {
+  RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+  if constexpr (ANALYTIC) *ra = RowA{0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  const float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
+  int a, b;
+  if (iv) {
+    const uint32_t p = *iv;
+    if (p == 0u) return s;
+    a = g.x0 + (int)(p & 0xffffu);
+    b = a + (int)(p >> 16) - 1;
+  } else if (!row_interval(sg, g, ev, dy, scan, a, b)) {
     return s;
   }
-  // the box, and the work item's columns [cx0, cx1) in the column-banded form
-  a = max(a, max(g.x0, cx0));
-  b = min(b, min(g.x1, cx1) - 1);
+  // the work item's columns [cx0, cx1) in the column-banded form
+  a = max(a, cx0);
+  b = min(b, cx1 - 1);
   if (a > b) return s;
   s.len = b - a + 1;
   s.w0 = row_w + a * cstride;

@@ -103,3 +103,27 @@ def test_variant_fused_tall_boxes_vs_oracle(gpu_pkg, kernel):
             print(f"{n_pix}x{n_pix} @ {size} m, kernel {kernel}, iteration {i + 1}: rel L2 "
                   f"vs FP64 {e:.2e}", flush=True)
             assert e < 1e-5
+
+
+@pytest.mark.parametrize("force_bands", [1, 2])
+def test_variant_gaussian_banded_vs_oracle(gpu_pkg, force_bands):
+    """The Gaussian kernel with the sensitivity-weighted denominator on the
+    banded walks (rho times the sensitivity in the band's pass 1, the
+    band-combined denominators in pass 2), row bands and row x column
+    bands, against the FP64 oracle variant."""
+    pkg = gpu_pkg
+    sc, ev = small(pkg)
+    with pkg.Reconstruction(sc, update=pkg.UPDATE_SENSITIVITY_DENOMINATOR, kernel=0,
+                            force_bands=force_bands) as rec:
+        rec.set_events(ev)
+        assert rec.stats()["fused"] == 1 + force_bands
+        imgs = []
+        for _ in range(2):
+            rec(1)
+            imgs.append(rec.rho)
+    osens = ol.sensitivity_f32(sc, 1).astype(np.float64)
+    ref = ol.mlem_variant_f64(sc, ev, 2, kernel=0, sens=osens, n_threads=16)
+    for i in range(2):
+        e = rel_l2(imgs[i], ref[i])
+        print(f"bands {force_bands}, iteration {i + 1}: rel L2 vs FP64 {e:.2e}", flush=True)
+        assert e < 1e-5
