@@ -327,6 +327,9 @@ def parse():
                    help="3D hybrid workload measured beside the headline line "
                         "('none' skips it)")
     p.add_argument("--no-grad", action="store_true")
+    p.add_argument("--no-row-table", action="store_true",
+                   help="form every box row in every iteration instead of reading "
+                        "the row table (A/B of the table)")
     p.add_argument("--no-variant", action="store_true",
                    help="skip the reference --gpu variant (analytic kernel) line")
     p.add_argument("--grad-workload", default="strip-256px-2mm-5e6ev",
@@ -491,7 +494,8 @@ def main():
     t_gen = time.perf_counter() - t0
     acc = {"fp32": pkg.ACCUM_FP32, "compensated": pkg.ACCUM_COMPENSATED,
            "fixed64": pkg.ACCUM_FIXED64}[args.accumulation]
-    rec = pkg.Reconstruction(sc, device=0, accumulation=acc)
+    rec = pkg.Reconstruction(sc, device=0, accumulation=acc,
+                             row_table=not args.no_row_table)
     t0 = time.perf_counter()
     rec.set_events(ev)
     t_setup = time.perf_counter() - t0
@@ -533,6 +537,7 @@ def main():
                      "flop_per_launch": st["flop_alg"]},
         "iteration_ms": {"median": float(np.median(iter_ms)),
                          "min": float(np.min(iter_ms))},
+        "row_table_bytes": st["row_table_bytes"],
         "footprint": {"box_px_per_event": st["n_box_pixels"] / max(n_ev, 1),
                       "ellipse_px_per_event": st["n_ellipse_pixels"] / max(n_ev, 1)},
         "setup_s": {"generate": t_gen, "upload_and_bin": t_setup},

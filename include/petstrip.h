@@ -109,7 +109,13 @@ typedef struct petstrip_options {
                            of 64 rows x 128 columns (the form used when a
                            row band does not fit either)                   */
   int32_t kernel;       /* PETSTRIP_KERNEL_*                               */
-  int32_t reserved[2];  /* zero                                            */
+  int32_t row_table;    /* 0 = the binned walks keep each event's box rows
+                           (ellipse columns, kernel row factor) in a device
+                           table built once per event set when it fits in
+                           a quarter of free device memory; 1 = no table,
+                           every iteration forms the rows again (identical
+                           results)                                      */
+  int32_t reserved;     /* zero                                            */
 } petstrip_options;
 
 typedef struct petstrip_handle petstrip_handle;
@@ -134,6 +140,7 @@ typedef struct petstrip_stats {
                                 pixel); every ellipse pixel's update is an
                                 LDS atomic.  0 for the per-event kernels,
                                 whose updates are all global atomics     */
+  int64_t row_table_bytes;   /* device bytes of the row table (0: none)   */
 } petstrip_stats;
 
 /* Called after an output is ready.  iteration >= 1 for images, -1 for the

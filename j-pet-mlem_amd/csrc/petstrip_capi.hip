@@ -199,6 +199,15 @@ struct petstrip_handle {
   bool fused_ok = false;    // forward, adjoint and gradient on the fused walks
   bool fused_iter = false;  // the iteration on the fused walks
   void* d_acc = nullptr;  // W*H accumulator, 8 bytes per pixel
+  // the row table of the unbanded walks (per sorted event rowtab_stride
+  // entries): built when events or the support change, its kernel values
+  // rebuilt when the kernel constants change
+  petstrip::RowEntry* d_rowtab = nullptr;
+  size_t cap_rowtab = 0;  // entries
+  int64_t rowtab_stride = 0;
+  bool rowtab_on = false;
+  StripConst rowtab_sk{};  // kernel constants of its u0, lsc and b1
+  float rowtab_spix = -1.0f, rowtab_du = -1.0f;
 };
 
 namespace {
@@ -224,6 +233,7 @@ int reserve_events(petstrip_handle* h, int64_t n) {
   dfree(h->d_sgeom);
   dfree(h->d_sidx);
   h->cap_events = 0;
+  h->rowtab_on = false;
   size_t c = (size_t)std::max<int64_t>(n, 1);
   int rc;
   if ((rc = dalloc(&h->d_resp, 3 * c))) return rc;
@@ -313,6 +323,47 @@ int update_kernel_table(petstrip_handle* h) {
   HIP_TRY(hipMemcpy(h->d_mtab, m.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice));
   h->s_pix = s_pix;
   h->du = du;
+  return PETSTRIP_OK;
+}
+
+bool rowtab_kernel_ok(const petstrip_handle* h) {
+  return h->rowtab_on && h->rowtab_spix == h->s_pix && h->rowtab_du == h->du &&
+         memcmp(&h->rowtab_sk, &h->sk, sizeof(StripConst)) == 0;
+}
+
+// (Re)computes the row table for the current support, kernel constants and
+// binned events.  The unbanded walks then read each box row's ellipse
+// columns, u0, row factor and chain ratio instead of forming them every
+// iteration: 16 bytes per (event, row) against ~100 VALU instructions per
+// row group.  Kept off in the banded form (its rows are clipped to bands),
+// with options.row_table = 1, for images wider than the entries' 16-bit
+// columns, and when the table would take more than a quarter of the free
+// device memory.
+int build_row_table(petstrip_handle* h) {
+  h->rowtab_on = false;
+  if (h->banded || !h->fused_iter || h->opts.row_table != 0 || h->n_sorted == 0 ||
+      h->sg.W > 65535)
+    return PETSTRIP_OK;
+  // slots of whole row groups: a wave's loads of a group stay in its event
+  const int64_t stride = ((int64_t)h->max_rows + 63) & ~(int64_t)63;
+  const size_t need = (size_t)h->n_sorted * (size_t)stride;
+  if (need > h->cap_rowtab) {
+    dfree(h->d_rowtab);
+    h->cap_rowtab = 0;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (need * sizeof(petstrip::RowEntry) > free_b / 4) return PETSTRIP_OK;
+    int rc;
+    if ((rc = dalloc(&h->d_rowtab, need))) return rc;
+    h->cap_rowtab = need;
+  }
+  h->rowtab_stride = stride;
+  HIP_TRY(petstrip::launch_row_table(h->stream, h->sg, h->sk, h->s_pix, h->du, h->d_sgeom,
+                                     h->n_sorted, h->opts.row_scan, stride, h->d_rowtab));
+  h->rowtab_on = true;
+  h->rowtab_sk = h->sk;
+  h->rowtab_spix = h->s_pix;
+  h->rowtab_du = h->du;
   return PETSTRIP_OK;
 }
 
@@ -504,7 +555,7 @@ int prepare_fused(petstrip_handle* h) {
       (walks && fp32 &&
        (h->opts.kernel == PETSTRIP_KERNEL_GAUSSIAN ||
         (h->opts.kernel == PETSTRIP_KERNEL_ANALYTIC && !h->banded && h->max_rows <= 256)));
-  return PETSTRIP_OK;
+  return build_row_table(h);
 }
 
 // Geometry and footprint statistics for the responses in d_resp.
@@ -576,6 +627,11 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.ncb = h->ncb;
   a.cshift = h->cshift;
   a.lam_sorted = h->d_lam_sorted;
+  if (h->rowtab_on) {
+    a.rowtab = h->d_rowtab;
+    a.rowtab_stride = h->rowtab_stride;
+    a.rowtab_kernel = rowtab_kernel_ok(h) ? 1 : 0;
+  }
   return a;
 }
 
@@ -636,6 +692,9 @@ int one_iteration(petstrip_handle* h) {
   const int64_t n = h->n_events;
   if (h->fused_iter) {
     const int accm = h->opts.accumulation;
+    // the table's kernel values after a change of the kernel's sigmas
+    int rc0;
+    if (h->rowtab_on && !rowtab_kernel_ok(h) && (rc0 = build_row_table(h))) return rc0;
     petstrip::FusedArgs a = fused_args(h);
     a.rho_weight = denominator_weight(h);  // the --gpu variant's sensitivity
     a.acc_kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
@@ -733,7 +792,7 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
                opts->update < 0 || opts->update > 1 || opts->path < 0 ||
                opts->path > 1 || opts->row_scan < 0 || opts->row_scan > 1 ||
                opts->force_bands < 0 || opts->force_bands > 2 || opts->kernel < 0 ||
-               opts->kernel > 1))
+               opts->kernel > 1 || opts->row_table < 0 || opts->row_table > 1))
     return fail(PETSTRIP_EINVAL, "invalid options");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -826,6 +885,7 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_mtab);
   dfree(h->d_part);
   dfree(h->d_lam_sorted);
+  dfree(h->d_rowtab);
   if (h->d_acc) (void)hipFree(h->d_acc);
   h->d_acc = nullptr;
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -858,7 +918,7 @@ int petstrip_set_options(petstrip_handle* h, const petstrip_options* opts) {
   if (opts->accumulation < 0 || opts->accumulation > 2 || opts->update < 0 ||
       opts->update > 1 || opts->path < 0 || opts->path > 1 || opts->row_scan < 0 ||
       opts->row_scan > 1 || opts->force_bands < 0 || opts->force_bands > 2 ||
-      opts->kernel < 0 || opts->kernel > 1)
+      opts->kernel < 0 || opts->kernel > 1 || opts->row_table < 0 || opts->row_table > 1)
     return fail(PETSTRIP_EINVAL, "invalid options");
   h->opts = *opts;
   h->lambda_valid = false;
@@ -1072,6 +1132,9 @@ int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
   out->n_work_items = h->n_items;
   out->lds_bytes = (int64_t)h->lds_bytes;
   out->n_window_pixels = h->fused_iter ? h->window_px : 0;
+  out->row_table_bytes = h->rowtab_on ? (int64_t)(h->n_sorted * h->rowtab_stride *
+                                                  (int64_t)sizeof(petstrip::RowEntry))
+                                      : 0;
   return PETSTRIP_OK;
 }
 

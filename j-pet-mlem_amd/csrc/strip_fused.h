@@ -18,6 +18,16 @@ struct WorkItem {
   int32_t pad;         // banded items: the row band index (rows / kBandRows)
 };
 
+// A box row's walk in the row table: iv = (ellipse columns << 16) | (first
+// ellipse column - the box's first column), 0 for a row without ellipse
+// pixels; u0 (u at the first ellipse column), lsc (log2 of the row factor)
+// and b1 (the chain's per-pixel ratio) for the kernel constants the table
+// was built with.
+struct __align__(16) RowEntry {
+  uint32_t iv;
+  float u0, lsc, b1;
+};
+
 enum { GLOBAL_F32 = 0, GLOBAL_F64 = 1, GLOBAL_U64 = 2 };
 // MODE_MLEM_K: the MLEM iteration with pass-2 values K / lambda_e (no rho),
 // the image then multiplied by rho per pixel (rho is constant within an
@@ -52,6 +62,11 @@ struct FusedArgs {
   int split;                 // BACKWARD: values up to 2^62 (else below 2^31)
   double inv_fixed_scale;    // 2^-k, the unit when flushing to floats
   int row_scan;              // 1: every row interval by the scanning path
+  const RowEntry* rowtab;    // unbanded walks: the row table (per sorted event
+                             // rowtab_stride entries, a multiple of 64, in the
+                             // walks' row order: k_row_table), or null
+  int64_t rowtab_stride;
+  int rowtab_kernel;         // 1: the entries' u0, lsc, b1 hold for sk and du
   float s_pix;               // sqrt(sk.isz log2 e): u = pq s_pix
   float du;                  // sg.pw s_pix, the u step per column
   const float* gtab;         // G(k) = 2^(-(k du)^2) for k < the image width + 8
@@ -97,6 +112,12 @@ constexpr int kMaxColStride = 256;   // largest LDS column stride (window height
 // path for every row.
 hipError_t launch_row_counts(hipStream_t st, const StripConst& sg, const EventGeom* geom,
                              int64_t n, int scan, int32_t* nbox, int32_t* nell);
+
+// The row table for n sorted events, stride entries per event (a multiple
+// of 64 at least the tallest box); scan as for launch_row_counts.
+hipError_t launch_row_table(hipStream_t st, const StripConst& sg, const StripConst& sk,
+                            float s_pix, float du, const EventGeom* sgeom, int64_t n, int scan,
+                            int64_t stride, RowEntry* tab);
 
 // LDS bytes per workgroup for windows of at most max_ww columns with column
 // stride col_stride; band = 2 (pass 2 of the banded form) holds the rho

@@ -110,12 +110,17 @@ __device__ __forceinline__ EventRows event_rows(const StripConst& sg, const Even
   return ev;
 }
 
+constexpr int kUnroll = 4;
+constexpr int kSpareCols = kUnroll;  // zero rho window columns before the window
+constexpr int kCachedGroups = 4;  // row groups kept in registers between passes
+
 // One box row of an event as a lane walks it.
 struct RowSeg {
   int len;        // ellipse pixels in the row (0: nothing to do)
   int w0;         // LDS window index of the first ellipse pixel
   float u0;       // u of the first ellipse pixel
   float lsc;      // log2 of the row factor
+  float b1;       // 2^(-2 u du) at the walk's first pixel: the chain's ratio
   float dz_row;   // row part of d lnK / d sigma_z (FORWARD)
   float ddl_row;  // d lnK / d sigma_dl (FORWARD)
 };
@@ -160,6 +165,15 @@ __device__ __forceinline__ float analytic_at(float c, float bq, const RowA& r, f
   const float bib = fmaf(bq * pq, pq, r.c1);
   const float e = bib - bia * bia * __builtin_amdgcn_rcpf(norm);
   return c * __builtin_amdgcn_rsqf(norm) * __builtin_amdgcn_exp2f(e * (-0.5f * 1.44269504088896341f));
+}
+
+// The first box row of an event's row group 0 (which takes the 64 rows
+// around the ellipse centre; later groups the rows above and below it, in
+// order) for a box of nrows rows from row ry0.
+__device__ __forceinline__ int group0_first_row(const StripConst& sg, const EventGeom& g,
+                                                int ry0, int nrows, float inv_ph) {
+  const int mid = (int)((sg.tl_y - g.y) * inv_ph) - ry0;
+  return nrows > 64 ? min(max(mid - 32, 0), nrows - 64) : 0;
 }
 
 // The row's ellipse columns [a, b] under the reference's float test
@@ -219,25 +233,37 @@ __device__ __forceinline__ uint32_t pack_interval(const EventGeom& g, bool any, 
   return any ? ((uint32_t)(b - a + 1) << 16) | (uint32_t)(a - g.x0) : 0u;
 }
 
-// ANALYTIC: also the row's analytic-kernel constants into *ra.  iv
-// (nullable): the row's precomputed interval (k_row_intervals), in place of
-// the boundary search.
+// The chain's per-pixel ratio b1 = 2^(-2 u du) at the first pixel of the
+// walk, which starts pad = 4 ceil(len / 4) - len pixels before the row.
+__device__ __forceinline__ float row_ratio(float u0, int len, float du) {
+  const int pad = ((len + kUnroll - 1) & ~(kUnroll - 1)) - len;
+  const float u = fmaf(-(float)pad, du, u0);
+  return __builtin_amdgcn_exp2f(-2.0f * u * du);
+}
+
+// The row's walk: its ellipse columns (clipped to the work item's [cx0,
+// cx1)), u0 and the row factor at the first one, the chain ratio b1 and,
+// SIGMA, the row's sigma derivatives; ANALYTIC: also the row's
+// analytic-kernel constants into *ra.  use_tab (wave-uniform): t is the
+// row's entry of the row table, whose columns replace the boundary search
+// and, with tab_kernel (entries built for sk, s_pix and du; unclipped rows),
+// whose u0, lsc and b1 are the row's.
 template <bool SIGMA, bool ANALYTIC = false>
 __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripConst& sk,
                                             const EventGeom& g, const EventRows& ev,
                                             int iy, int row_w, int cstride, float s_pix,
-                                            bool scan, int cx0, int cx1,
-                                            RowA* ra = nullptr,
-                                            const uint32_t* iv = nullptr) {
-  RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+                                            float du, bool scan, int cx0, int cx1,
+                                            RowA* ra = nullptr, bool use_tab = false,
+                                            const RowEntry& t = RowEntry{0u, 0.0f, 0.0f, 0.0f},
+                                            bool tab_kernel = false) {
+  RowSeg s{0, 0, 0.0f, -1e30f, 1.0f, 0.0f, 0.0f};
   if constexpr (ANALYTIC) *ra = RowA{0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   const float dy = fmaf(-(float)iy, sg.ph, sg.tl_y_half_h) - g.y;
   int a, b;
-  if (iv) {
-    const uint32_t p = *iv;
-    if (p == 0u) return s;
-    a = g.x0 + (int)(p & 0xffffu);
-    b = a + (int)(p >> 16) - 1;
+  if (use_tab) {
+    if (t.iv == 0u) return s;
+    a = g.x0 + (int)(t.iv & 0xffffu);
+    b = a + (int)(t.iv >> 16) - 1;
   } else if (!row_interval(sg, g, ev, dy, scan, a, b)) {
     return s;
   }
@@ -247,6 +273,12 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   if (a > b) return s;
   s.len = b - a + 1;
   s.w0 = row_w + a * cstride;
+  if (!SIGMA && !ANALYTIC && use_tab && tab_kernel) {
+    s.u0 = t.u0;
+    s.lsc = t.lsc;
+    s.b1 = t.b1;
+    return s;
+  }
   // pq at the first pixel, with the reference's rounding (gaussian_kernel.h:64)
   const float dz = fmaf((float)a, sg.pw, sg.tl_z_half_w) - g.z;
   s.u0 = fmaf(-dy, g.t, dz) * s_pix;
@@ -262,6 +294,7 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   const float brb = (br * sk.isdl) * br;
   s.lsc = __builtin_amdgcn_logf(sk.c) - 0.5f * __builtin_amdgcn_logf(norm) -
           (0.5f * kLog2e) * brb;
+  if constexpr (!ANALYTIC) s.b1 = row_ratio(s.u0, s.len, du);
   if constexpr (SIGMA) {
     // ln K = ln c - ln(norm)/2 - bSb/2 with c ~ 1/(sz^2 sdl), norm and bSb
     // linear in 1/sz^2 and 1/sdl^2:
@@ -275,10 +308,6 @@ __device__ __forceinline__ RowSeg row_setup(const StripConst& sg, const StripCon
   }
   return s;
 }
-
-constexpr int kUnroll = 4;
-constexpr int kSpareCols = kUnroll;  // zero rho window columns before the window
-constexpr int kCachedGroups = 4;  // row groups kept in registers between passes
 
 // k + sgn * log2(x) for x > 0 and integer k, formed as (k + sgn e) + sgn
 // log2(m) with x = m 2^e: scaling x by a power of two while k moves the
@@ -304,13 +333,12 @@ struct Chain {
   float b4;
 };
 
-__device__ __forceinline__ Chain chain_start(float u0, float lsc, float du, float e_add) {
-  // two exponentials: the first pixel's value and the per-pixel ratio b1;
-  // the other starts and b4 = b1^4 by multiplication
+// u0: u at the walk's first pixel; b1: the per-pixel ratio (row_ratio).
+__device__ __forceinline__ Chain chain_start(float u0, float lsc, float b1, float e_add) {
+  // one exponential, the first pixel's value; the other starts and b4 =
+  // b1^4 by multiplication
   Chain c;
-  const float e1 = -2.0f * u0 * du;
   const float e0 = fmaf(-u0, u0, lsc) + e_add;
-  const float b1 = __builtin_amdgcn_exp2f(e1);
   c.p[0] = __builtin_amdgcn_exp2f(e0);
 #pragma unroll
   for (int u = 1; u < kUnroll; ++u) c.p[u] = c.p[u - 1] * b1;
@@ -354,7 +382,7 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* 
   const int lastg = max(ng - 1, 0);
   const int maxng = uniform(wave_max_i(ng));
   float sum0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
-  Chain c = chain_start(u0, s.lsc, du, 0.0f);
+  Chain c = chain_start(u0, s.lsc, s.b1, 0.0f);
 #pragma unroll
   for (int u = 0; u < kUnroll; ++u) c.p[u] = ng > 0 ? c.p[u] : 0.0f;
   auto group = [&](int gk, bool first) {
@@ -486,7 +514,7 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
     pv[u] = 0ull;
     asm volatile("" : "+v"(pv[u]));
   }
-  Chain c = chain_start(u0, s.lsc, du, e_add);
+  Chain c = chain_start(u0, s.lsc, s.b1, e_add);
   auto group = [&](int gk, bool first) {
     float rv[kUnroll] = {0.0f, 0.0f, 0.0f, 0.0f};
     if constexpr (RHO) read4<HP>(rp + min(gk, lastg) * (kUnroll * HP), rv);
@@ -612,7 +640,8 @@ __device__ __forceinline__ void backproject_rows_analytic(const RowSeg& s, const
 // that short events do not leave most lanes idle; the denominator is then a
 // segment sum and every per-event value lives per lane.
 template <int MODE, bool COMP, int HP, int BAND, int PACK>
-__global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(PACK > 1 ? 8 : 1)))
+void k_strip_fused(FusedArgs a) {
   static_assert(PACK == 1 || (BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
                                             MODE == MODE_MLEM_KA || MODE == MODE_GRAD)),
                 "packed events: the unbanded iteration and gradient");
@@ -675,6 +704,24 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
 
   const float s_pix = a.s_pix;  // sqrt(isz log2 e)
   const bool scan = a.row_scan != 0;
+  // The row table (unbanded walks): a lane's entry for row group gg of
+  // sorted event e is e * stride + gg * 64 + its lane (its row in the event
+  // when packed), in the order the walks visit the rows.  The entries of
+  // groups 0 and 1 (when the stride has them) of the wave's next events load
+  // while the current ones are walked.
+  const bool use_tab = BAND == 0 && a.rowtab != nullptr;
+  const bool tab_kernel = a.rowtab_kernel != 0;
+  constexpr int kPre = PACK == 1 ? 2 : 1;
+  const bool pre1 = kPre > 1 && a.rowtab_stride > 64;
+  RowEntry pre[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) pre[k] = RowEntry{0u, 0.0f, 0.0f, 0.0f};
+  auto prefetch = [&](int e0n) {
+    const size_t base = (size_t)min(e0n + seg, it.end - 1) * a.rowtab_stride + lane_s;
+    pre[0] = a.rowtab[base];
+    if (pre1) pre[kPre - 1] = a.rowtab[base + 64];
+  };
+  if (use_tab && it.begin + wave * PACK < it.end) prefetch(it.begin + wave * PACK);
   const float du = a.du;        // u step per column, pw s_pix
   ConstF* gtab = (ConstF*)a.gtab;  // uniform reads through the scalar cache
   ConstF* mtab = (ConstF*)a.mtab;
@@ -704,11 +751,22 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
     const int ngroups = PACK > 1 ? 1 : (nrows + 63) >> 6;
     const int row0 = (ry0 - it.wy0) - it.wx0 * HP;  // window index of (ry0, 0)
     const int orig = a.sidx[live ? e : e0];
-    const EventRows ev = event_rows(sg, g, inv_pw);
-    // group 0 takes the 64 rows around the ellipse centre, later groups the
-    // rows above and below it in order
-    const int mid = (int)((sg.tl_y - g.y) * inv_ph) - ry0;
-    const int m0 = nrows > 64 ? min(max(mid - 32, 0), nrows - 64) : 0;
+    // this event's entries of groups 0 and 1 (loaded with the last one);
+    // those of the next event start loading now
+    RowEntry cur[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) cur[k] = pre[k];
+    if (use_tab && e0 + kWaves * PACK < it.end) prefetch(e0 + kWaves * PACK);
+    // without the table: the per-event constants of the boundary search
+    EventRows ev{0.0f, 0.0f, 0.0f, 0.0f};
+    if (!use_tab) ev = event_rows(sg, g, inv_pw);
+    // the lane's entry of row group gg (gi: its unrolled index)
+    auto entry = [&](int gi, int gg) -> RowEntry {
+      if (!use_tab) return RowEntry{0u, 0.0f, 0.0f, 0.0f};
+      if (gi < kPre && (gi == 0 || pre1)) return cur[gi < kPre ? gi : 0];
+      return a.rowtab[(size_t)(live ? e : e0) * a.rowtab_stride + gg * 64 + lane_s];
+    };
+    const int m0 = group0_first_row(sg, g, ry0, nrows, inv_ph);
     RowSeg cache[kCachedGroups];
     RowA acache[kAnalytic ? kCachedGroups : 1];
 
@@ -723,11 +781,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
           if (gg >= ngroups) break;
           const int j = gg * 64 + lane - 64;
           const int r = PACK > 1 ? lane_s : gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
-          RowSeg s{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+          RowSeg s{0, 0, 0.0f, -1e30f, 1.0f, 0.0f, 0.0f};
           RowA ra{0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
           if ((PACK > 1 ? lane_s : gg * 64 + lane) < nrows)
-            s = row_setup<kSigma, kAnalytic>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan,
-                                             cx0, cx1, &ra);
+            s = row_setup<kSigma, kAnalytic>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, du,
+                                             scan, cx0, cx1, &ra, use_tab, entry(gi, gg),
+                                             tab_kernel);
           float s0 = 0.0f, s1 = 0.0f;
           if constexpr (kAnalytic) {
             // the row's largest K bounds its pass-2 values (the chain's row
@@ -831,12 +890,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_strip_fused(FusedArgs a) {
             if (PACK > 1 && !(lam > 0.0f)) s.len = 0;
           } else if constexpr (kAnalytic) {
             // the host keeps boxes within the cached groups for the analytic walks
-            s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+            s = RowSeg{0, 0, 0.0f, -1e30f, 1.0f, 0.0f, 0.0f};
           } else {
-            s = RowSeg{0, 0, 0.0f, -1e30f, 0.0f, 0.0f};
+            s = RowSeg{0, 0, 0.0f, -1e30f, 1.0f, 0.0f, 0.0f};
             if ((PACK > 1 ? lane_s : gg * 64 + lane) < nrows && (PACK == 1 || lam > 0.0f))
-              s = row_setup<false>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, scan, cx0,
-                                   cx1);
+              s = row_setup<false>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, du, scan, cx0,
+                                   cx1, nullptr, use_tab, entry(gi, gg), tab_kernel);
           }
           if constexpr (MODE == MODE_MLEM) {
             // K rho 2^bits / lambda <= 2^bits, but the chain's K 2^bits /
@@ -1024,7 +1083,8 @@ __global__ __launch_bounds__(256) void k_row_counts(StripConst sg, const EventGe
   int cnt = 0;
   if (ncols > 0) {
     for (int r = lane; r < nrows; r += 64)
-      cnt += row_setup<false>(sg, sg, g, ev, g.y0 + r, 0, 0, 1.0f, scan != 0, g.x0, g.x1).len;
+      cnt += row_setup<false>(sg, sg, g, ev, g.y0 + r, 0, 0, 1.0f, 0.0f, scan != 0, g.x0, g.x1)
+                 .len;
   }
   for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
   if (lane == 0) {
@@ -1042,6 +1102,51 @@ hipError_t launch_row_counts(hipStream_t st, const StripConst& sg, const EventGe
     const int64_t m = std::min(kSlice, n - e0);
     hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st, sg,
                        geom + e0, m, scan, nbox + e0, nell + e0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// The row table, one wave per sorted event: the entry e * stride + slot
+// holds the walk (columns, u0, row factor, chain ratio, as row_setup forms
+// them) of the box row that the fused kernel's lane visits at that slot:
+// slot gg * 64 + lane for row group gg (group 0 the 64 rows around the
+// ellipse centre from group0_first_row, then the rows above and below it in
+// order); slots past the box's rows hold empty entries.
+__global__ __launch_bounds__(256) void k_row_table(StripConst sg, StripConst sk, float s_pix,
+                                                   float du, const EventGeom* geom, int64_t n,
+                                                   int scan, int64_t stride, RowEntry* tab) {
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= n) return;  // uniform per wave
+  const int lane = __lane_id();
+  const EventGeom g = geom[e];
+  const int nrows = g.x1 > g.x0 ? max(g.y1 - g.y0, 0) : 0;
+  const int m0 = group0_first_row(sg, g, g.y0, nrows, 1.0f / sg.ph);
+  const EventRows ev = event_rows(sg, g, 1.0f / sg.pw);
+  for (int slot = lane; slot < stride; slot += 64) {
+    const int j = slot - 64;
+    const int r = slot < 64 ? m0 + slot : (j < m0 ? j : j + 64);
+    RowEntry t{0u, 0.0f, 0.0f, 0.0f};
+    if (r < nrows) {
+      // window index = column: w0 is the first ellipse column
+      const RowSeg s =
+          row_setup<false>(sg, sk, g, ev, g.y0 + r, 0, 1, s_pix, du, scan != 0, g.x0, g.x1);
+      if (s.len > 0)
+        t = RowEntry{((uint32_t)s.len << 16) | (uint32_t)(s.w0 - g.x0), s.u0, s.lsc, s.b1};
+    }
+    tab[e * stride + slot] = t;
+  }
+}
+
+hipError_t launch_row_table(hipStream_t st, const StripConst& sg, const StripConst& sk,
+                            float s_pix, float du, const EventGeom* sgeom, int64_t n, int scan,
+                            int64_t stride, RowEntry* tab) {
+  constexpr int64_t kSlice = (int64_t)1 << 24;
+  for (int64_t e0 = 0; e0 < n; e0 += kSlice) {
+    const int64_t m = std::min(kSlice, n - e0);
+    hipLaunchKernelGGL(k_row_table, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st, sg, sk,
+                       s_pix, du, sgeom + e0, m, scan, stride, tab + e0 * stride);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

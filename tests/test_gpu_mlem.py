@@ -210,6 +210,49 @@ def test_fixed64_bit_reproducible(gpu_pkg, path, variant):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
+@pytest.mark.parametrize("n_pix,size", [(64, 0.008), (100, 0.005), (200, 0.002)])
+def test_row_table_identical(gpu_pkg, n_pix, size):
+    """The row table (each binned event's box rows formed once per event set
+    and kernel sigmas) gives the bits of the walks that form every row each
+    iteration: the deterministic image, the denominators and, after a change
+    of the kernel's sigmas with the support frozen (the table's kernel values
+    rebuilt), the image again; the gradient and the FP32 image within their
+    summation-order rounding.  Boxes of at most 16, 17-32
+    and more than 32 rows (four, two and one events per wave)."""
+    pkg = gpu_pkg
+    sc, ev = small(pkg, n_pix=n_pix, size=size, n=30000)
+    print(f"case: {n_pix}x{n_pix} @ {size} m, {len(ev)} events", flush=True)
+    res = {}
+    for table in (True, False):
+        kw = opts(pkg, "auto", "fixed64")
+        with pkg.Reconstruction(sc, row_table=table, **kw) as rec:
+            rec.set_events(ev)
+            st = rec.stats()
+            assert st["fused"] == 1
+            assert (st["row_table_bytes"] > 0) == table
+            rec(3)
+            out = [rec.rho, rec.forward()]
+            rec.set_sigma(0.012, 0.045, freeze_support=True)
+            rec.rho = np.ones(rec.shape, np.float32)
+            rec(2)
+            out.append(rec.rho)
+            gr, gs, _ = rec.backward()
+            out += [gr, np.asarray(gs)]
+        with pkg.Reconstruction(sc, row_table=table) as rec:  # FP32
+            rec.set_events(ev)
+            rec(2)
+            out.append(rec.rho)
+        res[table] = out
+    t, f = res[True], res[False]
+    for i in range(3):
+        assert np.array_equal(t[i].view(np.uint32), f[i].view(np.uint32)), i
+    assert rel_l2(t[3], f[3]) < 1e-6
+    # the sigma sums are FP64 sums over the events in their binned order,
+    # which follows the scatter's atomics: equal up to their last bits
+    np.testing.assert_allclose(t[4], f[4], rtol=1e-11)
+    assert rel_l2(t[5], f[5]) < 1e-6
+
+
 def test_per_event_compensated_denominators(gpu_pkg):
     """The per-event kernels' compensated denominators (two-sum per lane and
     across the wave) are closer to the FP64 sums than the plain FP32 ones."""
