@@ -378,61 +378,84 @@ int prepare_fused(petstrip_handle* h) {
   if ((rc0 = update_kernel_table(h))) return rc0;
   const int64_t n = h->n_events;
   const int W = h->sg.W, H = h->sg.H;
-  const int nbx = (W + kBinW - 1) / kBinW, nby = (H + kBinH - 1) / kBinH;
-  const int n_spatial = nbx * nby;
-  const int nb = n_spatial * petstrip::kHeightClasses;
-  dfree(h->d_count);
-  int rc;
-  if ((rc = dalloc(&h->d_count, (size_t)nb * 2))) return rc;
-  HIP_TRY(hipMemsetAsync(h->d_count, 0, (size_t)nb * 2 * sizeof(int32_t), h->stream));
-  HIP_TRY(petstrip::launch_bin_count(h->stream, h->d_geom, n, kBinW, kBinH, nbx, n_spatial,
-                                     h->d_bin, h->d_count));
-  std::vector<int32_t> cnt((size_t)nb), off((size_t)nb);
-  HIP_TRY(hipMemcpyAsync(cnt.data(), h->d_count, nb * sizeof(int32_t),
-                         hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  int64_t acc = 0;
   std::vector<petstrip::WorkItem> items;
-  for (int b = 0; b < nb; ++b) {
-    // the first items of the 17-32 and the at most 16 row classes
-    if (b == n_spatial) h->pack_begin[0] = (int)items.size();
-    if (b == 2 * n_spatial) h->pack_begin[1] = (int)items.size();
-    off[(size_t)b] = (int32_t)acc;
-    for (int64_t s0 = acc; s0 < acc + cnt[(size_t)b]; s0 += kChunk) {
-      petstrip::WorkItem it{};
-      it.begin = (int32_t)s0;
-      it.end = (int32_t)std::min<int64_t>(s0 + kChunk, acc + cnt[(size_t)b]);
-      items.push_back(it);
-    }
-    acc += cnt[(size_t)b];
-  }
-  int32_t* d_cursor = h->d_count + nb;
-  HIP_TRY(hipMemcpyAsync(d_cursor, off.data(), nb * sizeof(int32_t),
-                         hipMemcpyHostToDevice, h->stream));
-  HIP_TRY(petstrip::launch_bin_scatter(h->stream, h->d_geom, n, h->d_bin, d_cursor,
-                                       h->d_sgeom, h->d_sidx));
-  h->n_items = (int)items.size();
-  if (items.size() > h->cap_items) {
-    dfree(h->d_items);
-    if ((rc = dalloc(&h->d_items, items.size()))) return rc;
-    h->cap_items = items.size();
-  }
-  if (!items.empty()) {
-    HIP_TRY(hipMemcpyAsync(h->d_items, items.data(),
-                           items.size() * sizeof(petstrip::WorkItem),
-                           hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(petstrip::launch_item_windows(h->stream, h->d_sgeom, h->d_items,
-                                          h->n_items));
-    HIP_TRY(hipMemcpyAsync(items.data(), h->d_items,
-                           items.size() * sizeof(petstrip::WorkItem),
-                           hipMemcpyDeviceToHost, h->stream));
-  }
-  HIP_TRY(hipStreamSynchronize(h->stream));
+  int64_t acc = 0;
   int max_ww = 1, max_wh = 1, max_rows = 1;
-  for (auto& it : items) {
-    max_ww = std::max(max_ww, it.ww);
-    max_wh = std::max(max_wh, it.wh);
-    max_rows = std::max(max_rows, it.max_rows);
+  int rc;
+  // Bins of kBinW x bin_h pixels.  A window is about bin_h - 1 rows taller
+  // than its tallest box: when that takes it past the column stride the
+  // boxes alone need (64-row multiples), the events are binned again with
+  // bins low enough to keep it, which halves (or better) the window's LDS.
+  auto bin_events = [&](int bin_h) -> int {
+    const int nbx = (W + kBinW - 1) / kBinW, nby = (H + bin_h - 1) / bin_h;
+    const int n_spatial = nbx * nby;
+    const int nb = n_spatial * petstrip::kHeightClasses;
+    dfree(h->d_count);
+    int rc1;
+    if ((rc1 = dalloc(&h->d_count, (size_t)nb * 2))) return rc1;
+    HIP_TRY(hipMemsetAsync(h->d_count, 0, (size_t)nb * 2 * sizeof(int32_t), h->stream));
+    HIP_TRY(petstrip::launch_bin_count(h->stream, h->d_geom, n, kBinW, bin_h, nbx, n_spatial,
+                                       h->d_bin, h->d_count));
+    std::vector<int32_t> cnt((size_t)nb), off((size_t)nb);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), h->d_count, nb * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    acc = 0;
+    items.clear();
+    for (int b = 0; b < nb; ++b) {
+      // the first items of the 17-32 and the at most 16 row classes
+      if (b == n_spatial) h->pack_begin[0] = (int)items.size();
+      if (b == 2 * n_spatial) h->pack_begin[1] = (int)items.size();
+      off[(size_t)b] = (int32_t)acc;
+      for (int64_t s0 = acc; s0 < acc + cnt[(size_t)b]; s0 += kChunk) {
+        petstrip::WorkItem it{};
+        it.begin = (int32_t)s0;
+        it.end = (int32_t)std::min<int64_t>(s0 + kChunk, acc + cnt[(size_t)b]);
+        items.push_back(it);
+      }
+      acc += cnt[(size_t)b];
+    }
+    int32_t* d_cursor = h->d_count + nb;
+    HIP_TRY(hipMemcpyAsync(d_cursor, off.data(), nb * sizeof(int32_t),
+                           hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(petstrip::launch_bin_scatter(h->stream, h->d_geom, n, h->d_bin, d_cursor,
+                                         h->d_sgeom, h->d_sidx));
+    h->n_items = (int)items.size();
+    if (items.size() > h->cap_items) {
+      dfree(h->d_items);
+      h->cap_items = 0;
+      if ((rc1 = dalloc(&h->d_items, items.size()))) return rc1;
+      h->cap_items = items.size();
+    }
+    if (!items.empty()) {
+      HIP_TRY(hipMemcpyAsync(h->d_items, items.data(),
+                             items.size() * sizeof(petstrip::WorkItem),
+                             hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(petstrip::launch_item_windows(h->stream, h->d_sgeom, h->d_items,
+                                            h->n_items));
+      HIP_TRY(hipMemcpyAsync(items.data(), h->d_items,
+                             items.size() * sizeof(petstrip::WorkItem),
+                             hipMemcpyDeviceToHost, h->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    max_ww = max_wh = max_rows = 1;
+    for (auto& it : items) {
+      max_ww = std::max(max_ww, it.ww);
+      max_wh = std::max(max_wh, it.wh);
+      max_rows = std::max(max_rows, it.max_rows);
+    }
+    return PETSTRIP_OK;
+  };
+  if ((rc = bin_events(kBinH))) return rc;
+  {
+    // a bin's centres span bin_h rows, so its window spans at most
+    // max_rows + bin_h - 1 rows
+    const int want = petstrip::fused_col_stride(max_rows);
+    int bh = kBinH;
+    while (bh > 1 && max_rows + bh - 1 > want) bh >>= 1;
+    if (want != 0 && petstrip::fused_col_stride(max_wh) != want && bh < kBinH &&
+        max_rows + bh - 1 <= want && (rc = bin_events(bh)))
+      return rc;
   }
   h->max_rows = max_rows;
   h->max_ww = max_ww;

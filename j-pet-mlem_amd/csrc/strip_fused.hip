@@ -112,7 +112,6 @@ __device__ __forceinline__ EventRows event_rows(const StripConst& sg, const Even
 
 constexpr int kUnroll = 4;
 constexpr int kSpareCols = kUnroll;  // zero rho window columns before the window
-constexpr int kCachedGroups = 4;  // row groups kept in registers between passes
 
 // One box row of an event as a lane walks it.
 struct RowSeg {
@@ -640,7 +639,7 @@ __device__ __forceinline__ void backproject_rows_analytic(const RowSeg& s, const
 // that short events do not leave most lanes idle; the denominator is then a
 // segment sum and every per-event value lives per lane.
 template <int MODE, bool COMP, int HP, int BAND, int PACK>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(PACK > 1 ? 8 : 1)))
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(PACK > 1 || HP == 64 ? 8 : 1)))
 void k_strip_fused(FusedArgs a) {
   static_assert(PACK == 1 || (BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
                                             MODE == MODE_MLEM_KA || MODE == MODE_GRAD)),
@@ -648,6 +647,9 @@ void k_strip_fused(FusedArgs a) {
   static_assert(MODE != MODE_MLEM_KA || BAND == 0, "the analytic walks are unbanded");
   constexpr bool kAnalytic = MODE == MODE_MLEM_KA;
   constexpr int kSeg = 64 / PACK;  // lanes per event
+  // row groups kept in registers between the passes: every group of an
+  // event (a box has at most HP rows, a band's rows at most HP = kBandRows)
+  constexpr int kCached = HP / 64;
   constexpr bool kPass1 = BAND == 1 || (BAND == 0 && MODE != MODE_BACKWARD);
   constexpr bool kPass2 = BAND == 2 || (BAND == 0 && MODE != MODE_FORWARD);
   constexpr bool kRho = kPass1 || (kPass2 && MODE == MODE_MLEM);  // rho window in LDS
@@ -711,7 +713,7 @@ void k_strip_fused(FusedArgs a) {
   // while the current ones are walked.
   const bool use_tab = BAND == 0 && a.rowtab != nullptr;
   const bool tab_kernel = a.rowtab_kernel != 0;
-  constexpr int kPre = PACK == 1 ? 2 : 1;
+  constexpr int kPre = PACK == 1 && kCached > 1 ? 2 : 1;
   const bool pre1 = kPre > 1 && a.rowtab_stride > 64;
   RowEntry pre[kPre];
 #pragma unroll
@@ -767,17 +769,16 @@ void k_strip_fused(FusedArgs a) {
       return a.rowtab[(size_t)(live ? e : e0) * a.rowtab_stride + gg * 64 + lane_s];
     };
     const int m0 = group0_first_row(sg, g, ry0, nrows, inv_ph);
-    RowSeg cache[kCachedGroups];
-    RowA acache[kAnalytic ? kCachedGroups : 1];
+    RowSeg cache[kCached];
+    RowA acache[kAnalytic ? kCached : 1];
 
     // ---- pass 1: denominator (and sigma sums) -------------------------------
     float lam = 0.0f;
     if constexpr (kPass1) {
       float acc = 0.0f, comp = 0.0f, sz = 0.0f, sdl = 0.0f, lmax = -1e30f;
 #pragma unroll
-      for (int gi = 0; gi < kCachedGroups + 1; ++gi) {
-        // groups beyond the cached ones share the last iteration as a loop
-        for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
+      for (int gi = 0; gi < kCached; ++gi) {
+        for (int gg = gi; gg < gi + 1; ++gg) {
           if (gg >= ngroups) break;
           const int j = gg * 64 + lane - 64;
           const int r = PACK > 1 ? lane_s : gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
@@ -794,12 +795,12 @@ void k_strip_fused(FusedArgs a) {
             float km = 0.0f;
             forward_rows_analytic<HP>(s, ra, sg.pw, sk.c, 2.0f * sk.isz, rho0, s0, km);
             s.lsc = km > 0.0f ? __builtin_amdgcn_logf(km) : -1e30f;
-            if (gi < kCachedGroups) acache[gi] = ra;
+            acache[gi] = ra;
           } else {
             forward_rows<HP, COMP, kSigma, kPass1Unroll>(s, du, gtab, mtab, rho0, s0, comp,
                                                          s1);
           }
-          if (gi < kCachedGroups) cache[gi] = s;
+          cache[gi] = s;
           if constexpr (COMP) {
             float sum, err;
             two_sum(acc, s0, sum, err);
@@ -877,20 +878,17 @@ void k_strip_fused(FusedArgs a) {
         lw = log2_shift(a.log2_scale, l, -1.0f);
       }
 #pragma unroll
-      for (int gi = 0; gi < kCachedGroups + 1; ++gi) {
-        for (int gg = gi; gg < (gi < kCachedGroups ? gi + 1 : ngroups); ++gg) {
+      for (int gi = 0; gi < kCached; ++gi) {
+        for (int gg = gi; gg < gi + 1; ++gg) {
           if (gg >= ngroups) break;
           RowSeg s;
           RowA ra{0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
           const int j = gg * 64 + lane - 64;
           const int r = PACK > 1 ? lane_s : gg == 0 ? m0 + lane : (j < m0 ? j : j + 64);
-          if (kPass1 && gi < kCachedGroups) {
+          if constexpr (kPass1) {
             s = cache[gi];
             if constexpr (kAnalytic) ra = acache[gi];
             if (PACK > 1 && !(lam > 0.0f)) s.len = 0;
-          } else if constexpr (kAnalytic) {
-            // the host keeps boxes within the cached groups for the analytic walks
-            s = RowSeg{0, 0, 0.0f, -1e30f, 1.0f, 0.0f, 0.0f};
           } else {
             s = RowSeg{0, 0, 0.0f, -1e30f, 1.0f, 0.0f, 0.0f};
             if ((PACK > 1 ? lane_s : gg * 64 + lane) < nrows && (PACK == 1 || lam > 0.0f))
