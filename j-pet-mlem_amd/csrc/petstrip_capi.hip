@@ -346,7 +346,9 @@ int build_row_table(petstrip_handle* h) {
     return PETSTRIP_OK;
   // slots of whole row groups: a wave's loads of a group stay in its event
   const int64_t stride = ((int64_t)h->max_rows + 63) & ~(int64_t)63;
-  const size_t need = (size_t)h->n_sorted * (size_t)stride;
+  // three spare events past the last (the idle segments of a packed wave
+  // step read there)
+  const size_t need = (size_t)(h->n_sorted + 3) * (size_t)stride;
   if (need > h->cap_rowtab) {
     dfree(h->d_rowtab);
     h->cap_rowtab = 0;
@@ -358,6 +360,8 @@ int build_row_table(petstrip_handle* h) {
     h->cap_rowtab = need;
   }
   h->rowtab_stride = stride;
+  HIP_TRY(hipMemsetAsync(h->d_rowtab + (size_t)h->n_sorted * stride, 0,
+                         3 * (size_t)stride * sizeof(petstrip::RowEntry), h->stream));
   HIP_TRY(petstrip::launch_row_table(h->stream, h->sg, h->sk, h->s_pix, h->du, h->d_sgeom,
                                      h->n_sorted, h->opts.row_scan, stride, h->d_rowtab));
   h->rowtab_on = true;

@@ -122,7 +122,13 @@ struct RowSeg {
   float b1;       // 2^(-2 u du) at the walk's first pixel: the chain's ratio
   float dz_row;   // row part of d lnK / d sigma_z (FORWARD)
   float ddl_row;  // d lnK / d sigma_dl (FORWARD)
+  int mg;         // groups of four the wave walks: its lanes' largest ceil(len / 4)
 };
+
+// The wave's walk length for its lanes' rows s (wave-uniform).
+__device__ __forceinline__ void set_walk_groups(RowSeg& s) {
+  s.mg = uniform(wave_max_i((s.len + 3) >> 2));
+}
 
 // The analytic kernel (AnalyticKernel::operator(), analytic_kernel.h:28-62)
 // along a box row: with pq = pq0 + k pw at the walk's k-th pixel,
@@ -379,7 +385,7 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* 
   // an empty row reads the front spare columns
   const float* rp = s.len > 0 ? rho0 + (s.w0 - pad * HP) : rho0 - kSpareCols * HP;
   const int lastg = max(ng - 1, 0);
-  const int maxng = uniform(wave_max_i(ng));
+  const int maxng = s.mg;
   float sum0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
   Chain c = chain_start(u0, s.lsc, s.b1, 0.0f);
 #pragma unroll
@@ -503,7 +509,7 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
   const int w0 = s.w0 - pad * HP;
   const float* rp = s.len > 0 ? rho0 + w0 : rho0 - kSpareCols * HP;
   const int lastg = max(ng - 1, 0);
-  const int maxng = uniform(wave_max_i(ng));
+  const int maxng = s.mg;
   const float e_add = SPLIT ? lw - 32.0f : lw;
   const float uo[kUnroll] = {u0, u0 + du, fmaf(2.0f, du, u0), fmaf(3.0f, du, u0)};
   unsigned lo[kUnroll], hi[kUnroll];
@@ -556,7 +562,7 @@ __device__ __forceinline__ void forward_rows_analytic(const RowSeg& s, const Row
   const int pad = ng * kUnroll - s.len;
   const float* rp = s.len > 0 ? rho0 + (s.w0 - pad * HP) : rho0 - kSpareCols * HP;
   const int lastg = max(ng - 1, 0);
-  const int maxng = uniform(wave_max_i(ng));
+  const int maxng = s.mg;
   float sum = 0.0f, m = 0.0f;
   for (int gk = 0; gk < maxng; ++gk) {
     float rv[kUnroll];
@@ -587,7 +593,7 @@ __device__ __forceinline__ void backproject_rows_analytic(const RowSeg& s, const
   const int ng = (s.len + kUnroll - 1) >> 2;
   const int pad = ng * kUnroll - s.len;
   const int w0 = s.w0 - pad * HP;
-  const int maxng = uniform(wave_max_i(ng));
+  const int maxng = s.mg;
   const float e_add = SPLIT ? lw - 32.0f : lw;
   const float f1 = __builtin_amdgcn_exp2f(0.5f * e_add);
   const float f2 = __builtin_amdgcn_exp2f(e_add - 0.5f * e_add);
@@ -718,10 +724,14 @@ void k_strip_fused(FusedArgs a) {
   RowEntry pre[kPre];
 #pragma unroll
   for (int k = 0; k < kPre; ++k) pre[k] = RowEntry{0u, 0.0f, 0.0f, 0.0f};
+  // a lane's entries sit at the wave step's first event (a uniform base)
+  // plus loff; the table holds PACK - 1 spare events past the last, for the
+  // idle segments of a step at the end of the events
+  const uint32_t loff = (uint32_t)(seg * (int)a.rowtab_stride + lane_s);
   auto prefetch = [&](int e0n) {
-    const size_t base = (size_t)min(e0n + seg, it.end - 1) * a.rowtab_stride + lane_s;
-    pre[0] = a.rowtab[base];
-    if (pre1) pre[kPre - 1] = a.rowtab[base + 64];
+    const RowEntry* wb = a.rowtab + (size_t)e0n * a.rowtab_stride;
+    pre[0] = wb[loff];
+    if (pre1) pre[kPre - 1] = wb[loff + 64u];
   };
   if (use_tab && it.begin + wave * PACK < it.end) prefetch(it.begin + wave * PACK);
   const float du = a.du;        // u step per column, pw s_pix
@@ -766,7 +776,7 @@ void k_strip_fused(FusedArgs a) {
     auto entry = [&](int gi, int gg) -> RowEntry {
       if (!use_tab) return RowEntry{0u, 0.0f, 0.0f, 0.0f};
       if (gi < kPre && (gi == 0 || pre1)) return cur[gi < kPre ? gi : 0];
-      return a.rowtab[(size_t)(live ? e : e0) * a.rowtab_stride + gg * 64 + lane_s];
+      return (a.rowtab + (size_t)e0 * a.rowtab_stride)[loff + (uint32_t)(gg * 64)];
     };
     const int m0 = group0_first_row(sg, g, ry0, nrows, inv_ph);
     RowSeg cache[kCached];
@@ -788,6 +798,7 @@ void k_strip_fused(FusedArgs a) {
             s = row_setup<kSigma, kAnalytic>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, du,
                                              scan, cx0, cx1, &ra, use_tab, entry(gi, gg),
                                              tab_kernel);
+          set_walk_groups(s);
           float s0 = 0.0f, s1 = 0.0f;
           if constexpr (kAnalytic) {
             // the row's largest K bounds its pass-2 values (the chain's row
@@ -894,6 +905,7 @@ void k_strip_fused(FusedArgs a) {
             if ((PACK > 1 ? lane_s : gg * 64 + lane) < nrows && (PACK == 1 || lam > 0.0f))
               s = row_setup<false>(sg, sk, g, ev, ry0 + r, row0 + r, HP, s_pix, du, scan, cx0,
                                    cx1, nullptr, use_tab, entry(gi, gg), tab_kernel);
+            set_walk_groups(s);
           }
           if constexpr (MODE == MODE_MLEM) {
             // K rho 2^bits / lambda <= 2^bits, but the chain's K 2^bits /
