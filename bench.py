@@ -330,6 +330,9 @@ def parse():
     p.add_argument("--no-row-table", action="store_true",
                    help="form every box row in every iteration instead of reading "
                         "the row table (A/B of the table)")
+    p.add_argument("--no-horner", action="store_true",
+                   help="sum each row's denominator terms by advancing the kernel "
+                        "chain instead of by Horner's rule (A/B of the pass-1 walk)")
     p.add_argument("--no-variant", action="store_true",
                    help="skip the reference --gpu variant (analytic kernel) line")
     p.add_argument("--grad-workload", default="strip-256px-2mm-5e6ev",
@@ -495,7 +498,7 @@ def main():
     acc = {"fp32": pkg.ACCUM_FP32, "compensated": pkg.ACCUM_COMPENSATED,
            "fixed64": pkg.ACCUM_FIXED64}[args.accumulation]
     rec = pkg.Reconstruction(sc, device=0, accumulation=acc,
-                             row_table=not args.no_row_table)
+                             row_table=not args.no_row_table, horner=not args.no_horner)
     t0 = time.perf_counter()
     rec.set_events(ev)
     t_setup = time.perf_counter() - t0

@@ -115,7 +115,12 @@ typedef struct petstrip_options {
                            a quarter of free device memory; 1 = no table,
                            every iteration forms the rows again (identical
                            results)                                      */
-  int32_t reserved;     /* zero                                            */
+  int32_t pass1_walk;   /* 0 = the binned iterations sum each row's
+                           denominator terms as polynomials in the kernel's
+                           geometric ratio by Horner's rule; 1 = by
+                           advancing a chain of kernel values (results agree
+                           to FP32 rounding; the compensated accumulation
+                           and the sigma derivative always use the chain) */
 } petstrip_options;
 
 typedef struct petstrip_handle petstrip_handle;
@@ -141,6 +146,9 @@ typedef struct petstrip_stats {
                                 LDS atomic.  0 for the per-event kernels,
                                 whose updates are all global atomics     */
   int64_t row_table_bytes;   /* device bytes of the row table (0: none)   */
+  int64_t horner_walks;      /* 1: the binned iterations' denominators by
+                                Horner's rule (options pass1_walk), 0: by
+                                the chain                                 */
 } petstrip_stats;
 
 /* Called after an output is ready.  iteration >= 1 for images, -1 for the

@@ -300,6 +300,9 @@ int update_kernel_table(petstrip_handle* h) {
   const float s_pix = sqrtf(h->sk.isz * 1.44269504088896341f);
   const float du = h->sg.pw * s_pix;
   const size_t n = (size_t)h->sg.W + 8;
+  // the tables depend on du alone, but u = pq s_pix takes the new sigma's
+  // s_pix even when du rounds to the same float
+  h->s_pix = s_pix;
   if (h->d_gtab && du == h->du && n == h->gtab_n) return PETSTRIP_OK;
   std::vector<float> t(n), m(2 * n);
   for (size_t k = 0; k < n; ++k) {
@@ -654,6 +657,7 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.ncb = h->ncb;
   a.cshift = h->cshift;
   a.lam_sorted = h->d_lam_sorted;
+  a.horner = h->opts.pass1_walk == 0 ? 1 : 0;
   if (h->rowtab_on) {
     a.rowtab = h->d_rowtab;
     a.rowtab_stride = h->rowtab_stride;
@@ -1162,6 +1166,7 @@ int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
   out->row_table_bytes = h->rowtab_on ? (int64_t)(h->n_sorted * h->rowtab_stride *
                                                   (int64_t)sizeof(petstrip::RowEntry))
                                       : 0;
+  out->horner_walks = h->fused_iter && h->opts.pass1_walk == 0 ? 1 : 0;
   return PETSTRIP_OK;
 }
 

@@ -67,6 +67,8 @@ struct FusedArgs {
                              // walks' row order: k_row_table), or null
   int64_t rowtab_stride;
   int rowtab_kernel;         // 1: the entries' u0, lsc, b1 hold for sk and du
+  int horner;                // 1: pass 1 by Horner's rule (forward_rows_h),
+                             // 0: by the chain (forward_rows)
   float s_pix;               // sqrt(sk.isz log2 e): u = pq s_pix
   float du;                  // sg.pw s_pix, the u step per column
   const float* gtab;         // G(k) = 2^(-(k du)^2) for k < the image width + 8

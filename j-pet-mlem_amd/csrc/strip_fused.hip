@@ -433,6 +433,57 @@ __device__ __forceinline__ void forward_rows(const RowSeg& s, float du, ConstF* 
   }
 }
 
+// Pass 1 over one row group by Horner's rule (the denominator alone).  With
+// the chain's values (chain_start) the walk's k-th pixel has
+// K = P0 r^k G(k), r = b1 the chain's per-pixel ratio and P0 = 2^(lsc - us^2)
+// (us: u at the walk's first pixel), so the row sum is
+// P0 sum_j r^j p_j(R) over the group's four pixels j, with
+// p_j(R) = sum_g R^g a_gj, R = r^4 and a_gj = G(4g + j) rho(4g + j): four
+// polynomials in R, each evaluated by Horner's rule from the row's last group
+// down to its first.  A pixel then takes the table weight and one FMA (the
+// chain took a third operation to advance it), and the per-group chain
+// zeroing and read clamping go: every lane walks groups G = mg - 1 down to 0
+// in step (G is wave-uniform, so G(4G + j) comes through the scalar cache), a
+// lane whose row has fewer groups reads zero columns of the window (zp) until
+// its last group comes up, so its polynomials stay zero, and group 0's pixels
+// before the row are zeroed.  The values R^g a_gj r^j are those of K rho / P0,
+// as in the chain.  zp: the window's zero columns at the lane's row.
+template <int HP, int UNR>
+__device__ __forceinline__ void forward_rows_h(const RowSeg& s, float du, ConstF* gtab,
+                                               const float* rho0, const float* zp, float& s0) {
+  const int ng = (s.len + kUnroll - 1) >> 2;
+  const int pad = ng * kUnroll - s.len;
+  const float r = s.b1;  // the ratio at the walk's first pixel (row_ratio)
+  const float r2 = r * r;
+  const float R = r2 * r2;
+  const float* base = rho0 + (s.w0 - pad * HP);
+  float h[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) h[u] = 0.0f;
+  auto step = [&](int G, const float(&rv)[kUnroll]) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) h[u] = fmaf(h[u], R, gtab[G * kUnroll + u] * rv[u]);
+  };
+  const int mg = s.mg;
+#pragma unroll UNR
+  for (int G = mg - 1; G > 0; --G) {
+    float rv[kUnroll];
+    read4<HP>(G < ng ? base + G * (kUnroll * HP) : zp, rv);
+    step(G, rv);
+  }
+  if (mg > 0) {
+    float rv[kUnroll];
+    read4<HP>(ng > 0 ? base : zp, rv);
+#pragma unroll
+    for (int u = 0; u < kUnroll - 1; ++u) rv[u] = u < pad ? 0.0f : rv[u];
+    step(0, rv);
+  }
+  if (s.len <= 0) return;
+  const float us = fmaf(-(float)pad, du, s.u0);
+  const float p0 = __builtin_amdgcn_exp2f(fmaf(-us, us, s.lsc));
+  s0 += p0 * fmaf(fmaf(fmaf(h[3], r, h[2]), r, h[1]), r, h[0]);
+}
+
 // x -> the integer round(x) as (lo, hi); SPLIT: kv = x 2^-32 < 2^30 and the
 // halves are converted apart; RPI: x < 2^31 is rounded by the conversion
 // itself (v_cvt_rpi_i32_f32, floor(x + 0.5)); else x + 0.5 < 2^32 arrives
@@ -666,6 +717,11 @@ void k_strip_fused(FusedArgs a) {
   // pass 1's group loop unrolled four times, except with the sigma sums in
   // the two-pass kernel, where the extra registers slowed the whole kernel
   constexpr int kPass1Unroll = (kSigma && BAND == 0) ? 1 : 4;
+  // pass 1 by Horner's rule (forward_rows_h): the Gaussian walks with plain
+  // sums and no sigma moments (the chain issues fewer instructions for those);
+  // a.horner = 0 keeps the chain (forward_rows) for comparison
+  constexpr bool kHorner = kPass1 && !COMP && !kAnalytic && !kSigma;
+  const bool horner = kHorner && a.horner != 0;
   using U = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem[];
   const StripConst& sg = a.sg;
@@ -808,8 +864,14 @@ void k_strip_fused(FusedArgs a) {
             s.lsc = km > 0.0f ? __builtin_amdgcn_logf(km) : -1e30f;
             acache[gi] = ra;
           } else {
-            forward_rows<HP, COMP, kSigma, kPass1Unroll>(s, du, gtab, mtab, rho0, s0, comp,
-                                                         s1);
+            if (horner) {
+              // the zero columns at the lane's window row
+              const float* zp = rho_w + ((ry0 + r - it.wy0) & 63);
+              forward_rows_h<HP, 2>(s, du, gtab, rho0, zp, s0);
+            } else {
+              forward_rows<HP, COMP, kSigma, kPass1Unroll>(s, du, gtab, mtab, rho0, s0, comp,
+                                                           s1);
+            }
           }
           cache[gi] = s;
           if constexpr (COMP) {

@@ -45,7 +45,7 @@ class COptions(C.Structure):
     _fields_ = [("accumulation", C.c_int32), ("update", C.c_int32),
                 ("path", C.c_int32), ("row_scan", C.c_int32),
                 ("force_bands", C.c_int32), ("kernel", C.c_int32),
-                ("row_table", C.c_int32), ("reserved", C.c_int32)]
+                ("row_table", C.c_int32), ("pass1_walk", C.c_int32)]
 
 
 class CPhantomGrid(C.Structure):
@@ -70,6 +70,7 @@ class CStats(C.Structure):
         ("last_kernel_ms", C.c_double), ("fused", C.c_int32),
         ("n_work_items", C.c_int32), ("lds_bytes", C.c_int64),
         ("n_window_pixels", C.c_int64), ("row_table_bytes", C.c_int64),
+        ("horner_walks", C.c_int64),
     ]
 
 
@@ -343,12 +344,13 @@ class Reconstruction:
 
     def __init__(self, scanner, device=0, accumulation=ACCUM_FP32,
                  update=UPDATE_CPU, path=PATH_AUTO, row_scan=False, force_bands=False,
-                 kernel=KERNEL_GAUSSIAN, row_table=True):
+                 kernel=KERNEL_GAUSSIAN, row_table=True, horner=True):
         self.lib = load_library()
         self.scanner = scanner
         self._h = C.c_void_p()
         opts = COptions(accumulation, update, path, 1 if row_scan else 0,
-                        int(force_bands), kernel, 0 if row_table else 1)
+                        int(force_bands), kernel, 0 if row_table else 1,
+                        0 if horner else 1)
         sc = scanner.c()
         _check(self.lib.petstrip_create(C.byref(sc), device, C.byref(opts),
                                         C.byref(self._h)))
@@ -387,9 +389,9 @@ class Reconstruction:
 
     def set_options(self, accumulation=ACCUM_FP32, update=UPDATE_CPU, path=PATH_AUTO,
                     row_scan=False, force_bands=False, kernel=KERNEL_GAUSSIAN,
-                    row_table=True):
+                    row_table=True, horner=True):
         o = COptions(accumulation, update, path, 1 if row_scan else 0,
-                     int(force_bands), kernel, 0 if row_table else 1)
+                     int(force_bands), kernel, 0 if row_table else 1, 0 if horner else 1)
         _check(self.lib.petstrip_set_options(self._h, C.byref(o)))
 
     def set_events(self, responses):

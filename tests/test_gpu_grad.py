@@ -283,3 +283,25 @@ def test_workload_256px_fd_64_pixels(gpu_pkg):
     n = len(ev)
     assert gsig[0] == pytest.approx(fd_sig[0], rel=1e-3, abs=1e-5 * n / s0[0])
     assert gsig[1] == pytest.approx(fd_sig[1], rel=1e-3, abs=1e-5 * n / s0[1])
+
+
+def test_loglik_independent_of_sigma_history(gpu_pkg):
+    """The log-likelihood and sigma gradient at a sigma do not depend on the
+    sigmas set before it: sigma_z values a few ulps apart share the kernel
+    tables' u step (du = pixel width x s_pix rounds to the same float) while
+    their s_pix differ, and each must take its own s_pix."""
+    pkg = gpu_pkg
+    sc, ev, rho, _ = setup(pkg)
+    sz, sdl = 0.0111, 0.043
+    with pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        rec.rho = rho
+        rec.set_sigma(sz, sdl, freeze_support=True)
+        _, g0, ll0 = rec.backward(want_rho=False, want_loglik=True)
+        print(f"case: sigma ({sz}, {sdl}), loglik {ll0!r}", flush=True)
+        for k in range(1, 25):
+            rec.set_sigma(sz * (1.0 + 3e-8 * k), sdl, freeze_support=True)
+            rec.backward(want_rho=False, want_loglik=True)
+            rec.set_sigma(sz, sdl, freeze_support=True)
+            _, g, ll = rec.backward(want_rho=False, want_loglik=True)
+            assert ll == ll0 and tuple(g) == tuple(g0), k

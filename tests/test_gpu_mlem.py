@@ -253,6 +253,52 @@ def test_row_table_identical(gpu_pkg, n_pix, size):
     assert rel_l2(t[5], f[5]) < 1e-6
 
 
+@pytest.mark.parametrize("n_pix,size,bands", [(160, 0.004, False), (200, 0.002, False),
+                                              (200, 0.002, True), (64, 0.008, False)])
+def test_horner_walks_against_chain(gpu_pkg, n_pix, size, bands):
+    """The iterations' pass 1 by Horner's rule against the chain walks
+    (options pass1_walk = 1) on the same events: two FP32 iterations and the
+    deterministic image; with bands also the denominators of the forward
+    projection (its banded pass 1 is the iteration's); the gradient (whose
+    walks keep the chain for the sigma moments) and the FP64 oracle's
+    denominators unchanged."""
+    pkg = gpu_pkg
+    sc, ev = small(pkg, n_pix=n_pix, size=size, n=40000)
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0.2, 2.0, (n_pix, n_pix)).astype(np.float32)
+    res = {}
+    for horner in (True, False):
+        with pkg.Reconstruction(sc, horner=horner, force_bands=bands) as rec:
+            rec.set_events(ev)
+            st = rec.stats()
+            assert st["fused"] == (2 if bands else 1)
+            assert st["horner_walks"] == int(horner)
+            rec.rho = rho
+            lam = rec.forward()
+            g, gs, ll = rec.backward(want_loglik=True)
+            rec.rho = np.ones(rec.shape, np.float32)
+            rec(2)
+            out = [lam, g, np.asarray(gs), ll, rec.rho]
+        with pkg.Reconstruction(sc, horner=horner, force_bands=bands,
+                                accumulation=pkg.ACCUM_FIXED64) as rec:
+            rec.set_events(ev)
+            rec(2)
+            out.append(rec.rho)
+        res[horner] = out
+    print(f"case: {n_pix}x{n_pix} @ {size} m, bands {bands}", flush=True)
+    (lt, gt, st_, llt, it, dt), (lf, gf, sf, llf, if_, df) = res[True], res[False]
+    assert rel_l2(lt, lf) < 1e-6
+    m = lf > 1e-3 * lf.max()
+    assert np.max(np.abs(lt[m] - lf[m]) / lf[m]) < 2e-6
+    assert rel_l2(gt, gf) < 1e-6
+    np.testing.assert_allclose(st_, sf, rtol=1e-4)
+    assert llt == pytest.approx(llf, rel=1e-8)
+    assert rel_l2(it, if_) < 2e-6
+    assert rel_l2(dt, df) < 2e-6
+    lam64 = ol.loglik_f64(sc, ev, rho, want_lambda=True)[1]
+    assert rel_l2(lt, lam64) < 2e-6
+
+
 def test_per_event_compensated_denominators(gpu_pkg):
     """The per-event kernels' compensated denominators (two-sum per lane and
     across the wave) are closer to the FP64 sums than the plain FP32 ones."""
