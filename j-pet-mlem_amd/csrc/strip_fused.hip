@@ -676,6 +676,15 @@ __device__ __forceinline__ void backproject_rows_analytic(const RowSeg& s, const
 
 }  // namespace
 
+// A row-table entry, read once per iteration: a streaming (non-temporal)
+// load, so that the table's tens of GB per launch do not evict the image
+// lines the window flushes add to.
+__device__ __forceinline__ RowEntry load_entry(const RowEntry* p) {
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+  const v4u v = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(p));
+  return RowEntry{v.x, __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w)};
+}
+
 // One workgroup of kWaves waves per work item (a chunk of events sharing a
 // spatial bin).  The LDS windows are column-major with column stride HP, a
 // multiple of 64 at least the window height: the 64 lanes of a wave walk 64
@@ -786,8 +795,8 @@ void k_strip_fused(FusedArgs a) {
   const uint32_t loff = (uint32_t)(seg * (int)a.rowtab_stride + lane_s);
   auto prefetch = [&](int e0n) {
     const RowEntry* wb = a.rowtab + (size_t)e0n * a.rowtab_stride;
-    pre[0] = wb[loff];
-    if (pre1) pre[kPre - 1] = wb[loff + 64u];
+    pre[0] = load_entry(wb + loff);
+    if (pre1) pre[kPre - 1] = load_entry(wb + loff + 64u);
   };
   if (use_tab && it.begin + wave * PACK < it.end) prefetch(it.begin + wave * PACK);
   const float du = a.du;        // u step per column, pw s_pix
@@ -832,7 +841,7 @@ void k_strip_fused(FusedArgs a) {
     auto entry = [&](int gi, int gg) -> RowEntry {
       if (!use_tab) return RowEntry{0u, 0.0f, 0.0f, 0.0f};
       if (gi < kPre && (gi == 0 || pre1)) return cur[gi < kPre ? gi : 0];
-      return (a.rowtab + (size_t)e0 * a.rowtab_stride)[loff + (uint32_t)(gg * 64)];
+      return load_entry(a.rowtab + (size_t)e0 * a.rowtab_stride + loff + (uint32_t)(gg * 64));
     };
     const int m0 = group0_first_row(sg, g, ry0, nrows, inv_ph);
     RowSeg cache[kCached];
