@@ -253,7 +253,7 @@ int reserve_events(petstrip_handle* h, int64_t n) {
 // of the tall (sigma_dl > sigma_z) ellipses within a 128-row column stride.
 constexpr int kBinW = 16;
 constexpr int kBinH = 8;
-constexpr int kChunk = 1024;      // events per work item
+constexpr int kChunk = petstrip::kItemEvents;  // events per work item
 constexpr int kMinGroupItems = 2048;  // banded items per pipelined group, at least
 // MLEM backprojection values K rho / lambda <= 1 are summed in 2^-31 units
 // (one conversion per value), or in 2^-40 units in the deterministic
@@ -267,11 +267,13 @@ constexpr int kFixed64Bits = 40;
 // 1e-6 of itself, and values up to 2^10 times the typical one (cold pixels,
 // small denominators) still take the single conversion.
 constexpr int kGradTypicalBits = 20;
-// The FP32 MLEM iteration sums the same values K / lambda (its image is
-// rho times the sums): a typical value at 2^24 units is rounded to about
-// 3e-8 of itself, as fine as the K rho / lambda values in 2^-31 units, and
-// values up to 90 times the typical one take the single conversion.
-constexpr int kMlemTypicalBits = 24;
+// The FP32 MLEM iteration sums the same values K / lambda (its image is rho
+// times the sums) in 32-bit windows: each work item of the binned kernel
+// picks its own unit from its window's rho and event count (strip_fused.h,
+// kLog2Sum32), values above its cap add to the image directly.  The launch's
+// unit, a typical value at 2^13 units, serves the items whose window sums to
+// zero.
+constexpr int kMlemTypicalBits = 13;
 
 // Along a box row the reference's float ellipse test has no holes when the
 // quadratic's column-to-column curvature dominates the rounding of q.  With
@@ -658,6 +660,8 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
   a.cshift = h->cshift;
   a.lam_sorted = h->d_lam_sorted;
   a.horner = h->opts.pass1_walk == 0 ? 1 : 0;
+  a.log2_ell = (float)std::log2(std::max(
+      1.0, (double)h->stats.n_ellipse_pixels / (double)std::max<int64_t>(h->stats.n_events, 1)));
   if (h->rowtab_on) {
     a.rowtab = h->d_rowtab;
     a.rowtab_stride = h->rowtab_stride;
@@ -728,9 +732,10 @@ int one_iteration(petstrip_handle* h) {
     if (h->rowtab_on && !rowtab_kernel_ok(h) && (rc0 = build_row_table(h))) return rc0;
     petstrip::FusedArgs a = fused_args(h);
     a.rho_weight = denominator_weight(h);  // the --gpu variant's sensitivity
-    a.acc_kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64
-                 : accm == PETSTRIP_ACCUM_COMPENSATED ? petstrip::GLOBAL_F64
-                                                       : petstrip::GLOBAL_F32;
+    // FP32: the 32-bit windows' rows above an item's cap add their values
+    // to the image directly, possibly many to one pixel, so the image sums
+    // in FP64 (the flush adds one value per window pixel)
+    a.acc_kind = accm == PETSTRIP_ACCUM_FIXED64 ? petstrip::GLOBAL_U64 : petstrip::GLOBAL_F64;
     // FP32: pass 2 adds K / lambda_e (as the gradient does) and the sums are
     // multiplied by rho at the end; its unit puts a typical value at
     // 2^kGradTypicalBits.  The other forms add K rho / lambda_e.
@@ -755,9 +760,10 @@ int one_iteration(petstrip_handle* h) {
       const size_t lds2 = petstrip::fused_lds_bytes(h->max_ww, h->col_stride, mode2, 2);
       if ((rc = banded_pipeline(h, a, petstrip::MODE_MLEM, false, mode2, lds2))) return rc;
     } else {
-      HIP_TRY(petstrip::launch_strip_fused(h->stream, mode2, h->col_stride,
-                                           accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
-                                           h->n_items, h->lds_bytes, h->pack_begin));
+      HIP_TRY(petstrip::launch_strip_fused(
+          h->stream, mode2, h->col_stride, accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
+          h->n_items, petstrip::fused_lds_bytes(h->max_ww, h->col_stride, mode2),
+          h->pack_begin));
     }
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     // float accumulators already hold image units; the u64 one holds 2^-40

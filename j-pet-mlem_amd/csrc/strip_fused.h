@@ -58,7 +58,10 @@ struct FusedArgs {
   const float* g;            // upstream weights per event, or null for 1/lambda
   void* acc;                 // W*H global accumulator (MLEM, BACKWARD)
   int acc_kind;              // GLOBAL_*
-  float log2_scale;          // k: pass-2 values are added in units 2^-k
+  float log2_scale;          // k: pass-2 values are added in units 2^-k (the
+                             // K / lambda modes: each work item its own k,
+                             // this one for windows whose rho sums to zero)
+  float log2_ell;            // log2 of the mean ellipse pixels per event
   int split;                 // BACKWARD: values up to 2^62 (else below 2^31)
   double inv_fixed_scale;    // 2^-k, the unit when flushing to floats
   int row_scan;              // 1: every row interval by the scanning path
@@ -108,6 +111,20 @@ constexpr int kBandRows = 64;        // rows per band of the banded form
 constexpr int kColBandShift = 7;     // columns per column band: 128 (windows
                                      // too wide for LDS even in row bands)
 constexpr int kMaxColStride = 256;   // largest LDS column stride (window height)
+constexpr int kItemEvents = 1024;    // most events in a work item
+// The iteration's K / lambda sums (MODE_MLEM_K, MODE_MLEM_KA) accumulate in
+// 32-bit LDS windows: in a work item of n events (n <= kItemEvents), values
+// below 2^(kLog2Sum32 - log2 n) units, at most n of them per window pixel,
+// stay below 2^32; rows with larger values add to the global image
+// directly.  Each item's unit puts a typical value of its window
+// kItemHeadroomBits below that cap (2^12.9 units at n = 1024: values up to
+// 512 times the typical one take the window; a pixel's N rounding errors,
+// independent and of mean zero, leave its sum within ~4e-5 / sqrt(N) of the
+// typical value's relative size).
+constexpr float kLog2Sum32 = 31.9f;
+constexpr float kItemHeadroomBits = 9.0f;
+constexpr int kLdsHead = 128;        // LDS bytes before the windows (two
+                                     // floats per wave)
 
 // n_box and n_ell per event from the row intervals of the fused walks
 // (valid when rows are hole-free intervals); scan = 1 forces the scanning

@@ -53,6 +53,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "strip_fused.h"
 #include "strip_geometry.h"
@@ -525,6 +526,12 @@ __device__ __forceinline__ void lds_accumulate(unsigned long long* p, unsigned l
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// The 32-bit windows of the K / lambda sums (ds_add_u32, 1.5x the rate of
+// ds_add_u64 on gfx950, one LDS bank per lane).
+__device__ __forceinline__ void lds_accumulate32(uint32_t* p, uint32_t v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // (lo, hi) into a 64-bit register pair that lives across the loop: when hi
 // is the constant 0 the pair's upper register keeps it, instead of a copy
 // of 0 per value.
@@ -548,12 +555,14 @@ __device__ __forceinline__ void set_pair(unsigned long long& v, unsigned lo, uns
 // itself.  The conversion is inline assembly, which the compiler does not
 // separate from a transcendental result by the wait state gfx950 needs; the
 // values it reads come from a multiplication by the table.
-template <int HP, bool RHO, bool FOLD, bool SPLIT, bool SUB, bool RPI = false>
+template <int HP, bool RHO, bool FOLD, bool SPLIT, bool SUB, bool RPI = false,
+          typename AccT = unsigned long long>
 __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, float lw,
                                                  float lam, float unit_scale, ConstF* gtab,
-                                                 const float* rho0,
-                                                 unsigned long long* acc_w) {
+                                                 const float* rho0, AccT* acc_w) {
   static_assert(!RPI || (FOLD && !SPLIT), "the rounding conversion reads chain values");
+  constexpr bool k32 = sizeof(AccT) == 4;
+  static_assert(!k32 || (RPI && !SUB), "32-bit windows take single rounded values");
   const int ng = (s.len + kUnroll - 1) >> 2;
   const int pad = ng * kUnroll - s.len;
   const float u0 = fmaf(-(float)pad, du, s.u0);
@@ -592,8 +601,12 @@ __device__ __forceinline__ void backproject_rows(const RowSeg& s, float du, floa
     if (gk < ng) {
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
-        if (!first || u == kUnroll - 1 || u >= pad)
-          lds_accumulate<SUB>(acc_w + (w0 + (gk * kUnroll + u) * HP), pv[u]);
+        if (!first || u == kUnroll - 1 || u >= pad) {
+          if constexpr (k32)
+            lds_accumulate32(acc_w + (w0 + (gk * kUnroll + u) * HP), lo[u]);
+          else
+            lds_accumulate<SUB>(acc_w + (w0 + (gk * kUnroll + u) * HP), pv[u]);
+        }
     }
   };
   if (maxng > 0) group(0, true);
@@ -636,11 +649,12 @@ __device__ __forceinline__ void forward_rows_analytic(const RowSeg& s, const Row
 // overflows), rounded by the conversion itself (values below 2^31) or, with
 // SPLIT, carried with a further 2^-32 and converted in two halves (below
 // 2^62); the same masks as backproject_rows.
-template <int HP, bool SPLIT>
+template <int HP, bool SPLIT, typename AccT = unsigned long long>
 __device__ __forceinline__ void backproject_rows_analytic(const RowSeg& s, const RowA& ra,
                                                           float pw, float c, float bq,
-                                                          float lw,
-                                                          unsigned long long* acc_w) {
+                                                          float lw, AccT* acc_w) {
+  constexpr bool k32 = sizeof(AccT) == 4;
+  static_assert(!k32 || !SPLIT, "32-bit windows take single rounded values");
   const int ng = (s.len + kUnroll - 1) >> 2;
   const int pad = ng * kUnroll - s.len;
   const int w0 = s.w0 - pad * HP;
@@ -668,8 +682,12 @@ __device__ __forceinline__ void backproject_rows_analytic(const RowSeg& s, const
     if (gk < ng) {
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
-        if (gk > 0 || u >= pad)
-          lds_accumulate<false>(acc_w + (w0 + (gk * kUnroll + u) * HP), pv[u]);
+        if (gk > 0 || u >= pad) {
+          if constexpr (k32)
+            lds_accumulate32(acc_w + (w0 + (gk * kUnroll + u) * HP), lo[u]);
+          else
+            lds_accumulate<false>(acc_w + (w0 + (gk * kUnroll + u) * HP), pv[u]);
+        }
     }
   }
 }
@@ -732,6 +750,13 @@ void k_strip_fused(FusedArgs a) {
   constexpr bool kHorner = kPass1 && !COMP && !kAnalytic && !kSigma;
   const bool horner = kHorner && a.horner != 0;
   using U = unsigned long long;
+  // the iteration's K / lambda sums accumulate in 32-bit windows (the item's
+  // unit and cap below keep every window sum below 2^32); the gradient's (its
+  // image term sum K / lambda - 1 cancels near the MLEM fixed point, so it
+  // keeps the finer 64-bit unit), the K rho / lambda and the signed sums keep
+  // 64 bits
+  constexpr bool kAcc32 = MODE == MODE_MLEM_K || MODE == MODE_MLEM_KA;
+  using AccT = std::conditional_t<kAcc32, uint32_t, U>;
   extern __shared__ __align__(16) unsigned char smem[];
   const StripConst& sg = a.sg;
   const StripConst& sk = a.sk;
@@ -742,10 +767,12 @@ void k_strip_fused(FusedArgs a) {
   // group starts up to three pixels before its row, and an empty row reads
   // the four of them (values discarded); rho0 is the window's column 0
   const int nrho = (ww + kSpareCols) * HP;
-  float* rho_w = reinterpret_cast<float*>(smem);
+  // the first kLdsHead bytes hold two floats per wave (the window sums below)
+  float* wred = reinterpret_cast<float*>(smem);
+  float* rho_w = reinterpret_cast<float*>(smem + kLdsHead);
   const float* rho0 = rho_w + kSpareCols * HP;
-  U* acc_w = reinterpret_cast<U*>(
-      smem + (kRho ? (((size_t)nrho * 4 + 15) & ~(size_t)15) : 0));
+  AccT* acc_w = reinterpret_cast<AccT*>(
+      smem + kLdsHead + (kRho ? (((size_t)nrho * 4 + 15) & ~(size_t)15) : 0));
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = __lane_id();
   const int seg = PACK > 1 ? lane / kSeg : 0;      // the lane's event of the wave's PACK
@@ -754,6 +781,8 @@ void k_strip_fused(FusedArgs a) {
   // Window load and flush visit tiles of 16 columns x 4 rows per wave, so
   // that the global reads and atomics of a wave cover 64-byte row segments.
   const int tc_lane = lane & 15, tr_lane = lane >> 4;
+  // the thread's shares of the window's sums of rho and rho^2 (kAcc32)
+  float vsum = 0.0f, vsq = 0.0f;
   if constexpr (kRho) {
     // tile columns from -16: tile 0 holds the spare columns
     const int ntc = ((ww + 15) >> 4) + 1, ntr = HP / 4;
@@ -766,14 +795,63 @@ void k_strip_fused(FusedArgs a) {
         const size_t p = (size_t)(it.wy0 + r) * sg.W + (it.wx0 + c);
         v = a.rho[p];
         if (kPass1 && a.rho_weight) v *= a.rho_weight[p];  // the variant's rho s
+        if constexpr (kAcc32) {
+          vsum += v;
+          vsq = fmaf(v, v, vsq);
+        }
       }
       rho_w[(c + kSpareCols) * HP + r] = v;
     }
+  } else if constexpr (kAcc32) {
+    // pass 2 alone (banded): the window's rho for its sum only
+    for (int i = threadIdx.x; i < ww * wh; i += kThreads) {
+      const int r = i / ww, c = i - r * ww;
+      const float v = a.rho[(size_t)(it.wy0 + r) * sg.W + (it.wx0 + c)];
+      vsum += v;
+      vsq = fmaf(v, v, vsq);
+    }
   }
   if constexpr (kPass2) {
-    for (int i = threadIdx.x; i < npx; i += kThreads) acc_w[i] = 0ull;
+    for (int i = threadIdx.x; i < npx; i += kThreads) acc_w[i] = 0;
+  }
+  if constexpr (kAcc32) {
+    vsum = wave_sum(vsum);
+    vsq = wave_sum(vsq);
+    if (lane == 0) {
+      wred[wave] = vsum;
+      wred[kWaves + wave] = vsq;
+    }
   }
   __syncthreads();
+  // The work item's fixed-point unit 2^-k of the K / lambda sums (kAcc32).
+  // Its n events add to a window pixel at most once each, so values below
+  // 2^vcap = 2^kLog2Sum32 / n keep every window sum below 2^32; rows whose
+  // values could exceed it add to the image directly.  A typical value,
+  // 1 / (rho x ellipse pixels), sits kItemHeadroomBits below the cap, with
+  // rho the window's rho-weighted mean sum rho^2 / sum rho: the level an
+  // event's denominator sees, which empty pixels do not dilute.  The window
+  // follows the local activity, so cold and hot regions each get their own
+  // resolution.  An empty window keeps the launch's unit.
+  // (and below 2^30.5, the range of the rounding conversion to int32)
+  const float vcap =
+      fminf(kLog2Sum32 - __builtin_amdgcn_logf((float)max(it.end - it.begin, 1)), 30.5f);
+  float item_bits = a.log2_scale;
+  if constexpr (kAcc32) {
+    float wsum = 0.0f, wsq = 0.0f;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      wsum += wred[w];
+      wsq += wred[kWaves + w];
+    }
+    const float mean = wsum > 0.0f ? wsq / wsum : 0.0f;
+    if (mean > 0.0f)
+      item_bits = fminf(fmaxf(floorf(vcap - kItemHeadroomBits + __builtin_amdgcn_logf(mean) +
+                                     a.log2_ell),
+                              -60.0f),
+                        60.0f);
+  }
+  const double item_unit = kAcc32 ? (double)__builtin_amdgcn_exp2f(-item_bits)
+                                  : a.inv_fixed_scale;
 
   const float s_pix = a.s_pix;  // sqrt(isz log2 e)
   const bool scan = a.row_scan != 0;
@@ -948,7 +1026,7 @@ void k_strip_fused(FusedArgs a) {
         // packed: the lanes of an event without a positive denominator walk
         // empty rows (below)
         if (PACK == 1 ? !(lam > 0.0f) : !__any(lam > 0.0f)) continue;
-        lw = log2_shift(a.log2_scale, lam > 0.0f ? lam : 1.0f, -1.0f);
+        lw = log2_shift(item_bits, lam > 0.0f ? lam : 1.0f, -1.0f);
       } else if (a.g) {
         const float ge = a.g[orig];
         if (ge == 0.0f) continue;
@@ -1000,34 +1078,36 @@ void k_strip_fused(FusedArgs a) {
             }
           } else if constexpr (kAnalytic) {
             const float vmax = s.lsc + lw;
-            if (!__any(vmax > 30.5f)) {
+            if (!__any(vmax > vcap)) {
               backproject_rows_analytic<HP, false>(s, ra, sg.pw, sk.c, 2.0f * sk.isz, lw,
                                                    acc_w);
-            } else if (!__any(vmax > 52.0f)) {
-              backproject_rows_analytic<HP, true>(s, ra, sg.pw, sk.c, 2.0f * sk.isz, lw,
-                                                  acc_w);
             } else if (s.len > 0) {
               const int col0 = it.wx0 + (s.w0 - (row0 + r + it.wx0 * HP)) / HP;
               const size_t p0 = (size_t)(ry0 + r) * sg.W + col0;
               for (int k = 0; k < s.len; ++k) {
                 const float kv = analytic_at(sk.c, 2.0f * sk.isz, ra,
                                              fmaf((float)k, sg.pw, ra.pq0));
-                unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + p0 + k,
-                                (float)((double)kv / (double)lam));
+                if (a.acc_kind == GLOBAL_F64)
+                  unsafeAtomicAdd(reinterpret_cast<double*>(a.acc) + p0 + k,
+                                  (double)kv / (double)lam);
+                else
+                  unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + p0 + k,
+                                  (float)((double)kv / (double)lam));
               }
             }
           } else if constexpr (kNoRho2) {
-            // K / lambda has no bound: rows whose values stay below 2^30.5
-            // (or 2^52) fixed-point units take the single rounding (split)
-            // conversion, the rare rest is added to the global image in
-            // its own format
+            // K / lambda has no bound: rows whose values stay below 2^vcap
+            // fixed-point units (32-bit windows; 2^30.5 or, split, 2^52 in
+            // the 64-bit ones) take the single rounding conversion, the rare
+            // rest is added to the global image in its own format
             const float vmax = s.lsc + lw;
-            if (!__any(vmax > 30.5f)) {
+            if (!__any(vmax > (kAcc32 ? vcap : 30.5f))) {
               backproject_rows<HP, false, true, false, false, true>(
                   s, du, lw, 0.0f, unit_scale, gtab, rho0, acc_w);
-            } else if (!__any(vmax > 52.0f)) {
-              backproject_rows<HP, false, true, true, false>(s, du, lw, 0.0f, unit_scale,
-                                                             gtab, rho0, acc_w);
+            } else if (!kAcc32 && !__any(vmax > 52.0f)) {
+              if constexpr (!kAcc32)
+                backproject_rows<HP, false, true, true, false>(s, du, lw, 0.0f, unit_scale,
+                                                               gtab, rho0, acc_w);
             } else if (s.len > 0) {
               const int col0 = it.wx0 + (s.w0 - (row0 + r + it.wx0 * HP)) / HP;
               const size_t p0 = (size_t)(ry0 + r) * sg.W + col0;
@@ -1071,17 +1151,17 @@ void k_strip_fused(FusedArgs a) {
       const int tr = t / ntc;
       const int c = (t - tr * ntc) * 16 + tc_lane, r = tr * 4 + tr_lane;
       if (c >= ww || r >= wh) continue;
-      U v = acc_w[c * HP + r];
-      if (v == 0ull) continue;
+      const AccT v = acc_w[c * HP + r];
+      if (v == 0) continue;
       size_t p = (size_t)(it.wy0 + r) * sg.W + (it.wx0 + c);
+      // 64-bit sums are signed (BACKWARD subtracts), 32-bit ones are not
+      const double dv = kAcc32 ? (double)v : (double)(long long)v;
       if (a.acc_kind == GLOBAL_U64)
-        atomicAdd(reinterpret_cast<U*>(a.acc) + p, v);
+        atomicAdd(reinterpret_cast<U*>(a.acc) + p, (U)v);
       else if (a.acc_kind == GLOBAL_F64)
-        unsafeAtomicAdd(reinterpret_cast<double*>(a.acc) + p,
-                        (double)(long long)v * a.inv_fixed_scale);
+        unsafeAtomicAdd(reinterpret_cast<double*>(a.acc) + p, dv * item_unit);
       else
-        unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + p,
-                        (float)((double)(long long)v * a.inv_fixed_scale));
+        unsafeAtomicAdd(reinterpret_cast<float*>(a.acc) + p, (float)(dv * item_unit));
     }
   }
 
@@ -1240,8 +1320,9 @@ size_t fused_lds_bytes(int max_ww, int col_stride, int mode, int band) {
   const size_t hp = (size_t)col_stride;
   const bool no_rho = mode == MODE_BACKWARD || (band == 2 && mode != MODE_MLEM);
   size_t rho = no_rho ? 0 : ((((size_t)max_ww + kSpareCols) * hp * 4 + 15) & ~(size_t)15);
-  size_t acc = mode == MODE_FORWARD ? 0 : (size_t)max_ww * hp * 8;
-  return rho + acc + 16;
+  const bool acc32 = mode == MODE_MLEM_K || mode == MODE_MLEM_KA;
+  size_t acc = mode == MODE_FORWARD ? 0 : (size_t)max_ww * hp * (acc32 ? 4 : 8);
+  return kLdsHead + rho + acc + 16;
 }
 
 int fused_col_stride(int max_wh) {

@@ -250,7 +250,11 @@ def test_row_table_identical(gpu_pkg, n_pix, size):
     # the sigma sums are FP64 sums over the events in their binned order,
     # which follows the scatter's atomics: equal up to their last bits
     np.testing.assert_allclose(t[4], f[4], rtol=1e-11)
-    assert rel_l2(t[5], f[5]) < 1e-6
+    # the FP32 iteration rounds its values in a unit per work item, chosen
+    # from the item's window, and the scatter's order decides which events
+    # share an item: two runs differ by that rounding (~1e-6 here, ~3e5 adds
+    # per pixel make it ~1e-7 at 512 x 512)
+    assert rel_l2(t[5], f[5]) < 5e-6
 
 
 @pytest.mark.parametrize("n_pix,size,bands", [(160, 0.004, False), (200, 0.002, False),
@@ -293,7 +297,9 @@ def test_horner_walks_against_chain(gpu_pkg, n_pix, size, bands):
     assert rel_l2(gt, gf) < 1e-6
     np.testing.assert_allclose(st_, sf, rtol=1e-4)
     assert llt == pytest.approx(llf, rel=1e-8)
-    assert rel_l2(it, if_) < 2e-6
+    # FP32 images of two runs: within the per-item rounding of the FP32
+    # iteration (test_row_table_identical); the deterministic ones closer
+    assert rel_l2(it, if_) < 5e-6
     assert rel_l2(dt, df) < 2e-6
     lam64 = ol.loglik_f64(sc, ev, rho, want_lambda=True)[1]
     assert rel_l2(lt, lam64) < 2e-6
@@ -390,7 +396,9 @@ def test_banded_matches_unbanded(gpu_pkg, n_pix, size):
         assert b[3][0] == pytest.approx(a[3][0], rel=1e-5)
         assert b[3][1] == pytest.approx(a[3][1], rel=1e-5)
         assert b[4] == pytest.approx(a[4], rel=1e-7)
-        assert rel_l2(b[5], a[5]) < 1e-6
+        # FP32 images: band items round in their own units (the FP32
+        # iteration's per-item unit, test_row_table_identical)
+        assert rel_l2(b[5], a[5]) < 5e-6
 
 
 def test_banded_tall_boxes_vs_oracle(gpu_pkg):
