@@ -37,30 +37,39 @@ VALU_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 
 
-def atomic_report(pkg, st, n_pix, kernel_ms):
+def atomic_report(pkg, st, n_pix, kernel_ms, accumulation="fp32"):
     """Atomic updates of one MLEM iteration against the device's measured
-    atomic rates.  Pass 2 makes one LDS ds_add_u64 per ellipse pixel; the
-    flush at most one global atomic per window pixel (zero pixels are
-    skipped).  The reference's GPU path makes one global atomic per ellipse
-    pixel (src/2d/strip/cuda/warp_granularity.cuh:145-150).  Rates are over the
-    whole kernel time, so each fraction is the share of the kernel that the
-    updates would take at the probe's rate."""
+    atomic rates.  Pass 2 makes one LDS atomic per ellipse pixel (ds_add_u32
+    in the FP32 iteration's 32-bit windows, ds_add_u64 in the other modes);
+    the flush at most one global atomic per window pixel (zero pixels are
+    skipped; FP64 in the FP32 and compensated iterations).  The reference's
+    GPU path makes one global atomic per ellipse pixel
+    (src/2d/strip/cuda/warp_granularity.cuh:145-150), priced at the FP32
+    rate.  Rates are over the whole kernel time, so each fraction is the
+    share of the kernel that the updates would take at the probe's rate."""
+    lds_kind = "u32" if accumulation == "fp32" else "u64"
+    flush_kind = "u64" if accumulation == "fixed64" else "f64"
     try:
-        g_rate, _ = pkg.probe_atomics(pkg.PROBE_GLOBAL_F32, n_pix, n_pix, reps=1024)
-        l_rate, _ = pkg.probe_atomics(pkg.PROBE_LDS_U64, reps=16384)
+        g32, _ = pkg.probe_atomics(pkg.PROBE_GLOBAL_F32, n_pix, n_pix, reps=1024)
+        g_rate, _ = pkg.probe_atomics(
+            pkg.PROBE_GLOBAL_U64 if flush_kind == "u64" else pkg.PROBE_GLOBAL_F64,
+            n_pix, n_pix, reps=1024)
+        l_rate, _ = pkg.probe_atomics(
+            pkg.PROBE_LDS_U32 if lds_kind == "u32" else pkg.PROBE_LDS_U64, reps=16384)
     except (ImportError, OSError, pkg.PetstripError) as e:
         return {"error": str(e)}
     sec = kernel_ms * 1e-3
     n_lds = float(st["n_ellipse_pixels"])
     n_glob = float(st["n_window_pixels"])
     return {
-        "lds_per_iteration": n_lds, "lds_rate": n_lds / sec,
+        "lds_kind": lds_kind, "lds_per_iteration": n_lds, "lds_rate": n_lds / sec,
         "lds_peak": l_rate, "lds_frac": n_lds / sec / l_rate,
+        "flush_kind": flush_kind,
         "global_per_iteration_max": n_glob, "global_rate_max": n_glob / sec,
-        "global_peak_f32": g_rate, "global_frac_max": n_glob / sec / g_rate,
+        "global_peak": g_rate, "global_frac_max": n_glob / sec / g_rate,
         "updates_absorbed_in_lds": 1.0 - n_glob / max(n_lds, 1.0),
         "reference_global_per_iteration": n_lds,
-        "reference_global_time_ms_at_peak": n_lds / g_rate * 1e3,
+        "reference_global_time_ms_at_peak": n_lds / g32 * 1e3,
     }
 
 
@@ -545,7 +554,7 @@ def main():
                       "ellipse_px_per_event": st["n_ellipse_pixels"] / max(n_ev, 1)},
         "setup_s": {"generate": t_gen, "upload_and_bin": t_setup},
     }
-    out["atomics"] = atomic_report(pkg, st, n_pix, k_ms)
+    out["atomics"] = atomic_report(pkg, st, n_pix, k_ms, args.accumulation)
     # the same events generated on the device (Philox stream of the host
     # generator) straight into a fresh handle's buffers, binned
     phantom, seed = pkg.WORKLOADS[name][4], pkg.WORKLOADS[name][5]
