@@ -305,3 +305,28 @@ def test_loglik_independent_of_sigma_history(gpu_pkg):
             rec.set_sigma(sz, sdl, freeze_support=True)
             _, g, ll = rec.backward(want_rho=False, want_loglik=True)
             assert ll == ll0 and tuple(g) == tuple(g0), k
+
+
+@pytest.mark.parametrize("bands", [0, 1])
+def test_gradient_cold_region(gpu_pkg, bands):
+    """An image with a square 1e12 times colder than the rest: the events
+    whose ellipses lie in it have tiny denominators, and the values
+    K / lambda_e of their rows exceed the range of the 64-bit windows, so
+    those rows add to the image directly.  Image gradient, sigma gradient
+    and log-likelihood against the FP64 oracle."""
+    pkg = gpu_pkg
+    sc, ev, rho, _ = setup(pkg, n=30000, seed=7)
+    rho[20:44, 20:44] = 1e-12
+    with pkg.Reconstruction(sc, force_bands=bands) as rec:
+        rec.set_events(ev)
+        rec.rho = rho
+        grad, gsig, ll = rec.backward(want_loglik=True)
+    ll_ref, _, g_ref, gs_ref = ol.loglik_f64(sc, ev, rho)
+    print(f"case: bands {bands}: sigma {gsig} vs {gs_ref}, loglik {ll} vs {ll_ref}",
+          flush=True)
+    # the cold square holds the largest gradient values
+    assert g_ref[20:44, 20:44].max() > 1e9
+    assert rel(grad, g_ref) < 1e-5
+    assert gsig[0] == pytest.approx(gs_ref[0], rel=2e-4)
+    assert gsig[1] == pytest.approx(gs_ref[1], rel=2e-4)
+    assert ll == pytest.approx(ll_ref, rel=1e-6)

@@ -301,8 +301,10 @@ def test_horner_walks_against_chain(gpu_pkg, n_pix, size, bands):
     # iteration (test_row_table_identical); the deterministic ones closer
     assert rel_l2(it, if_) < 5e-6
     assert rel_l2(dt, df) < 2e-6
-    lam64 = ol.loglik_f64(sc, ev, rho, want_lambda=True)[1]
+    _, lam64, _, gs64 = ol.loglik_f64(sc, ev, rho, want_lambda=True)
+    print(f"sigma gradient: {st_} (chain {sf}), oracle {gs64}", flush=True)
     assert rel_l2(lt, lam64) < 2e-6
+    np.testing.assert_allclose(st_, gs64, rtol=2e-4)
 
 
 def test_per_event_compensated_denominators(gpu_pkg):
