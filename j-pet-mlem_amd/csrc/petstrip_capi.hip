@@ -185,8 +185,15 @@ struct petstrip_handle {
   float* d_part = nullptr;
   float* d_lam_sorted = nullptr;
   size_t cap_part = 0;
-  int* d_flag = nullptr;  // device overflow flag of the fixed-point image
-  unsigned long long* d_nzero = nullptr;  // events whose last denominator was not > 0
+  // two words on the device: the events whose last denominator was not > 0,
+  // then the overflow flag of the fixed-point image (an int); read back
+  // together into h_status (pinned) at the end of an iterate call
+  unsigned long long* d_nzero = nullptr;
+  int* d_flag = nullptr;  // = (int*)(d_nzero + 1)
+  unsigned long long* h_status = nullptr;
+  // the denominators of the events outside the bins (empty boxes) are 0:
+  // the unbinned iteration writes those of the binned events only
+  bool lambda_clean = false;
   // sum of rho, exact after set_rho and an estimate after iterations
   // (sum rho' = events with a positive denominator); it sets the
   // fixed-point unit of the log-likelihood gradient
@@ -383,6 +390,7 @@ int build_row_table(petstrip_handle* h) {
 int prepare_fused(petstrip_handle* h) {
   h->fused_ok = false;
   h->n_items = 0;
+  h->lambda_clean = false;
   int rc0;
   if ((rc0 = update_kernel_table(h))) return rc0;
   const int64_t n = h->n_events;
@@ -753,7 +761,12 @@ int one_iteration(petstrip_handle* h) {
       a.log2_scale = (float)k;
       a.inv_fixed_scale = std::ldexp(1.0, -k);
     }
-    HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
+    // the unbanded launch writes every binned event's denominator; the
+    // others stay at the 0 of the last fill
+    if (h->banded || !h->lambda_clean) {
+      HIP_TRY(petstrip::launch_fill(h->stream, h->d_lambda, 0.0f, n));
+      h->lambda_clean = true;
+    }
     HIP_TRY(hipEventRecord(h->evk0, h->stream));
     if (h->banded) {
       int rc;
@@ -770,7 +783,7 @@ int one_iteration(petstrip_handle* h) {
     double unit = a.acc_kind == petstrip::GLOBAL_U64 ? a.inv_fixed_scale : 1.0;
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, a.acc_kind, unit, 0.0f, 1,
                                       h->d_flag, h->d_out, (int64_t)h->n_pix,
-                                      k_form ? h->d_rho : nullptr));
+                                      k_form ? h->d_rho : nullptr, h->d_nzero));
     std::swap(h->d_rho, h->d_out);
     h->lambda_valid = false;
     // the next unit's image scale: sum rho' = events with a positive
@@ -784,6 +797,7 @@ int one_iteration(petstrip_handle* h) {
   const int accm = h->opts.accumulation;
   const bool fixed = accm == PETSTRIP_ACCUM_FIXED64;
   HIP_TRY(hipEventRecord(h->evk0, h->stream));
+  h->lambda_clean = false;
   HIP_TRY(petstrip::launch_denominator(h->stream, h->sg, h->sk, h->d_geom, n,
                                        h->d_rho, denominator_weight(h),
                                        h->d_lambda, h->d_inv, h->opts.kernel,
@@ -797,7 +811,7 @@ int one_iteration(petstrip_handle* h) {
   HIP_TRY(petstrip::launch_finalize(
       h->stream, h->d_acc, fixed ? petstrip::GLOBAL_U64 : petstrip::GLOBAL_F64,
       fixed ? std::ldexp(1.0, -kFixed64Bits) : 1.0, 0.0f, fixed ? 1 : 0,
-      fixed ? h->d_flag : nullptr, h->d_out, (int64_t)h->n_pix));
+      fixed ? h->d_flag : nullptr, h->d_out, (int64_t)h->n_pix, nullptr, h->d_nzero));
   std::swap(h->d_rho, h->d_out);
   h->lambda_valid = false;
   return PETSTRIP_OK;
@@ -870,9 +884,13 @@ int petstrip_create(const petstrip_scanner* scanner, int device,
   if ((rc = dalloc(&h->d_sens, h->n_pix))) return bail(rc);
   if ((rc = dalloc(&h->d_scalars, 8))) return bail(rc);
   if ((rc = dalloc((double**)&h->d_acc, h->n_pix))) return bail(rc);
-  if ((rc = dalloc(&h->d_flag, 1))) return bail(rc);
-  if ((rc = dalloc(&h->d_nzero, 1))) return bail(rc);
-  if (hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream) != hipSuccess)
+  if ((rc = dalloc(&h->d_nzero, 2))) return bail(rc);
+  h->d_flag = reinterpret_cast<int*>(h->d_nzero + 1);
+  if (hipHostMalloc((void**)&h->h_status, 2 * sizeof(unsigned long long)) != hipSuccess) {
+    h->h_status = nullptr;
+    return bail(fail(PETSTRIP_EDEVICE, "hipHostMalloc failed"));
+  }
+  if (hipMemsetAsync(h->d_nzero, 0, 2 * sizeof(unsigned long long), h->stream) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipMemset failed"));
   if (hipMemsetAsync(h->d_acc, 0, h->n_pix * sizeof(double), h->stream) != hipSuccess)
     return bail(fail(PETSTRIP_EDEVICE, "hipMemset failed"));
@@ -916,8 +934,10 @@ int petstrip_destroy(petstrip_handle* h) {
   dfree(h->d_sgeom);
   dfree(h->d_sidx);
   dfree(h->d_items);
-  dfree(h->d_flag);
   dfree(h->d_nzero);
+  h->d_flag = nullptr;
+  if (h->h_status) (void)hipHostFree(h->h_status);
+  h->h_status = nullptr;
   dfree(h->d_gtab);
   dfree(h->d_mtab);
   dfree(h->d_part);
@@ -1081,13 +1101,17 @@ int petstrip_iterate(petstrip_handle* h, int n_iterations) {
     if ((rc = one_iteration(h))) return rc;
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
   }
-  if (n_iterations > 0)
+  // the last iteration's finalize cleared the counter; counter and overflow
+  // flag come back in one copy
+  if (n_iterations > 0) {
     HIP_TRY(petstrip::launch_count_nonpositive(h->stream, h->d_lambda, h->n_events,
-                                               h->d_nzero));
+                                               h->d_nzero, false));
+    HIP_TRY(hipMemcpyAsync(h->h_status, h->d_nzero, 2 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, h->stream));
+  }
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (n_iterations > 0) {
-    unsigned long long nzero = 0;
-    HIP_TRY(hipMemcpy(&nzero, h->d_nzero, sizeof(nzero), hipMemcpyDeviceToHost));
+    const unsigned long long nzero = h->h_status[0];
     h->stats.n_zero_denominator = (int64_t)nzero;
     // sum rho' = events with a positive denominator (the CPU update; within a
     // small factor for the sensitivity-weighted one)
@@ -1098,7 +1122,7 @@ int petstrip_iterate(petstrip_handle* h, int n_iterations) {
     HIP_TRY(hipEventElapsedTime(&ms, h->evk0, h->evk1));
     h->stats.last_kernel_ms = ms;
     int flag = 0;
-    HIP_TRY(hipMemcpy(&flag, h->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+    memcpy(&flag, &h->h_status[1], sizeof(int));
     if (flag) {
       HIP_TRY(hipMemset(h->d_flag, 0, sizeof(int)));
       return fail(PETSTRIP_ESTATE,
@@ -1202,6 +1226,7 @@ int petstrip_forward(petstrip_handle* h, float* lambda) {
     HIP_TRY(hipMemsetAsync(h->d_scalars, 0, 8 * sizeof(double), h->stream));
     if ((rc = fused_forward(h, a, false))) return rc;
   } else {
+    h->lambda_clean = false;
     HIP_TRY(petstrip::launch_denominator(
         h->stream, h->sg, h->sk, h->d_geom, n, h->d_rho, nullptr, h->d_lambda, h->d_inv,
         h->opts.kernel, h->opts.accumulation == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0));

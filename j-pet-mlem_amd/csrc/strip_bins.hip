@@ -91,11 +91,14 @@ __global__ __launch_bounds__(256) void k_item_windows(const EventGeom* __restric
 }
 
 // rho' from the accumulator (times mul[i], the image, for sums of K /
-// lambda_e); the accumulator is cleared for the next use.
+// lambda_e); the accumulator is cleared for the next use, and so is the
+// counter *clear when given.
 __global__ void k_finalize(void* __restrict__ acc, int kind, double unit, float bias,
                            int check_sign, int* __restrict__ overflow,
-                           float* __restrict__ out, int64_t n, const float* __restrict__ mul) {
+                           float* __restrict__ out, int64_t n, const float* __restrict__ mul,
+                           unsigned long long* __restrict__ clear) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && clear) *clear = 0ull;
   if (i >= n) return;
   const double u = mul ? unit * (double)mul[i] : unit;
   if (kind == GLOBAL_F32) {
@@ -146,9 +149,9 @@ hipError_t launch_item_windows(hipStream_t st, const EventGeom* sgeom,
 
 hipError_t launch_finalize(hipStream_t st, void* acc, int kind, double unit,
                            float bias, int check_sign, int* overflow, float* out,
-                           int64_t n, const float* mul) {
+                           int64_t n, const float* mul, unsigned long long* clear) {
   hipLaunchKernelGGL(k_finalize, dim3(nblk(n, 256)), dim3(256), 0, st, acc, kind,
-                     unit, bias, check_sign, overflow, out, n, mul);
+                     unit, bias, check_sign, overflow, out, n, mul, clear);
   return hipGetLastError();
 }
 

@@ -168,9 +168,10 @@ hipError_t launch_band_combine(hipStream_t st, const FusedArgs& args, int64_t n_
 // out = acc * unit (* mul, when given) + bias (acc read as kind; 64-bit
 // values as signed), the accumulator cleared; *overflow (nullable) set to 1
 // if a 64-bit value is negative where only non-negative sums are expected
-// (check_sign).
+// (check_sign); *clear (nullable) set to 0.
 hipError_t launch_finalize(hipStream_t st, void* acc, int kind, double unit,
                            float bias, int check_sign, int* overflow, float* out,
-                           int64_t n, const float* mul = nullptr);
+                           int64_t n, const float* mul = nullptr,
+                           unsigned long long* clear = nullptr);
 
 }  // namespace petstrip

@@ -36,9 +36,9 @@ hipError_t launch_sigma_grad(hipStream_t st, const StripConst& sg,
 hipError_t launch_fill(hipStream_t st, float* p, float v, int64_t n);
 hipError_t launch_add_scalar(hipStream_t st, float* p, float v, int64_t n);
 // *count = the number of entries of x[0, n) that are not > 0 (the counter
-// is cleared by a kernel first).
+// is cleared by a kernel first, unless clear is false: then it must hold 0).
 hipError_t launch_count_nonpositive(hipStream_t st, const float* x, int64_t n,
-                                    unsigned long long* count);
+                                    unsigned long long* count, bool clear = true);
 hipError_t launch_loglik_weights(hipStream_t st, const float* lambda,
                                  int64_t n, float* g, double* ll);
 hipError_t launch_sensitivity(hipStream_t st, const StripConst& s, int mode,

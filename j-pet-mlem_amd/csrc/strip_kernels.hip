@@ -478,8 +478,8 @@ hipError_t launch_fill(hipStream_t st, float* p, float v, int64_t n) {
 }
 
 hipError_t launch_count_nonpositive(hipStream_t st, const float* x, int64_t n,
-                                    unsigned long long* count) {
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, count);
+                                    unsigned long long* count, bool clear) {
+  if (clear) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, count);
   if (n > 0)
     hipLaunchKernelGGL(k_count_nonpositive, dim3(std::min<unsigned>(blocks_for(n, 256), 2048)),
                        dim3(256), 0, st, x, n, count);

@@ -165,6 +165,30 @@ def test_edge_events(gpu_pkg):
         assert rel_l2(out, ref) < 1e-6
 
 
+def test_zero_denominator_count_after_new_events(gpu_pkg):
+    """The count of events without a positive denominator (stats
+    n_zero_denominator; it sets the next iteration's fixed-point unit) after
+    each iterate call, over event sets of different sizes with and without
+    events outside the strips on one handle: the denominators of the events
+    outside the bins, which the binned iteration does not write, read 0."""
+    pkg = gpu_pkg
+    sc, ev = small(pkg, n=20000)
+    far = ev.copy()
+    far[::3, :2] = 5.0  # a third of the events far outside the strips
+    with pkg.Reconstruction(sc) as rec:
+        for evs in (ev, far, ev[:15000], far[:12000], ev):
+            rec.set_events(evs)
+            rec.rho = np.ones(rec.shape, np.float32)
+            for k in range(2):
+                rho_in = rec.rho
+                rec(1)
+                ref = ol.denominators_f32(sc, evs, rho_in)
+                want = int(np.count_nonzero(~(ref > 0)))
+                got = rec.stats()["n_zero_denominator"]
+                print(f"case: {len(evs)} events, call {k}: {got} vs {want}", flush=True)
+                assert got == want
+
+
 def test_run_callbacks(gpu_pkg):
     sc, ev = small(gpu_pkg, n=5000)
     seen, prog = [], []
