@@ -722,9 +722,17 @@ __device__ __forceinline__ RowEntry load_entry(const RowEntry* p) {
 // takes PACK events at a time, one per aligned segment of 64 / PACK lanes, so
 // that short events do not leave most lanes idle; the denominator is then a
 // segment sum and every per-event value lives per lane.
-template <int MODE, bool COMP, int HP, int BAND, int PACK>
+//
+// LEAN: the unbanded walks under the default options, fixed at compile time:
+// rows from the row table with its kernel values, pass 1 by Horner's rule
+// where the mode has it.  Without the other options' code the kernel keeps
+// its uniform values in SGPRs instead of spilling them to VGPR lanes (the
+// 64-row-stride kernels have 78 SGPRs at 8 waves per SIMD), and the 128-row
+// stride kernel of the iteration drops from 95 to 54 VGPRs.
+template <int MODE, bool COMP, int HP, int BAND, int PACK, bool LEAN>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(PACK > 1 || HP == 64 ? 8 : 1)))
 void k_strip_fused(FusedArgs a) {
+  static_assert(!LEAN || BAND == 0, "the row table serves the unbanded walks");
   static_assert(PACK == 1 || (BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
                                             MODE == MODE_MLEM_KA || MODE == MODE_GRAD)),
                 "packed events: the unbanded iteration and gradient");
@@ -748,7 +756,7 @@ void k_strip_fused(FusedArgs a) {
   // sums and no sigma moments (the chain issues fewer instructions for those);
   // a.horner = 0 keeps the chain (forward_rows) for comparison
   constexpr bool kHorner = kPass1 && !COMP && !kAnalytic && !kSigma;
-  const bool horner = kHorner && a.horner != 0;
+  const bool horner = kHorner && (LEAN || a.horner != 0);
   using U = unsigned long long;
   // the iteration's K / lambda sums accumulate in 32-bit windows (the item's
   // unit and cap below keep every window sum below 2^32); the gradient's (its
@@ -860,8 +868,8 @@ void k_strip_fused(FusedArgs a) {
   // when packed), in the order the walks visit the rows.  The entries of
   // groups 0 and 1 (when the stride has them) of the wave's next events load
   // while the current ones are walked.
-  const bool use_tab = BAND == 0 && a.rowtab != nullptr;
-  const bool tab_kernel = a.rowtab_kernel != 0;
+  const bool use_tab = LEAN || (BAND == 0 && a.rowtab != nullptr);
+  const bool tab_kernel = LEAN || a.rowtab_kernel != 0;
   constexpr int kPre = PACK == 1 && kCached > 1 ? 2 : 1;
   const bool pre1 = kPre > 1 && a.rowtab_stride > 64;
   RowEntry pre[kPre];
@@ -1338,7 +1346,12 @@ template <int MODE, bool COMP, int HP, int BAND, int PACK>
 hipError_t launch_range(hipStream_t st, const FusedArgs& a, int i_begin, int i_end,
                         size_t lds) {
   if (i_end <= i_begin) return hipSuccess;
-  auto kern = k_strip_fused<MODE, COMP, HP, BAND, PACK>;
+  // the default options' form (LEAN) when the launch has them
+  auto kern = k_strip_fused<MODE, COMP, HP, BAND, PACK, false>;
+  if constexpr (BAND == 0) {
+    if (a.rowtab != nullptr && a.rowtab_kernel != 0 && a.horner != 0)
+      kern = k_strip_fused<MODE, COMP, HP, BAND, PACK, true>;
+  }
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
