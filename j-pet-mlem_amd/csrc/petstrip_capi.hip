@@ -158,8 +158,8 @@ struct petstrip_handle {
   petstrip::WorkItem* d_items = nullptr;
   size_t cap_items = 0;
   int n_items = 0;
-  // first items of the boxes at most 32 and at most 16 rows tall
-  int pack_begin[2] = {0, 0};
+  // the unbanded launches' item ranges (height class x window width class)
+  petstrip::ItemRanges ranges;
   int64_t window_px = 0;  // sum over work items of window pixels
   int max_rows = 1;
   size_t lds_bytes = 0;
@@ -391,6 +391,7 @@ int prepare_fused(petstrip_handle* h) {
   h->fused_ok = false;
   h->n_items = 0;
   h->lambda_clean = false;
+  h->ranges = petstrip::ItemRanges{};
   int rc0;
   if ((rc0 = update_kernel_table(h))) return rc0;
   const int64_t n = h->n_events;
@@ -403,16 +404,20 @@ int prepare_fused(petstrip_handle* h) {
   // than its tallest box: when that takes it past the column stride the
   // boxes alone need (64-row multiples), the events are binned again with
   // bins low enough to keep it, which halves (or better) the window's LDS.
-  auto bin_events = [&](int bin_h) -> int {
+  // wide_cols: boxes wider than this bin apart from the others (their
+  // windows need more LDS than two workgroups per CU leave)
+  constexpr int kClasses = petstrip::kHeightClasses * petstrip::kWidthClasses;
+  int class_first[kClasses + 1] = {};
+  auto bin_events = [&](int bin_h, int wide_cols) -> int {
     const int nbx = (W + kBinW - 1) / kBinW, nby = (H + bin_h - 1) / bin_h;
     const int n_spatial = nbx * nby;
-    const int nb = n_spatial * petstrip::kHeightClasses;
+    const int nb = n_spatial * kClasses;
     dfree(h->d_count);
     int rc1;
     if ((rc1 = dalloc(&h->d_count, (size_t)nb * 2))) return rc1;
     HIP_TRY(hipMemsetAsync(h->d_count, 0, (size_t)nb * 2 * sizeof(int32_t), h->stream));
     HIP_TRY(petstrip::launch_bin_count(h->stream, h->d_geom, n, kBinW, bin_h, nbx, n_spatial,
-                                       h->d_bin, h->d_count));
+                                       wide_cols, h->d_bin, h->d_count));
     std::vector<int32_t> cnt((size_t)nb), off((size_t)nb);
     HIP_TRY(hipMemcpyAsync(cnt.data(), h->d_count, nb * sizeof(int32_t),
                            hipMemcpyDeviceToHost, h->stream));
@@ -420,9 +425,8 @@ int prepare_fused(petstrip_handle* h) {
     acc = 0;
     items.clear();
     for (int b = 0; b < nb; ++b) {
-      // the first items of the 17-32 and the at most 16 row classes
-      if (b == n_spatial) h->pack_begin[0] = (int)items.size();
-      if (b == 2 * n_spatial) h->pack_begin[1] = (int)items.size();
+      // the first item of each (height, width) class
+      if (b % n_spatial == 0) class_first[b / n_spatial] = (int)items.size();
       off[(size_t)b] = (int32_t)acc;
       for (int64_t s0 = acc; s0 < acc + cnt[(size_t)b]; s0 += kChunk) {
         petstrip::WorkItem it{};
@@ -432,6 +436,7 @@ int prepare_fused(petstrip_handle* h) {
       }
       acc += cnt[(size_t)b];
     }
+    class_first[kClasses] = (int)items.size();
     int32_t* d_cursor = h->d_count + nb;
     HIP_TRY(hipMemcpyAsync(d_cursor, off.data(), nb * sizeof(int32_t),
                            hipMemcpyHostToDevice, h->stream));
@@ -463,16 +468,32 @@ int prepare_fused(petstrip_handle* h) {
     }
     return PETSTRIP_OK;
   };
-  if ((rc = bin_events(kBinH))) return rc;
+  constexpr int kNoSplit = 1 << 30;
+  if ((rc = bin_events(kBinH, kNoSplit))) return rc;
   {
     // a bin's centres span bin_h rows, so its window spans at most
     // max_rows + bin_h - 1 rows
     const int want = petstrip::fused_col_stride(max_rows);
     int bh = kBinH;
     while (bh > 1 && max_rows + bh - 1 > want) bh >>= 1;
-    if (want != 0 && petstrip::fused_col_stride(max_wh) != want && bh < kBinH &&
-        max_rows + bh - 1 <= want && (rc = bin_events(bh)))
-      return rc;
+    const bool lower = want != 0 && petstrip::fused_col_stride(max_wh) != want &&
+                       bh < kBinH && max_rows + bh - 1 <= want;
+    if (!lower) bh = kBinH;
+    // Windows too wide for two workgroups per CU (the 32-bit windows of the
+    // iteration): the narrower boxes bin apart, so that their items' windows
+    // fit (a window spans its boxes plus the bin's kBinW centre columns and
+    // the rounding of the centres).
+    const int hp = lower ? want : petstrip::fused_col_stride(max_wh);
+    int wide_cols = kNoSplit;
+    if (hp != 0 &&
+        petstrip::fused_lds_bytes(max_ww, hp, petstrip::MODE_MLEM_K) > petstrip::kTwoPerCuLds) {
+      int t4 = 0;
+      while (petstrip::fused_lds_bytes(t4 + 1, hp, petstrip::MODE_MLEM_K) <=
+             petstrip::kTwoPerCuLds)
+        ++t4;
+      if (t4 - (kBinW + 3) >= 16) wide_cols = t4 - (kBinW + 3);
+    }
+    if ((lower || wide_cols != kNoSplit) && (rc = bin_events(bh, wide_cols))) return rc;
   }
   h->max_rows = max_rows;
   h->max_ww = max_ww;
@@ -557,8 +578,7 @@ int prepare_fused(petstrip_handle* h) {
       }
       items.swap(bands);
       h->n_items = (int)items.size();
-      // banded items walk one event per wave
-      h->pack_begin[0] = h->pack_begin[1] = h->n_items;
+      // banded items walk one event per wave (launch_strip_banded)
       if (items.size() > h->cap_items) {
         dfree(h->d_items);
         if ((rc = dalloc(&h->d_items, items.size()))) return rc;
@@ -572,6 +592,23 @@ int prepare_fused(petstrip_handle* h) {
   }
   h->window_px = 0;
   for (const auto& it : items) h->window_px += (int64_t)it.ww * it.wh;
+  // the unbanded launches: one range per non-empty (height, width) class
+  h->ranges = petstrip::ItemRanges{};
+  h->ranges.n_items = h->n_items;
+  if (!h->banded) {
+    for (int k = 0; k < kClasses; ++k) {
+      const int b0 = class_first[k], b1 = class_first[k + 1];
+      if (b1 <= b0) continue;
+      int mw = 1;
+      for (int i = b0; i < b1; ++i) mw = std::max(mw, items[(size_t)i].ww);
+      const int cls = k / petstrip::kWidthClasses;
+      const int r = h->ranges.n++;
+      h->ranges.begin[r] = b0;
+      h->ranges.end[r] = b1;
+      h->ranges.pack[r] = cls == 0 ? 1 : cls == 1 ? 2 : 4;
+      h->ranges.max_ww[r] = mw;
+    }
+  }
   {
     const size_t need = (size_t)std::max<int64_t>(h->n_items, petstrip::kCombineBlocks) *
                         petstrip::kScalStride;
@@ -684,9 +721,8 @@ petstrip::FusedArgs fused_args(petstrip_handle* h) {
 // lambda and the scalars.
 int fused_forward(petstrip_handle* h, const petstrip::FusedArgs& a, bool sigma) {
   if (!h->banded)
-    return hiperr(petstrip::launch_strip_fused(
-        h->stream, petstrip::MODE_FORWARD, h->col_stride, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_FORWARD)));
+    return hiperr(petstrip::launch_strip_fused(h->stream, petstrip::MODE_FORWARD,
+                                               h->col_stride, 0, a, h->ranges));
   const size_t part = (size_t)std::max<int64_t>(h->n_sorted, 1) * h->nbands * 4;
   HIP_TRY(hipMemsetAsync(h->d_part, 0, part * sizeof(float), h->stream));
   HIP_TRY(petstrip::launch_strip_banded(
@@ -773,10 +809,9 @@ int one_iteration(petstrip_handle* h) {
       const size_t lds2 = petstrip::fused_lds_bytes(h->max_ww, h->col_stride, mode2, 2);
       if ((rc = banded_pipeline(h, a, petstrip::MODE_MLEM, false, mode2, lds2))) return rc;
     } else {
-      HIP_TRY(petstrip::launch_strip_fused(
-          h->stream, mode2, h->col_stride, accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
-          h->n_items, petstrip::fused_lds_bytes(h->max_ww, h->col_stride, mode2),
-          h->pack_begin));
+      HIP_TRY(petstrip::launch_strip_fused(h->stream, mode2, h->col_stride,
+                                           accm == PETSTRIP_ACCUM_COMPENSATED ? 1 : 0, a,
+                                           h->ranges));
     }
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     // float accumulators already hold image units; the u64 one holds 2^-40
@@ -1316,10 +1351,8 @@ int gradient_fused(petstrip_handle* h, float* grad_rho, double grad_sigma[2],
              petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD, 2))))
       return rc;
   } else {
-    HIP_TRY(petstrip::launch_strip_fused(
-        h->stream, petstrip::MODE_GRAD, h->col_stride, 0, a, h->n_items,
-        petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_GRAD),
-        h->pack_begin));
+    HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_GRAD, h->col_stride, 0,
+                                         a, h->ranges));
   }
   HIP_TRY(hipEventRecord(h->evk1, h->stream));
   HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_F64, 1.0, -1.0f,
@@ -1396,9 +1429,8 @@ int backward_fused(petstrip_handle* h, const float* g, float* grad_rho,
           h->stream, petstrip::MODE_BACKWARD, 2, a, h->n_items,
           petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_BACKWARD)));
     else
-      HIP_TRY(petstrip::launch_strip_fused(
-          h->stream, petstrip::MODE_BACKWARD, h->col_stride, 0, a, h->n_items,
-          petstrip::fused_lds_bytes(h->max_ww, h->col_stride, petstrip::MODE_BACKWARD)));
+      HIP_TRY(petstrip::launch_strip_fused(h->stream, petstrip::MODE_BACKWARD,
+                                           h->col_stride, 0, a, h->ranges));
     HIP_TRY(hipEventRecord(h->evk1, h->stream));
     HIP_TRY(petstrip::launch_finalize(h->stream, h->d_acc, petstrip::GLOBAL_U64,
                                       a.inv_fixed_scale, bias, 0, nullptr, h->d_out,

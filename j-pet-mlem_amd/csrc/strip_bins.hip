@@ -11,9 +11,10 @@ namespace petstrip {
 
 
 // Bin of an event: its box height class (taller than 32 rows, 17-32, at
-// most 16), then the bin of its box centre; empty boxes get -1.
+// most 16) and width class (at most wide_cols columns, wider), then the bin
+// of its box centre; empty boxes get -1.
 __global__ void k_bin_count(const EventGeom* __restrict__ geom, int64_t n,
-                            int bin_w, int bin_h, int nbx, int n_spatial,
+                            int bin_w, int bin_h, int nbx, int n_spatial, int wide_cols,
                             int32_t* __restrict__ bin, int32_t* __restrict__ count) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -23,7 +24,8 @@ __global__ void k_bin_count(const EventGeom* __restrict__ geom, int64_t n,
     int cx = (g.x0 + g.x1) >> 1, cy = (g.y0 + g.y1) >> 1;
     const int rows = g.y1 - g.y0;
     const int cls = rows <= 16 ? 2 : rows <= 32 ? 1 : 0;
-    b = cls * n_spatial + (cy / bin_h) * nbx + (cx / bin_w);
+    const int wcls = g.x1 - g.x0 > wide_cols ? 1 : 0;
+    b = (cls * kWidthClasses + wcls) * n_spatial + (cy / bin_h) * nbx + (cx / bin_w);
     atomicAdd(&count[b], 1);
   }
   bin[e] = b;
@@ -123,11 +125,11 @@ __global__ void k_finalize(void* __restrict__ acc, int kind, double unit, float 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_bin_count(hipStream_t st, const EventGeom* geom, int64_t n,
-                            int bin_w, int bin_h, int nbx, int n_spatial, int32_t* bin,
-                            int32_t* count) {
+                            int bin_w, int bin_h, int nbx, int n_spatial, int wide_cols,
+                            int32_t* bin, int32_t* count) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bin_count, dim3(nblk(n, 256)), dim3(256), 0, st, geom, n,
-                     bin_w, bin_h, nbx, n_spatial, bin, count);
+                     bin_w, bin_h, nbx, n_spatial, wide_cols, bin, count);
   return hipGetLastError();
 }
 

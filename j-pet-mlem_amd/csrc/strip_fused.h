@@ -93,13 +93,16 @@ struct FusedArgs {
 constexpr int kScalStride = 8;       // doubles per workgroup in scal_part
 constexpr int kCombineBlocks = 1024; // workgroups of k_band_combine (grid-stride)
 
-// Bin of an event: (height class) x (spatial bin of its box centre), the
-// classes in the order of boxes taller than 32 rows, 17-32 rows, at most 16
-// (n_spatial spatial bins per class); empty boxes get -1.
+// Bin of an event: (height class, width class) x (spatial bin of its box
+// centre), the height classes in the order of boxes taller than 32 rows,
+// 17-32 rows, at most 16, each with the boxes at most wide_cols columns wide
+// first, then the wider ones (n_spatial spatial bins per class pair); empty
+// boxes get -1.
 constexpr int kHeightClasses = 3;
+constexpr int kWidthClasses = 2;
 hipError_t launch_bin_count(hipStream_t st, const EventGeom* geom, int64_t n,
-                            int bin_w, int bin_h, int nbx, int n_spatial, int32_t* bin,
-                            int32_t* count);
+                            int bin_w, int bin_h, int nbx, int n_spatial, int wide_cols,
+                            int32_t* bin, int32_t* count);
 hipError_t launch_bin_scatter(hipStream_t st, const EventGeom* geom, int64_t n,
                               const int32_t* bin, int32_t* cursor,
                               EventGeom* sgeom, int32_t* sidx);
@@ -125,6 +128,9 @@ constexpr float kLog2Sum32 = 31.9f;
 constexpr float kItemHeadroomBits = 9.0f;
 constexpr int kLdsHead = 128;        // LDS bytes before the windows (two
                                      // floats per wave)
+// LDS per workgroup at which two 16-wave workgroups share a CU's 160 KB
+// (less a 512-byte allocation granule)
+constexpr size_t kTwoPerCuLds = 80 * 1024 - 512;
 
 // n_box and n_ell per event from the row intervals of the fused walks
 // (valid when rows are hole-free intervals); scan = 1 forces the scanning
@@ -145,14 +151,24 @@ size_t fused_lds_bytes(int max_ww, int col_stride, int mode, int band = 0);
 // Column stride for windows of at most max_wh rows: the next multiple of
 // 64, or 0 when above kMaxColStride.
 int fused_col_stride(int max_wh);
+// The items of an unbanded launch in ranges, each of one box height class
+// and one window width class: pack = 1, 2 or 4 for boxes taller than 32
+// rows, 17-32 rows or at most 16 (MLEM, MLEM_K, MLEM_KA and GRAD then walk
+// two or four of their events per wave at a time; the other modes one),
+// max_ww the range's widest window, which sizes its LDS (ranges of narrow
+// windows can run two workgroups per CU).  The ranges cover items 0 ..
+// n_items - 1 in order.
+constexpr int kMaxRanges = kHeightClasses * kWidthClasses;
+struct ItemRanges {
+  int n = 0;
+  int n_items = 0;
+  int begin[kMaxRanges] = {}, end[kMaxRanges] = {}, pack[kMaxRanges] = {},
+      max_ww[kMaxRanges] = {};
+};
 // MODE_FORWARD and MODE_GRAD also add their five scalars to args.scalars,
-// through args.scal_part (n_items * kScalStride doubles).  pack_begin
-// (nullable, two entries): the first item whose boxes are at most 32 rows
-// tall and the first whose boxes are at most 16 (items sorted so); MLEM,
-// MLEM_K and GRAD then walk two or four of their events per wave at a time.
+// through args.scal_part (n_items * kScalStride doubles).
 hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int compensated,
-                              const FusedArgs& args, int n_items, size_t lds_bytes,
-                              const int* pack_begin = nullptr);
+                              const FusedArgs& args, const ItemRanges& ranges);
 // Banded form (column stride kBandRows): band = 1 runs pass 1 into the band
 // partials (mode MODE_FORWARD with sigma sums, else lambda only), band = 2
 // runs pass 2 of mode (MLEM, GRAD or BACKWARD) with the combined lambda.

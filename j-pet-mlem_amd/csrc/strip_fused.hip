@@ -1368,43 +1368,46 @@ hipError_t launch_range(hipStream_t st, const FusedArgs& a, int i_begin, int i_e
   return hipSuccess;
 }
 
-// pack_begin (nullable): the first item of the boxes at most 32 rows tall
-// and of those at most 16 (items are in that order); the iteration and the
-// gradient pack two or four of their events into a wave.
+// The banded launches: every item in one range of one event per wave.
 template <int MODE, bool COMP, int HP, int BAND>
-hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds,
-                    const int* pack_begin = nullptr) {
-  hipError_t e;
-  constexpr bool kPackable = BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
-                                           MODE == MODE_MLEM_KA || MODE == MODE_GRAD);
-  if (kPackable && pack_begin) {
-    const int p2 = std::min(pack_begin[0], n_items), p4 = std::min(pack_begin[1], n_items);
-    if ((e = launch_range<MODE, COMP, HP, BAND, 1>(st, a, 0, p2, lds)) != hipSuccess ||
-        (e = launch_range<MODE, COMP, HP, BAND, kPackable ? 2 : 1>(st, a, p2, p4, lds)) !=
-            hipSuccess ||
-        (e = launch_range<MODE, COMP, HP, BAND, kPackable ? 4 : 1>(st, a, p4, n_items, lds)) !=
-            hipSuccess)
-      return e;
-  } else if ((e = launch_range<MODE, COMP, HP, BAND, 1>(st, a, 0, n_items, lds)) !=
-             hipSuccess) {
-    return e;
+hipError_t launch_t(hipStream_t st, const FusedArgs& a, int n_items, size_t lds) {
+  return launch_range<MODE, COMP, HP, BAND, 1>(st, a, 0, n_items, lds);
+}
+
+// The unbanded launches, range by range (ItemRanges): the iteration and the
+// gradient pack two or four of their events into a wave for the short boxes,
+// and each range's LDS follows its widest window.
+template <int MODE, bool COMP, int HP>
+hipError_t launch_u(hipStream_t st, const FusedArgs& a, const ItemRanges& r) {
+  constexpr bool kPackable = MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
+                             MODE == MODE_MLEM_KA || MODE == MODE_GRAD;
+  for (int i = 0; i < r.n; ++i) {
+    const int b = r.begin[i], e1 = std::min(r.end[i], r.n_items);
+    const size_t lds = fused_lds_bytes(r.max_ww[i], HP, MODE);
+    hipError_t e;
+    if (kPackable && r.pack[i] == 4)
+      e = launch_range<MODE, COMP, HP, 0, kPackable ? 4 : 1>(st, a, b, e1, lds);
+    else if (kPackable && r.pack[i] == 2)
+      e = launch_range<MODE, COMP, HP, 0, kPackable ? 2 : 1>(st, a, b, e1, lds);
+    else
+      e = launch_range<MODE, COMP, HP, 0, 1>(st, a, b, e1, lds);
+    if (e != hipSuccess) return e;
   }
-  if constexpr (BAND == 0 && (MODE == MODE_FORWARD || MODE == MODE_GRAD)) {
+  if constexpr (MODE == MODE_FORWARD || MODE == MODE_GRAD) {
     hipLaunchKernelGGL(k_scalars_finish, dim3(1), dim3(256), 0, st, a.scal_part,
-                       (int64_t)n_items, a.scalars);
+                       (int64_t)r.n_items, a.scalars);
     return hipGetLastError();
   }
   return hipSuccess;
 }
 
 template <int MODE, bool COMP>
-hipError_t launch_h(int col_stride, hipStream_t st, const FusedArgs& a, int n_items,
-                    size_t lds, const int* pack_begin) {
+hipError_t launch_h(int col_stride, hipStream_t st, const FusedArgs& a, const ItemRanges& r) {
   switch (col_stride) {
-    case 64: return launch_t<MODE, COMP, 64, 0>(st, a, n_items, lds, pack_begin);
-    case 128: return launch_t<MODE, COMP, 128, 0>(st, a, n_items, lds, pack_begin);
-    case 192: return launch_t<MODE, COMP, 192, 0>(st, a, n_items, lds, pack_begin);
-    case 256: return launch_t<MODE, COMP, 256, 0>(st, a, n_items, lds, pack_begin);
+    case 64: return launch_u<MODE, COMP, 64>(st, a, r);
+    case 128: return launch_u<MODE, COMP, 128>(st, a, r);
+    case 192: return launch_u<MODE, COMP, 192>(st, a, r);
+    case 256: return launch_u<MODE, COMP, 256>(st, a, r);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1479,27 +1482,19 @@ __global__ __launch_bounds__(256) void k_band_combine(const float* part, int64_t
 }  // namespace
 
 hipError_t launch_strip_fused(hipStream_t st, int mode, int col_stride, int compensated,
-                              const FusedArgs& a, int n_items, size_t lds_bytes,
-                              const int* pack_begin) {
-  if (n_items <= 0) return hipSuccess;
-  const int* pb = pack_begin;
+                              const FusedArgs& a, const ItemRanges& r) {
+  if (r.n_items <= 0) return hipSuccess;
   switch (mode) {
     case MODE_MLEM:
-      return compensated
-                 ? launch_h<MODE_MLEM, true>(col_stride, st, a, n_items, lds_bytes, pb)
-                 : launch_h<MODE_MLEM, false>(col_stride, st, a, n_items, lds_bytes, pb);
+      return compensated ? launch_h<MODE_MLEM, true>(col_stride, st, a, r)
+                         : launch_h<MODE_MLEM, false>(col_stride, st, a, r);
     case MODE_FORWARD:
-      return compensated
-                 ? launch_h<MODE_FORWARD, true>(col_stride, st, a, n_items, lds_bytes, pb)
-                 : launch_h<MODE_FORWARD, false>(col_stride, st, a, n_items, lds_bytes, pb);
-    case MODE_GRAD:
-      return launch_h<MODE_GRAD, false>(col_stride, st, a, n_items, lds_bytes, pb);
-    case MODE_MLEM_K:
-      return launch_h<MODE_MLEM_K, false>(col_stride, st, a, n_items, lds_bytes, pb);
-    case MODE_MLEM_KA:
-      return launch_h<MODE_MLEM_KA, false>(col_stride, st, a, n_items, lds_bytes, pb);
-    default:
-      return launch_h<MODE_BACKWARD, false>(col_stride, st, a, n_items, lds_bytes, pb);
+      return compensated ? launch_h<MODE_FORWARD, true>(col_stride, st, a, r)
+                         : launch_h<MODE_FORWARD, false>(col_stride, st, a, r);
+    case MODE_GRAD: return launch_h<MODE_GRAD, false>(col_stride, st, a, r);
+    case MODE_MLEM_K: return launch_h<MODE_MLEM_K, false>(col_stride, st, a, r);
+    case MODE_MLEM_KA: return launch_h<MODE_MLEM_KA, false>(col_stride, st, a, r);
+    default: return launch_h<MODE_BACKWARD, false>(col_stride, st, a, r);
   }
 }
 
