@@ -550,6 +550,8 @@ def main():
         "iteration_ms": {"median": float(np.median(iter_ms)),
                          "min": float(np.min(iter_ms))},
         "row_table_bytes": st["row_table_bytes"],
+        "work_items": {"total": st["n_work_items"],
+                       "two_per_cu": st["n_items_two_per_cu"]},
         "footprint": {"box_px_per_event": st["n_box_pixels"] / max(n_ev, 1),
                       "ellipse_px_per_event": st["n_ellipse_pixels"] / max(n_ev, 1)},
         "setup_s": {"generate": t_gen, "upload_and_bin": t_setup},

@@ -149,6 +149,11 @@ typedef struct petstrip_stats {
   int64_t horner_walks;      /* 1: the binned iterations' denominators by
                                 Horner's rule (options pass1_walk), 0: by
                                 the chain                                 */
+  int64_t n_items_two_per_cu; /* unbanded binned kernel: work items whose
+                                 windows (32-bit sums, the FP32 iteration)
+                                 fit the LDS of two workgroups per CU; the
+                                 events bin by box width as well when the
+                                 widest window does not                  */
 } petstrip_stats;
 
 /* Called after an output is ready.  iteration >= 1 for images, -1 for the

@@ -1232,6 +1232,14 @@ int petstrip_stats_get(petstrip_handle* h, petstrip_stats* out) {
                                                   (int64_t)sizeof(petstrip::RowEntry))
                                       : 0;
   out->horner_walks = h->fused_iter && h->opts.pass1_walk == 0 ? 1 : 0;
+  out->n_items_two_per_cu = 0;
+  if (h->fused_iter && !h->banded) {
+    const petstrip::ItemRanges& r = h->ranges;
+    for (int i = 0; i < r.n; ++i)
+      if (petstrip::fused_lds_bytes(r.max_ww[i], h->col_stride, petstrip::MODE_MLEM_K) <=
+          petstrip::kTwoPerCuLds)
+        out->n_items_two_per_cu += r.end[i] - r.begin[i];
+  }
   return PETSTRIP_OK;
 }
 

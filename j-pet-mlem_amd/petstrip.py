@@ -70,7 +70,7 @@ class CStats(C.Structure):
         ("last_kernel_ms", C.c_double), ("fused", C.c_int32),
         ("n_work_items", C.c_int32), ("lds_bytes", C.c_int64),
         ("n_window_pixels", C.c_int64), ("row_table_bytes", C.c_int64),
-        ("horner_walks", C.c_int64),
+        ("horner_walks", C.c_int64), ("n_items_two_per_cu", C.c_int64),
     ]
 
 

@@ -68,6 +68,31 @@ def test_512px_prefix_10_iterations(gpu_pkg, oracle):
         assert e < tol
 
 
+def test_512px_width_classes(gpu_pkg):
+    """At 512 x 512 pixels of 1 mm the widest windows (the boxes of steep
+    events, ~95 columns) need more LDS than two workgroups per CU leave, so
+    the narrower boxes bin apart: some work items, not all, fit two per CU,
+    and the iteration over both kinds of launch matches the FP32 oracle."""
+    pkg = gpu_pkg
+    sc = pkg.workload_scanner(W512)
+    ev = pkg.workload_events(W512, n=200_000)
+    with pkg.Reconstruction(sc) as rec:
+        rec.set_events(ev)
+        st = rec.stats()
+        print(f"case: {W512} first 2e5 events: {st['n_items_two_per_cu']} of "
+              f"{st['n_work_items']} items two per CU", flush=True)
+        assert 0 < st["n_items_two_per_cu"] < st["n_work_items"]
+        got = []
+        for _ in range(2):
+            rec(1)
+            got.append(rec.rho)
+    ref = ol.mlem_f32(sc, ev, 2, n_threads=THREADS)
+    for i in range(2):
+        e = rel_l2(got[i], ref[i])
+        print(f"  iteration {i + 1}: rel L2 {e:.2e}", flush=True)
+        assert e < 1e-5
+
+
 def _sum_invariant(img, rec, n):
     st = rec.stats()
     n_pos = n - st["n_zero_denominator"]
