@@ -723,16 +723,16 @@ __device__ __forceinline__ RowEntry load_entry(const RowEntry* p) {
 // that short events do not leave most lanes idle; the denominator is then a
 // segment sum and every per-event value lives per lane.
 //
-// LEAN: the unbanded walks under the default options, fixed at compile time:
-// rows from the row table with its kernel values, pass 1 by Horner's rule
-// where the mode has it.  Without the other options' code the kernel keeps
-// its uniform values in SGPRs instead of spilling them to VGPR lanes (the
-// 64-row-stride kernels have 78 SGPRs at 8 waves per SIMD), and the 128-row
-// stride kernel of the iteration drops from 95 to 54 VGPRs.
+// LEAN: the walks under the default options, fixed at compile time: pass 1
+// by Horner's rule where the mode has it, rows from their roots (no scan)
+// and, unbanded, from the row table with its kernel values.  Without the
+// other options' code the kernel keeps its uniform values in SGPRs instead
+// of spilling them to VGPR lanes (the 64-row-stride kernels have 78 SGPRs at
+// 8 waves per SIMD), and the 128-row stride kernel of the iteration drops
+// from 95 to 54 VGPRs.
 template <int MODE, bool COMP, int HP, int BAND, int PACK, bool LEAN>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(PACK > 1 || HP == 64 ? 8 : 1)))
 void k_strip_fused(FusedArgs a) {
-  static_assert(!LEAN || BAND == 0, "the row table serves the unbanded walks");
   static_assert(PACK == 1 || (BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
                                             MODE == MODE_MLEM_KA || MODE == MODE_GRAD)),
                 "packed events: the unbanded iteration and gradient");
@@ -862,13 +862,13 @@ void k_strip_fused(FusedArgs a) {
                                   : a.inv_fixed_scale;
 
   const float s_pix = a.s_pix;  // sqrt(isz log2 e)
-  const bool scan = a.row_scan != 0;
+  const bool scan = !LEAN && a.row_scan != 0;
   // The row table (unbanded walks): a lane's entry for row group gg of
   // sorted event e is e * stride + gg * 64 + its lane (its row in the event
   // when packed), in the order the walks visit the rows.  The entries of
   // groups 0 and 1 (when the stride has them) of the wave's next events load
   // while the current ones are walked.
-  const bool use_tab = LEAN || (BAND == 0 && a.rowtab != nullptr);
+  const bool use_tab = BAND == 0 && (LEAN || a.rowtab != nullptr);
   const bool tab_kernel = LEAN || a.rowtab_kernel != 0;
   constexpr int kPre = PACK == 1 && kCached > 1 ? 2 : 1;
   const bool pre1 = kPre > 1 && a.rowtab_stride > 64;
@@ -1346,12 +1346,12 @@ template <int MODE, bool COMP, int HP, int BAND, int PACK>
 hipError_t launch_range(hipStream_t st, const FusedArgs& a, int i_begin, int i_end,
                         size_t lds) {
   if (i_end <= i_begin) return hipSuccess;
-  // the default options' form (LEAN) when the launch has them
+  // the default options' form (LEAN) when the launch has them (the row
+  // table's rows were formed under the launch's row_scan)
   auto kern = k_strip_fused<MODE, COMP, HP, BAND, PACK, false>;
-  if constexpr (BAND == 0) {
-    if (a.rowtab != nullptr && a.rowtab_kernel != 0 && a.horner != 0)
-      kern = k_strip_fused<MODE, COMP, HP, BAND, PACK, true>;
-  }
+  const bool lean = a.horner != 0 && (BAND == 0 ? a.rowtab != nullptr && a.rowtab_kernel != 0
+                                                : a.row_scan == 0);
+  if (lean) kern = k_strip_fused<MODE, COMP, HP, BAND, PACK, true>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
