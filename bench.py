@@ -361,9 +361,9 @@ def parse():
 
 
 def measured_traffic(workload):
-    """HBM bytes per launch of the fused kernel from the committed
-    rocprofv3 --pmc summary (profiles/traffic.json), if it covers this
-    workload."""
+    """HBM bytes per MLEM iteration of the fused kernel (its launches: one
+    per box height and window width class) from the committed rocprofv3
+    --pmc summary (profiles/traffic.json), if it covers this workload."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -371,7 +371,7 @@ def measured_traffic(workload):
     except (OSError, ValueError):
         return None
     entry = t.get(workload)
-    return None if entry is None else entry.get("bytes_per_launch")
+    return None if entry is None else entry.get("bytes_per_iteration")
 
 
 def _errors(img, ref):
