@@ -729,9 +729,15 @@ __device__ __forceinline__ RowEntry load_entry(const RowEntry* p) {
 // other options' code the kernel keeps its uniform values in SGPRs instead
 // of spilling them to VGPR lanes (the 64-row-stride kernels have 78 SGPRs at
 // 8 waves per SIMD), and the 128-row stride kernel of the iteration drops
-// from 95 to 54 VGPRs.
+// from 95 to 54 VGPRs.  The 128-row LEAN kernels of the 32-bit-window
+// iterations are compiled for 8 waves per SIMD (at most 64 VGPRs: the
+// analytic one takes 61 instead of 67), so that two workgroups share a CU
+// wherever the windows leave the LDS for it.
 template <int MODE, bool COMP, int HP, int BAND, int PACK, bool LEAN>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(PACK > 1 || HP == 64 ? 8 : 1)))
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(
+    PACK > 1 || HP == 64 || (LEAN && HP == 128 && (MODE == MODE_MLEM_K || MODE == MODE_MLEM_KA))
+        ? 8
+        : 1)))
 void k_strip_fused(FusedArgs a) {
   static_assert(PACK == 1 || (BAND == 0 && (MODE == MODE_MLEM || MODE == MODE_MLEM_K ||
                                             MODE == MODE_MLEM_KA || MODE == MODE_GRAD)),
