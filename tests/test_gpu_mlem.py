@@ -76,6 +76,24 @@ def test_row_scan_identical_image(gpu_pkg):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
+def test_row_scan_identical_image_banded(gpu_pkg):
+    """The same bit identity in the banded form, where the default options
+    run kernels compiled for them (rows from their roots) and row_scan runs
+    the general kernels: the deterministic image and the denominators."""
+    sc, ev = small(gpu_pkg, n_pix=200, size=0.002, n=20000)
+    outs = []
+    for row_scan in (False, True):
+        with gpu_pkg.Reconstruction(sc, accumulation=gpu_pkg.ACCUM_FIXED64,
+                                    row_scan=row_scan, force_bands=1) as rec:
+            rec.set_events(ev)
+            assert rec.stats()["fused"] == 2
+            rec(2)
+            outs.append((rec.rho, rec.forward()))
+    print(f"case: 200x200 @ 0.002 m, banded, {len(ev)} events", flush=True)
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+
+
 def test_denominators(gpu_pkg):
     sc, ev = small(gpu_pkg, n=20000)
     rng = np.random.default_rng(5)
